@@ -1,0 +1,288 @@
+/*
+ * nbvh.h -- C ABI of the MI355X-native N-BVH neural ray query (libnbvh_hip.so).
+ *
+ * The library replaces the inference side of the reference's Neural BVH Module:
+ *   load an N-BVH and network weights, submit a ray batch, get hit records.
+ * Every entry point returns an nbvh_status (0 = NBVH_OK); no exception crosses the ABI and
+ * nothing calls exit(). The message of the last failure on a context (or the last failure of a
+ * context-free call on this thread) is returned by nbvh_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   class NeuralBVHModule                      include/cuda/neural/neural_bvh_module.h:144-241
+ *   NeuralPathModule::neural_bvh_traversal      src/cuda/neural/path/path_module.cu:54-137
+ *   NeuralBVHRenderer::load_neural_bvh          src/cuda/neural/neural_bvh_renderer_accel.cpp:208-254
+ *   NeuralBVHRenderer::save_neural_bvh          src/cuda/neural/neural_bvh_renderer_accel.cpp:170-206
+ *   NeuralBVHRenderer::load_network_model       src/cuda/neural/neural_bvh_renderer_init.cu:394-404
+ *   tcnn::Network::inference (9 x n -> 8 x n)   call site src/cuda/neural/path/path_module.cu:76-92
+ *   CudaBVH2::intersect<ShadowRay>              include/cuda/accel/bvh2.cuh:13-88
+ *   CudaBackend::build_blas (BVH2, leaf <= 2)   src/cuda/cuda_backend.cpp:186-310
+ *   traverse_and_build_neural_bvh               src/cuda/neural/neural_bvh_renderer_accel.cpp:25-88
+ *   camera_raygen + get_lcg_rng jitter          include/cuda/neural/neural_bvh_renderer_kernels.cuh:74-79,
+ *                                               include/cuda/neural/neural_bvh_module_common.cuh:230-236
+ *   CudaBackend::update_camera                  src/cuda/cuda_backend.cpp:595-611
+ */
+#ifndef NBVH_H_
+#define NBVH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBVH_ABI_VERSION 1u
+
+typedef enum nbvh_status {
+    NBVH_OK                  = 0,
+    NBVH_ERR_INVALID_ARGUMENT = 1, /* null pointer, size out of range, bad enum */
+    NBVH_ERR_HIP             = 2,  /* a HIP runtime call failed */
+    NBVH_ERR_NO_DEVICE       = 3,  /* no usable GPU */
+    NBVH_ERR_OUT_OF_MEMORY   = 4,  /* device allocation failed or HBM budget exceeded */
+    NBVH_ERR_FORMAT          = 5,  /* corrupt or truncated .nbvh / parameter blob */
+    NBVH_ERR_STATE           = 6,  /* e.g. neural query on a scene without N-BVH or weights */
+    NBVH_ERR_UNSUPPORTED     = 7,  /* configuration the kernels do not implement */
+    NBVH_ERR_TIMEOUT         = 8,  /* nbvh_request_wait timed out */
+    NBVH_ERR_CAPACITY        = 9   /* caller buffer too small; required size returned */
+} nbvh_status;
+
+/* ---------------------------------------------------------------------------------------------
+ * Plain data types (identical layouts on host and device).
+ * ------------------------------------------------------------------------------------------- */
+
+/* 32-byte BVH node, identical to the reference's BVH2Node / NeuralBVH2Node
+ * (include/base/accel/bvh.h:66-73, include/cuda/neural/accel/neural_bvh2.cuh:33).
+ * Interior: prim_count == 0, children at left_or_prim_offset and left_or_prim_offset + 1.
+ * Triangle-BVH leaf: prim_count primitives starting at left_or_prim_offset in the index list.
+ * N-BVH leaf: prim_count in [1, 8] (1 = LEAF_FINAL, 2 + l = LEAF_LOD(l)) or
+ *             left_or_prim_offset == NBVH_NO_NEURAL_CHILD. */
+typedef struct nbvh_node {
+    float    aabb_min[3];
+    uint32_t prim_count;
+    float    aabb_max[3];
+    uint32_t left_or_prim_offset;
+} nbvh_node;
+
+#define NBVH_NO_NEURAL_CHILD 0xFFFFFFFFu
+#define NBVH_LEAF_FINAL      1u
+#define NBVH_LEAF_LOD(l)     (2u + (l))
+#define NBVH_NUM_LODS        7u
+
+/* Ray, identical to the reference's Ray3f (include/cuda/base/ray.cuh:65-79). */
+typedef struct nbvh_ray {
+    float o[3];
+    float tmin; /* triangle queries: closest/any hit in [tmin, tmax); neural queries ignore it */
+    float d[3];
+    float tmax; /* neural queries: leaves entered at or beyond tmax are skipped, depths must be < tmax */
+} nbvh_ray;
+
+enum nbvh_hit_kind { NBVH_HIT_NONE = 0, NBVH_HIT_NEURAL = 1, NBVH_HIT_TRIANGLE = 2 };
+
+enum nbvh_hit_flag {
+    NBVH_FLAG_STACK_OVERFLOW = 1u, /* traversal stack exceeded; the ray stopped (reference: silent miss) */
+    NBVH_FLAG_STEP_CAP       = 2u  /* max_traversal_steps reached; the ray stopped */
+};
+
+/* 48-byte hit record: the reference's NeuralTraversalResult (path_module.h:19-42) merged with
+ * PreliminaryItsData (ray.cuh:22-33), the way HybridIntersectionResult (path_module.h:46-129)
+ * overlays them (normal_or_uv, albedo_or_mesh_prim_id). */
+typedef struct nbvh_hit {
+    float    t;          /* hit distance, FLT_MAX when nothing was hit */
+    uint32_t prim_id;    /* TRIANGLE: triangle index; else 0xFFFFFFFF */
+    uint32_t node_id;    /* NEURAL: N-BVH leaf of the accepted segment; else 0xFFFFFFFF */
+    uint32_t info;       /* kind (bits 0-7) | flags (bits 8-15) | leaf segments evaluated (16-31, saturating) */
+    float    normal_or_uv[3];           /* NEURAL: normalize(2*out[2:5]-1); TRIANGLE: (u, v, 0) */
+    float    logit;                     /* NEURAL: visibility logit of the accepted segment */
+    float    albedo[3];                 /* NEURAL: out[5:8]; TRIANGLE: 0 */
+    float    reserved;
+} nbvh_hit;
+
+#define NBVH_HIT_KIND(info)     ((info) & 0xFFu)
+#define NBVH_HIT_FLAGS(info)    (((info) >> 8) & 0xFFu)
+#define NBVH_HIT_SEGMENTS(info) ((info) >> 16)
+
+/* Multiresolution hash grid (tcnn "HashGrid", CoherentPrime hash, linear interpolation),
+ * shared by every segment point (RepeatedComposite, Concatenation):
+ * src/cuda/neural/neural_bvh_module.cu:170-191, include/cuda/neural/utils/tcnn_config.h:69-110. */
+typedef struct nbvh_grid_config {
+    uint32_t n_levels;             /* L, <= 32 */
+    uint32_t n_features_per_level; /* F in {1, 2, 4, 8} */
+    uint32_t log2_hashmap_size;    /* log2 T, <= 26 */
+    uint32_t base_resolution;      /* N_min */
+    float    per_level_scale;      /* b */
+} nbvh_grid_config;
+
+enum nbvh_segment_distribution {
+    NBVH_SEGMENT_EQUIDISTANT_INTERIOR = 0, /* t_k = (k + 0.5) / P */
+    NBVH_SEGMENT_EQUIDISTANT_EXTERIOR = 1  /* t_k = k / (P - 1) (every shipped config) */
+};
+
+/* Network: segment -> P points -> hash grid -> MLP (no biases, ReLU hidden, Sigmoid output).
+ * include/cuda/neural/neural_bvh_module.h:79-131 (segment encoding) and
+ * include/cuda/neural/utils/tcnn_config.h:228-264 (FullyFusedMLP). */
+typedef struct nbvh_network_config {
+    nbvh_grid_config grid;
+    uint32_t n_points;             /* P segment points, 1..4 (reference: 3) */
+    uint32_t segment_distribution; /* nbvh_segment_distribution */
+    uint32_t n_neurons;            /* hidden width, 64 */
+    uint32_t n_hidden_layers;      /* 4 (=> 5 matrices) */
+    uint32_t n_outputs;            /* 8: vis, depth, normal3, albedo3 */
+} nbvh_network_config;
+
+/* Arithmetic of the neural query / network inference. */
+enum nbvh_precision {
+    NBVH_PREC_FP32 = 0, /* SIMT FP32, FP32 tables: the plain path */
+    NBVH_PREC_FP16 = 1, /* MFMA FP16 in / FP32 accumulate, FP16 tables */
+    NBVH_PREC_BF16 = 2, /* MFMA BF16 in / FP32 accumulate, FP16 tables */
+    NBVH_PREC_FP8  = 3  /* MFMA FP16 first/last layer, FP8 e4m3 hidden layers with per-layer scales */
+};
+
+enum nbvh_query_kind {
+    NBVH_QUERY_NEURAL_CLOSEST   = 0, /* N-BVH + network, closest decoded depth */
+    NBVH_QUERY_NEURAL_ANY       = 1, /* N-BVH + network, first occluded segment */
+    NBVH_QUERY_TRIANGLE_CLOSEST = 2, /* triangle BVH fallback, closest hit */
+    NBVH_QUERY_TRIANGLE_ANY     = 3  /* triangle BVH fallback, any hit (shadow rays) */
+};
+
+typedef struct nbvh_query_params {
+    uint32_t kind;                /* nbvh_query_kind */
+    uint32_t precision;           /* nbvh_precision (neural kinds) */
+    uint32_t lod;                 /* rendered_lod, 0..6 */
+    uint32_t max_traversal_steps; /* reference m_max_traversal_steps; 0 => 1000 */
+    float    visibility_threshold;/* primary_visibility_threshold on the sigmoid output; 0 => 0.5 */
+    uint32_t reserved[3];
+} nbvh_query_params;
+
+/* Pinhole camera in the reference's CameraParams form (origin, du, dv, dir_top_left). */
+typedef struct nbvh_camera {
+    float origin[3];
+    float du[3];
+    float dv[3];
+    float dir_top_left[3];
+} nbvh_camera;
+
+typedef struct nbvh_ctx   nbvh_ctx;
+typedef struct nbvh_scene nbvh_scene;
+
+typedef struct nbvh_create_info {
+    int32_t  device;          /* HIP device ordinal */
+    uint32_t reserved0;
+    uint64_t hbm_budget_bytes;/* 0 => 90% of free device memory at creation */
+} nbvh_create_info;
+
+/* ---------------------------------------------------------------------------------------------
+ * Library / context.
+ * ------------------------------------------------------------------------------------------- */
+uint32_t    nbvh_abi_version(void);
+const char* nbvh_status_string(int32_t status);
+/* Last failure message of ctx, or of the calling thread's last context-free call if ctx is NULL. */
+const char* nbvh_last_error(const nbvh_ctx* ctx);
+
+int32_t nbvh_create(const nbvh_create_info* info, nbvh_ctx** out_ctx);
+int32_t nbvh_destroy(nbvh_ctx* ctx);
+/* HBM accounting of the context (bytes). */
+int32_t nbvh_memory_usage(const nbvh_ctx* ctx, uint64_t* used_bytes, uint64_t* budget_bytes);
+
+/* ---------------------------------------------------------------------------------------------
+ * Host-side builders (no GPU needed).
+ * ------------------------------------------------------------------------------------------- */
+
+/* Procedural icosphere: 20*4^s triangles, unit radius scaled by `radius`, vertices displaced
+ * radially by amplitude * (3-octave value noise in [-1, 1], seed), centred at center[3].
+ * With verts/indices NULL only the counts are returned. */
+int32_t nbvh_mesh_icosphere(uint32_t subdivisions, uint64_t seed, float amplitude, const float center[3],
+                            float radius, float* verts, uint32_t* n_verts, uint32_t* indices, uint32_t* n_tris);
+
+/* Binned-SAH BVH2 over triangles (max_leaf_size primitives per leaf; reference uses 2,
+ * include/cuda/cuda_backend.h:232). nodes needs 2*n_tris-1 entries, prim_indices n_tris. */
+int32_t nbvh_bvh_build(const float* verts, uint32_t n_verts, const uint32_t* indices, uint32_t n_tris,
+                       uint32_t max_leaf_size, nbvh_node* nodes, uint32_t node_capacity, uint32_t* n_nodes,
+                       uint32_t* prim_indices);
+
+/* Cut a BVH2 to an N-BVH of at most n_leaves leaves (largest surface area split first).
+ * Cut nodes become LEAF_LOD(0) leaves, BVH leaves LEAF_FINAL (reference flags,
+ * neural_bvh_renderer_accel.cpp:47-52). out needs 2*n_leaves-1 entries. */
+int32_t nbvh_nbvh_cut(const nbvh_node* bvh_nodes, uint32_t n_bvh_nodes, uint32_t n_leaves, nbvh_node* out,
+                      uint32_t capacity, uint32_t* n_out);
+
+/* .nbvh (bvh-v2 serialisation of Node<float,3,64,32>): parse / write.
+ * nbvh_nbvh_parse with out == NULL returns the node count only. */
+int32_t nbvh_nbvh_parse(const void* bytes, size_t size, nbvh_node* out, uint32_t capacity, uint32_t* n_nodes);
+int32_t nbvh_nbvh_serialize(const nbvh_node* nodes, uint32_t n_nodes, void* out, size_t capacity, size_t* size);
+
+/* Number of FP32 parameters of the hash tables and of the MLP for a network configuration. */
+int32_t nbvh_network_param_counts(const nbvh_network_config* cfg, uint64_t* n_table_params, uint64_t* n_mlp_params);
+/* Seeded initialisation: tables ~ U(+-table_range), MLP Kaiming-uniform U(+-sqrt(6/fan_in)).
+ * MLP layout: W1 [n_neurons x P*L*F], W2..W_H [n_neurons x n_neurons], W_out [n_outputs x n_neurons],
+ * each row-major (out x in), concatenated. */
+int32_t nbvh_network_init_params(const nbvh_network_config* cfg, uint64_t seed, float table_range, float* tables,
+                                 float* mlp);
+
+/* Pinhole camera from eye/target/up, vertical fov (degrees) and image size (update_camera). */
+int32_t nbvh_camera_look_at(const float eye[3], const float target[3], const float up[3], float fovy_degrees,
+                            uint32_t width, uint32_t height, nbvh_camera* out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Device-resident scenes.
+ * ------------------------------------------------------------------------------------------- */
+
+/* Upload a triangle mesh and build its BVH2 (leaf <= 2). The neural scene bbox defaults to the
+ * BVH root box. */
+int32_t nbvh_scene_create(nbvh_ctx* ctx, const float* verts, uint32_t n_verts, const uint32_t* indices,
+                          uint32_t n_tris, nbvh_scene** out_scene);
+/* Scene without triangles (an N-BVH + weights only, e.g. loaded from a .nbvh). */
+int32_t nbvh_scene_create_empty(nbvh_ctx* ctx, nbvh_scene** out_scene);
+int32_t nbvh_scene_destroy(nbvh_scene* scene);
+
+/* Copy out the triangle BVH (nodes, prim_indices) of the scene. */
+int32_t nbvh_scene_get_bvh(const nbvh_scene* scene, nbvh_node* nodes, uint32_t node_capacity, uint32_t* n_nodes,
+                           uint32_t* prim_indices, uint32_t prim_capacity, uint32_t* n_prims);
+
+/* N-BVH: cut from the triangle BVH, parse a .nbvh blob, or set nodes directly. */
+int32_t nbvh_scene_build_nbvh(nbvh_scene* scene, uint32_t n_leaves);
+int32_t nbvh_scene_load_nbvh(nbvh_scene* scene, const void* bytes, size_t size);
+int32_t nbvh_scene_set_nbvh(nbvh_scene* scene, const nbvh_node* nodes, uint32_t n_nodes);
+int32_t nbvh_scene_get_nbvh(const nbvh_scene* scene, nbvh_node* nodes, uint32_t capacity, uint32_t* n_nodes);
+/* Encoding domain (neural_scene_bbox, neural_bvh_renderer_accel.cpp:553-554): min[3], max[3]. */
+int32_t nbvh_scene_set_bbox(nbvh_scene* scene, const float bbox_min[3], const float bbox_max[3]);
+int32_t nbvh_scene_get_bbox(const nbvh_scene* scene, float bbox_min[3], float bbox_max[3]);
+
+/* Network weights (FP32 master copy; FP16/BF16/FP8 forms are derived on upload). */
+int32_t nbvh_scene_load_network(nbvh_scene* scene, const nbvh_network_config* cfg, const float* tables,
+                                uint64_t n_table_params, const float* mlp, uint64_t n_mlp_params);
+int32_t nbvh_scene_init_network(nbvh_scene* scene, const nbvh_network_config* cfg, uint64_t seed, float table_range);
+int32_t nbvh_scene_get_network(const nbvh_scene* scene, nbvh_network_config* cfg, float* tables,
+                               uint64_t table_capacity, float* mlp, uint64_t mlp_capacity);
+
+/* ---------------------------------------------------------------------------------------------
+ * Queries (device pointers, asynchronous on `stream`, a hipStream_t; NULL = default stream).
+ * ------------------------------------------------------------------------------------------- */
+
+/* Primary rays: pixel i = y*width + x, jitter from the reference's LCG seeded by
+ * (index = sample, frame = frame, linear = i). */
+int32_t nbvh_raygen(nbvh_ctx* ctx, const nbvh_camera* camera, uint32_t width, uint32_t height, uint32_t sample,
+                    uint32_t frame, nbvh_ray* d_rays, void* stream);
+
+/* One query over n rays already in device memory; hit records to d_hits. */
+int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
+                   uint32_t n_rays, nbvh_hit* d_hits, void* stream);
+
+/* Same with host buffers; synchronous. */
+int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
+                        const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);
+
+/* Network inference: inputs [n x 3P] encoded segment points (in[i*3P + 3k + c]),
+ * outputs [n x n_outputs] post-sigmoid (out[i*n_outputs + j]); optional raw pre-sigmoid
+ * outputs (d_logits may be NULL). Replaces tcnn::Network::inference. */
+int32_t nbvh_network_inference(nbvh_ctx* ctx, const nbvh_scene* scene, uint32_t precision, const float* d_inputs,
+                               uint32_t n, float* d_outputs, float* d_logits, void* stream);
+
+/* Host-pointer variant of nbvh_network_inference; synchronous. */
+int32_t nbvh_network_inference_host(nbvh_ctx* ctx, const nbvh_scene* scene, uint32_t precision, const float* inputs,
+                                    uint32_t n, float* outputs, float* logits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBVH_H_ */

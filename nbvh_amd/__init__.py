@@ -1,0 +1,289 @@
+"""nbvh_amd -- MI355X-native N-BVH neural ray query.
+
+Python mirror of the reference's Neural BVH Module interface
+(include/cuda/neural/neural_bvh_module.h:144-241 in WeiPhil/nbvh) over the C ABI of
+libnbvh_hip.so (include/nbvh.h). All compute runs in the HIP library; this layer only moves
+arguments and raises NbvhError on any non-OK status.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, GridConfig, QueryParams, Camera,
+                   PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8, QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY,
+                   QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY, HIT_NONE, HIT_NEURAL, HIT_TRIANGLE,
+                   FLAG_STACK_OVERFLOW, FLAG_STEP_CAP, NO_NEURAL_CHILD, SEGMENT_EQUIDISTANT_EXTERIOR,
+                   SEGMENT_EQUIDISTANT_INTERIOR)
+from ._ffi import check as _check, lib as _lib, ptr as _ptr
+
+__all__ = ["Context", "Scene", "NeuralBVHModule", "NbvhError", "network_config", "icosphere", "bvh_build",
+           "nbvh_cut", "nbvh_parse", "nbvh_serialize", "network_param_counts", "network_init_params",
+           "camera_look_at", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
+
+
+def hit_kind(hits):
+    return hits["info"] & 0xFF
+
+
+def hit_flags(hits):
+    return (hits["info"] >> 8) & 0xFF
+
+
+def hit_segments(hits):
+    return hits["info"] >> 16
+
+
+def network_config(n_levels=16, n_features_per_level=2, log2_hashmap_size=19, base_resolution=16,
+                   per_level_scale=1.5, n_points=3, segment_distribution=SEGMENT_EQUIDISTANT_EXTERIOR,
+                   n_neurons=64, n_hidden_layers=4, n_outputs=8):
+    """nbvh_network_config; defaults are the benchmark network (16 levels x 2 features, 2^19)."""
+    return NetworkConfig(GridConfig(n_levels, n_features_per_level, log2_hashmap_size, base_resolution,
+                                    per_level_scale), n_points, segment_distribution, n_neurons, n_hidden_layers,
+                         n_outputs)
+
+
+# ------------------------------------------------------------------------------------------
+# Host-side builders (no GPU needed)
+# ------------------------------------------------------------------------------------------
+def icosphere(subdivisions, seed=0, amplitude=0.1, center=(0.0, 0.0, 0.0), radius=1.0):
+    """Displaced icosphere: (verts float32 [n,3], tris uint32 [m,3])."""
+    L = _lib()
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    c = np.asarray(center, np.float32)
+    _check(L.nbvh_mesh_icosphere(subdivisions, seed, amplitude, _ptr(c), radius, None, ctypes.byref(nv), None,
+                                 ctypes.byref(nt)))
+    v = np.empty((nv.value, 3), np.float32)
+    t = np.empty((nt.value, 3), np.uint32)
+    _check(L.nbvh_mesh_icosphere(subdivisions, seed, amplitude, _ptr(c), radius, _ptr(v), ctypes.byref(nv), _ptr(t),
+                                 ctypes.byref(nt)))
+    return v, t
+
+
+def bvh_build(verts, tris, max_leaf_size=2):
+    """Binned-SAH BVH2: (nodes NODE_DTYPE, prim_indices uint32)."""
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    nt = len(tris)
+    nodes = np.zeros(max(2 * nt - 1, 1), NODE_DTYPE)
+    prims = np.zeros(nt, np.uint32)
+    n = ctypes.c_uint32()
+    _check(_lib().nbvh_bvh_build(_ptr(verts), len(verts), _ptr(tris), nt, max_leaf_size, _ptr(nodes), len(nodes),
+                                 ctypes.byref(n), _ptr(prims)))
+    return nodes[: n.value].copy(), prims
+
+
+def nbvh_cut(bvh_nodes, n_leaves):
+    bvh_nodes = np.ascontiguousarray(bvh_nodes, NODE_DTYPE)
+    out = np.zeros(2 * n_leaves - 1, NODE_DTYPE)
+    n = ctypes.c_uint32()
+    _check(_lib().nbvh_nbvh_cut(_ptr(bvh_nodes), len(bvh_nodes), n_leaves, _ptr(out), len(out), ctypes.byref(n)))
+    return out[: n.value].copy()
+
+
+def nbvh_parse(data):
+    """Parse .nbvh bytes into N-BVH nodes (NODE_DTYPE)."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    n = ctypes.c_uint32()
+    _check(_lib().nbvh_nbvh_parse(_ptr(buf), len(buf), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, NODE_DTYPE)
+    _check(_lib().nbvh_nbvh_parse(_ptr(buf), len(buf), _ptr(out), len(out), ctypes.byref(n)))
+    return out
+
+
+def nbvh_serialize(nodes):
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    size = ctypes.c_size_t()
+    _check(_lib().nbvh_nbvh_serialize(_ptr(nodes), len(nodes), None, 0, ctypes.byref(size)))
+    out = np.zeros(size.value, np.uint8)
+    _check(_lib().nbvh_nbvh_serialize(_ptr(nodes), len(nodes), _ptr(out), len(out), ctypes.byref(size)))
+    return out.tobytes()
+
+
+def network_param_counts(cfg):
+    nt, nm = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(_lib().nbvh_network_param_counts(ctypes.byref(cfg), ctypes.byref(nt), ctypes.byref(nm)))
+    return nt.value, nm.value
+
+
+def network_init_params(cfg, seed, table_range=1e-4):
+    nt, nm = network_param_counts(cfg)
+    tables = np.empty(nt, np.float32)
+    mlp = np.empty(nm, np.float32)
+    _check(_lib().nbvh_network_init_params(ctypes.byref(cfg), seed, table_range, _ptr(tables), _ptr(mlp)))
+    return tables, mlp
+
+
+def camera_look_at(eye, target, up, fovy_degrees, width, height):
+    cam = Camera()
+    e, t, u = (np.asarray(x, np.float32) for x in (eye, target, up))
+    _check(_lib().nbvh_camera_look_at(_ptr(e), _ptr(t), _ptr(u), fovy_degrees, width, height, ctypes.byref(cam)))
+    return cam
+
+
+def camera_array(cam):
+    """Camera as float32[12] (origin, du, dv, dir_top_left)."""
+    return np.array(list(cam.origin) + list(cam.du) + list(cam.dv) + list(cam.dir_top_left), np.float32)
+
+
+# ------------------------------------------------------------------------------------------
+# Device objects
+# ------------------------------------------------------------------------------------------
+class Context:
+    """nbvh_create / nbvh_destroy: one GPU, an HBM budget, counters for concurrent queries."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        info = _ffi.CreateInfo(device, 0, hbm_budget_bytes)
+        h = ctypes.c_void_p()
+        _check(_lib().nbvh_create(ctypes.byref(info), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _lib().nbvh_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, status):
+        _check(status, self.handle)
+
+    def memory_usage(self):
+        used, budget = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(_lib().nbvh_memory_usage(self.handle, ctypes.byref(used), ctypes.byref(budget)))
+        return used.value, budget.value
+
+    def raygen(self, cam, width, height, d_rays_ptr, sample=0, frame=0, stream=None):
+        """Primary rays into device memory at d_rays_ptr (width*height nbvh_ray records)."""
+        self.check(_lib().nbvh_raygen(self.handle, ctypes.byref(cam), width, height, sample, frame,
+                                      ctypes.c_void_p(d_rays_ptr), ctypes.c_void_p(stream or 0)))
+
+
+class Scene:
+    """A device-resident scene: triangle BVH, N-BVH, encoding box and network weights."""
+
+    def __init__(self, ctx, verts=None, tris=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        if verts is None:
+            ctx.check(_lib().nbvh_scene_create_empty(ctx.handle, ctypes.byref(h)))
+        else:
+            v = np.ascontiguousarray(verts, np.float32)
+            t = np.ascontiguousarray(tris, np.uint32)
+            ctx.check(_lib().nbvh_scene_create(ctx.handle, _ptr(v), len(v), _ptr(t), len(t), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            _lib().nbvh_scene_destroy(self.handle)
+            self.handle = None
+
+    def bvh(self):
+        nn, npr = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx.check(_lib().nbvh_scene_get_bvh(self.handle, None, 0, ctypes.byref(nn), None, 0, ctypes.byref(npr)))
+        nodes = np.zeros(nn.value, NODE_DTYPE)
+        prims = np.zeros(npr.value, np.uint32)
+        self.ctx.check(_lib().nbvh_scene_get_bvh(self.handle, _ptr(nodes), len(nodes), ctypes.byref(nn), _ptr(prims),
+                                                 len(prims), ctypes.byref(npr)))
+        return nodes, prims
+
+    # N-BVH (load_neural_bvh, neural_bvh_renderer_accel.cpp:208)
+    def build_nbvh(self, n_leaves):
+        self.ctx.check(_lib().nbvh_scene_build_nbvh(self.handle, n_leaves))
+
+    def load_neural_bvh(self, path_or_bytes):
+        data = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+        buf = np.frombuffer(bytes(data), np.uint8)
+        self.ctx.check(_lib().nbvh_scene_load_nbvh(self.handle, _ptr(buf), len(buf)))
+
+    def set_nbvh(self, nodes):
+        nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+        self.ctx.check(_lib().nbvh_scene_set_nbvh(self.handle, _ptr(nodes), len(nodes)))
+
+    def nbvh(self):
+        n = ctypes.c_uint32()
+        self.ctx.check(_lib().nbvh_scene_get_nbvh(self.handle, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, NODE_DTYPE)
+        self.ctx.check(_lib().nbvh_scene_get_nbvh(self.handle, _ptr(out), len(out), ctypes.byref(n)))
+        return out
+
+    def set_bbox(self, bmin, bmax):
+        a, b = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+        self.ctx.check(_lib().nbvh_scene_set_bbox(self.handle, _ptr(a), _ptr(b)))
+
+    def bbox(self):
+        a, b = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        self.ctx.check(_lib().nbvh_scene_get_bbox(self.handle, _ptr(a), _ptr(b)))
+        return a, b
+
+    # Network (load_network_model, neural_bvh_renderer_init.cu:394)
+    def load_network_model(self, cfg, tables, mlp):
+        t = np.ascontiguousarray(tables, np.float32)
+        m = np.ascontiguousarray(mlp, np.float32)
+        self.ctx.check(_lib().nbvh_scene_load_network(self.handle, ctypes.byref(cfg), _ptr(t), len(t), _ptr(m),
+                                                      len(m)))
+
+    def init_network(self, cfg, seed, table_range=1e-4):
+        self.ctx.check(_lib().nbvh_scene_init_network(self.handle, ctypes.byref(cfg), seed, table_range))
+
+    def network(self):
+        cfg = NetworkConfig()
+        self.ctx.check(_lib().nbvh_scene_get_network(self.handle, ctypes.byref(cfg), None, 0, None, 0))
+        nt, nm = network_param_counts(cfg)
+        t = np.zeros(nt, np.float32)
+        m = np.zeros(nm, np.float32)
+        self.ctx.check(_lib().nbvh_scene_get_network(self.handle, ctypes.byref(cfg), _ptr(t), nt, _ptr(m), nm))
+        return cfg, t, m
+
+
+class NeuralBVHModule:
+    """Inference side of the reference's NeuralBVHModule / NeuralPathModule.
+
+    neural_bvh_traversal runs the whole per-ray leaf loop of
+    NeuralPathModule::neural_bvh_traversal (src/cuda/neural/path/path_module.cu:54-137) and returns
+    one hit record per ray; network_inference replaces tcnn::Network::inference (path_module.cu:92).
+    """
+
+    def __init__(self, ctx, scene):
+        self.ctx = ctx
+        self.scene = scene
+
+    @staticmethod
+    def params(kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0, max_traversal_steps=0,
+               visibility_threshold=0.0):
+        return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold)
+
+    def neural_bvh_traversal(self, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0,
+                             max_traversal_steps=0, visibility_threshold=0.0):
+        """Host rays (RAY_DTYPE) -> host hits (HIT_DTYPE); synchronous."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        p = self.params(kind, precision, lod, max_traversal_steps, visibility_threshold)
+        self.ctx.check(_lib().nbvh_query_host(self.ctx.handle, self.scene.handle, ctypes.byref(p), _ptr(rays),
+                                              len(rays), _ptr(hits)))
+        return hits
+
+    def query_device(self, params, d_rays_ptr, n, d_hits_ptr, stream=None):
+        """Asynchronous query on device buffers (raw device pointers), on `stream` (hipStream_t)."""
+        self.ctx.check(_lib().nbvh_query(self.ctx.handle, self.scene.handle, ctypes.byref(params),
+                                         ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
+                                         ctypes.c_void_p(stream or 0)))
+
+    def network_inference(self, inputs, precision=PREC_FP32):
+        """inputs float32 [n, 3P] -> (outputs [n, 8] post-sigmoid, logits [n, 8])."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        n = x.shape[0]
+        out = np.zeros((n, 8), np.float32)
+        logits = np.zeros((n, 8), np.float32)
+        self.ctx.check(_lib().nbvh_network_inference_host(self.ctx.handle, self.scene.handle, precision, _ptr(x), n,
+                                                          _ptr(out), _ptr(logits)))
+        return out, logits
+
+    def network_inference_device(self, precision, d_in_ptr, n, d_out_ptr, d_logits_ptr=0, stream=None):
+        self.ctx.check(_lib().nbvh_network_inference(self.ctx.handle, self.scene.handle, precision,
+                                                     ctypes.c_void_p(d_in_ptr), n, ctypes.c_void_p(d_out_ptr),
+                                                     ctypes.c_void_p(d_logits_ptr or 0), ctypes.c_void_p(stream or 0)))

@@ -1,0 +1,152 @@
+"""ctypes binding of libnbvh_hip.so (the C ABI declared in include/nbvh.h).
+
+The library is loaded from this package directory only (it is built in-tree by
+nbvh_amd.build); a missing library raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbvh_hip.so")
+
+# ---- status codes (nbvh_status) ----
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_HIP = 2
+ERR_NO_DEVICE = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_FORMAT = 5
+ERR_STATE = 6
+ERR_UNSUPPORTED = 7
+ERR_TIMEOUT = 8
+ERR_CAPACITY = 9
+
+# ---- enums ----
+PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
+QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
+HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
+FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
+SEGMENT_EQUIDISTANT_INTERIOR, SEGMENT_EQUIDISTANT_EXTERIOR = 0, 1
+NO_NEURAL_CHILD = 0xFFFFFFFF
+
+# ---- numpy record layouts (identical to the C structs) ----
+NODE_DTYPE = np.dtype([("aabb_min", "<f4", 3), ("prim_count", "<u4"), ("aabb_max", "<f4", 3),
+                       ("left_or_prim_offset", "<u4")])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<u4"), ("node_id", "<u4"), ("info", "<u4"),
+                      ("normal_or_uv", "<f4", 3), ("logit", "<f4"), ("albedo", "<f4", 3), ("reserved", "<f4")])
+assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 48
+
+
+class GridConfig(ctypes.Structure):
+    _fields_ = [("n_levels", ctypes.c_uint32), ("n_features_per_level", ctypes.c_uint32),
+                ("log2_hashmap_size", ctypes.c_uint32), ("base_resolution", ctypes.c_uint32),
+                ("per_level_scale", ctypes.c_float)]
+
+
+class NetworkConfig(ctypes.Structure):
+    _fields_ = [("grid", GridConfig), ("n_points", ctypes.c_uint32), ("segment_distribution", ctypes.c_uint32),
+                ("n_neurons", ctypes.c_uint32), ("n_hidden_layers", ctypes.c_uint32),
+                ("n_outputs", ctypes.c_uint32)]
+
+
+class QueryParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("precision", ctypes.c_uint32), ("lod", ctypes.c_uint32),
+                ("max_traversal_steps", ctypes.c_uint32), ("visibility_threshold", ctypes.c_float),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("du", ctypes.c_float * 3), ("dv", ctypes.c_float * 3),
+                ("dir_top_left", ctypes.c_float * 3)]
+
+
+class CreateInfo(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("reserved0", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class NbvhError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"nbvh status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+SZ = ctypes.c_size_t
+F32 = ctypes.c_float
+PU32 = ctypes.POINTER(ctypes.c_uint32)
+PU64 = ctypes.POINTER(ctypes.c_uint64)
+PSZ = ctypes.POINTER(ctypes.c_size_t)
+
+# name -> argtypes (restype int32 unless listed in _RESTYPES)
+SIGNATURES = {
+    "nbvh_abi_version": [],
+    "nbvh_status_string": [ctypes.c_int32],
+    "nbvh_last_error": [P],
+    "nbvh_create": [ctypes.POINTER(CreateInfo), ctypes.POINTER(P)],
+    "nbvh_destroy": [P],
+    "nbvh_memory_usage": [P, PU64, PU64],
+    "nbvh_mesh_icosphere": [U32, U64, F32, P, F32, P, PU32, P, PU32],
+    "nbvh_bvh_build": [P, U32, P, U32, U32, P, U32, PU32, P],
+    "nbvh_nbvh_cut": [P, U32, U32, P, U32, PU32],
+    "nbvh_nbvh_parse": [P, SZ, P, U32, PU32],
+    "nbvh_nbvh_serialize": [P, U32, P, SZ, PSZ],
+    "nbvh_network_param_counts": [ctypes.POINTER(NetworkConfig), PU64, PU64],
+    "nbvh_network_init_params": [ctypes.POINTER(NetworkConfig), U64, F32, P, P],
+    "nbvh_camera_look_at": [P, P, P, F32, U32, U32, ctypes.POINTER(Camera)],
+    "nbvh_scene_create": [P, P, U32, P, U32, ctypes.POINTER(P)],
+    "nbvh_scene_create_empty": [P, ctypes.POINTER(P)],
+    "nbvh_scene_destroy": [P],
+    "nbvh_scene_get_bvh": [P, P, U32, PU32, P, U32, PU32],
+    "nbvh_scene_build_nbvh": [P, U32],
+    "nbvh_scene_load_nbvh": [P, P, SZ],
+    "nbvh_scene_set_nbvh": [P, P, U32],
+    "nbvh_scene_get_nbvh": [P, P, U32, PU32],
+    "nbvh_scene_set_bbox": [P, P, P],
+    "nbvh_scene_get_bbox": [P, P, P],
+    "nbvh_scene_load_network": [P, ctypes.POINTER(NetworkConfig), P, U64, P, U64],
+    "nbvh_scene_init_network": [P, ctypes.POINTER(NetworkConfig), U64, F32],
+    "nbvh_scene_get_network": [P, ctypes.POINTER(NetworkConfig), P, U64, P, U64],
+    "nbvh_raygen": [P, ctypes.POINTER(Camera), U32, U32, U32, U32, P, P],
+    "nbvh_query": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
+    "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
+    "nbvh_network_inference": [P, P, U32, P, U32, P, P, P],
+    "nbvh_network_inference_host": [P, P, U32, P, U32, P, P],
+}
+_RESTYPES = {"nbvh_abi_version": ctypes.c_uint32, "nbvh_status_string": ctypes.c_char_p,
+             "nbvh_last_error": ctypes.c_char_p}
+
+
+def lib():
+    """Load libnbvh_hip.so from the package directory (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not found: build it with `python -m nbvh_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, ctypes.c_int32)
+        _lib = L
+    return _lib
+
+
+def check(status, ctx=None):
+    if status != OK:
+        msg = lib().nbvh_last_error(ctx)
+        raise NbvhError(status, msg.decode() if msg else lib().nbvh_status_string(status).decode())
+
+
+def ptr(a):
+    """ctypes void pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,166 @@
+// wave_query.h -- the persistent wave scheduler shared by every neural query kernel.
+//
+// Reference structure replaced: NeuralPathModule::neural_bvh_traversal
+// (src/cuda/neural/path/path_module.cu:54-137) runs one kernel per traversal step over all active
+// rays, compacts the survivors with atomicAdd into a ping-pong buffer
+// (path_inference_kernels.cu:152-193) and copies the active count to the host after every step.
+// Here one wave owns 64 ray slots for the whole launch. Each iteration it
+//   (A) refills empty slots from a global ray queue (one atomic per wave) and advances every
+//       slot's N-BVH traversal (neural_bvh2.cuh:170-292) until the slot holds a leaf segment,
+//       finishing rays whose stack is exhausted;
+//   (B) evaluates the 64 segments together (encoding + hash grid + MLP, done by the policy);
+//   (C) decodes the outputs per slot (neural_bvh_module_common.cuh:160-209,497-550,
+//       path_inference_kernels.cu:61-131) and keeps the closest depth.
+// The per-ray semantics are those of the reference's step loop: each step evaluates the next
+// accepted leaf in traversal order with current_maxt = closest depth so far; at most
+// max_steps + 1 segments per ray. No host round trip, no compaction traffic.
+#pragma once
+
+#include "common.h"
+
+namespace nbvh {
+namespace dev {
+
+// Traversal stack in LDS, one column per lane (entry e of lane l at [e * 64 + l]).
+struct LdsStack {
+    uint32_t* col;
+    int n;
+    __device__ __forceinline__ void push(uint32_t v)
+    {
+        col[n * 64] = v;
+        n++;
+    }
+    __device__ __forceinline__ uint32_t pop()
+    {
+        n--;
+        return col[n * 64];
+    }
+    __device__ __forceinline__ bool empty() const { return n == 0; }
+    __device__ __forceinline__ int size() const { return n; }
+};
+
+enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DONE = 3 };
+
+struct Slot {
+    uint32_t state;
+    uint32_t ray_id;
+    Ray ray;
+    float best_t;
+    float seg_mint, seg_maxt;
+    uint32_t seg_node;
+    uint32_t segments, flags, kind, hit_node;
+    float hit_logit;
+    float nrm[3], alb[3];
+};
+
+__device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, nbvh_hit* __restrict__ hits)
+{
+    nbvh_hit h;
+    h.t       = (s.kind != 0u && !a.any_hit) ? s.best_t : FLT_MAX;
+    h.prim_id = 0xFFFFFFFFu;
+    h.node_id = s.kind ? s.hit_node : 0xFFFFFFFFu;
+    h.info    = s.kind | (s.flags << 8) | ((s.segments < 65535u ? s.segments : 65535u) << 16);
+    h.normal_or_uv[0] = s.nrm[0];
+    h.normal_or_uv[1] = s.nrm[1];
+    h.normal_or_uv[2] = s.nrm[2];
+    h.logit     = s.kind ? s.hit_logit : 0.f;
+    h.albedo[0] = s.alb[0];
+    h.albedo[1] = s.alb[1];
+    h.albedo[2] = s.alb[2];
+    h.reserved  = 0.f;
+    store_hit(hits, s.ray_id, h);
+    s.state = SLOT_EMPTY;
+}
+
+// Phase A. Returns false when the wave has no segment left to evaluate (all slots DONE).
+__device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& st, const nbvh_ray* __restrict__ rays,
+                                           uint32_t n_rays, nbvh_hit* __restrict__ hits, uint32_t* __restrict__ queue)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    while (true) {
+        if (s.state == SLOT_EMPTY) {
+            const uint64_t m = __ballot(1);
+            const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (r < n_rays) {
+                s.state    = SLOT_RAY;
+                s.ray_id   = r;
+                s.ray      = load_ray(rays, r);
+                s.best_t   = s.ray.tmax;
+                s.segments = 0;
+                s.flags    = 0;
+                s.kind     = 0;
+                s.hit_node = 0xFFFFFFFFu;
+                s.hit_logit = 0.f;
+                s.nrm[0] = s.nrm[1] = s.nrm[2] = 0.f;
+                s.alb[0] = s.alb[1] = s.alb[2] = 0.f;
+                st.n = 0;
+                st.push(0u);
+            } else {
+                s.state = SLOT_DONE;
+            }
+        }
+        if (s.state == SLOT_RAY) {
+            if (st.empty()) {
+                finish_ray(a, s, hits);
+            } else {
+                const int r = nbvh_next_leaf(a.nodes, a.lod, s.ray, st, s.best_t, s.seg_mint, s.seg_maxt, s.seg_node);
+                if (r == 1) {
+                    if (s.segments == a.max_steps + 1u) {
+                        s.flags |= NBVH_FLAG_STEP_CAP;
+                        finish_ray(a, s, hits);
+                    } else {
+                        s.state = SLOT_SEG;
+                    }
+                } else {
+                    if (r < 0) s.flags |= NBVH_FLAG_STACK_OVERFLOW;
+                    finish_ray(a, s, hits);
+                }
+            }
+        }
+        if (__all(s.state == SLOT_SEG || s.state == SLOT_DONE)) break;
+    }
+    return !__all(s.state == SLOT_DONE);
+}
+
+// Phase C: outputs z[0..7] (pre-sigmoid logits) of the slot's current segment.
+__device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const float z[8], nbvh_hit* __restrict__ hits)
+{
+    s.segments++;
+    s.state = SLOT_RAY;
+    if (!(z[0] > a.thr_logit)) return;
+    const bool inside = s.seg_mint < 0.f;
+    if (a.any_hit) {
+        bool h = true;
+        if (inside) h = decode_depth(sigmoidf_cw(z[1]), s.seg_mint, s.seg_maxt) != FLT_MAX;
+        if (h) {
+            s.kind      = NBVH_HIT_NEURAL;
+            s.hit_node  = s.seg_node;
+            s.hit_logit = z[0];
+            finish_ray(a, s, hits);
+        }
+        return;
+    }
+    const float depth = decode_depth(sigmoidf_cw(z[1]), s.seg_mint, s.seg_maxt);
+    if (depth != FLT_MAX && depth < s.best_t) {
+        s.best_t    = depth;
+        s.kind      = NBVH_HIT_NEURAL;
+        s.hit_node  = s.seg_node;
+        s.hit_logit = z[0];
+        float v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c]     = sigmoidf_cw(z[2 + c]) * 2.f - 1.f;
+            s.alb[c] = sigmoidf_cw(z[5 + c]);
+        }
+        const float inv = 1.f / sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s.nrm[c] = v[c] * inv;
+    }
+}
+
+}  // namespace dev
+}  // namespace nbvh

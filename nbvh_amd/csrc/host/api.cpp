@@ -1,0 +1,763 @@
+// api.cpp -- the extern "C" boundary of libnbvh_hip.so (declared in include/nbvh.h).
+//
+// Mirrors the reference's NeuralBVHModule life cycle (include/cuda/neural/neural_bvh_module.h:144-241):
+// scenes own device memory (the reference: tcnn GPUMemoryArena + CUDABuffer), N-BVH and weights
+// are loaded per scene (load_neural_bvh / load_network_model), a query runs the traversal loop
+// of NeuralPathModule::neural_bvh_traversal (src/cuda/neural/path/path_module.cu:54-137) as one
+// persistent kernel. Errors are status codes with a message (the reference throws from
+// CUDA_CHECK and calls exit(2) from CUDA_SYNC_CHECK, include/ntwr.h:30-54).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../kernels/launch.h"
+#include "host_common.h"
+#include "mfma_layout.h"
+
+#define NBVH_API extern "C" __attribute__((visibility("default")))
+
+struct nbvh_ctx {
+    int device = 0;
+    int n_cus  = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};
+    mutable std::mutex mu;
+    std::string last_error;
+    uint32_t* d_counters = nullptr;  // ring of queue counters for concurrent launches
+    std::atomic<uint32_t> counter_next{0};
+    static constexpr uint32_t kCounters = 4096;
+};
+
+namespace nbvh {
+namespace {
+
+thread_local std::string t_last_error;
+
+void set_error(const nbvh_ctx* ctx, const std::string& msg)
+{
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        const_cast<nbvh_ctx*>(ctx)->last_error = msg;
+    }
+    t_last_error = msg;
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(NBVH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int32_t guarded(const nbvh_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return NBVH_OK;
+    } catch (const Error& e) {
+        set_error(ctx, e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return NBVH_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(ctx, e.what());
+        return NBVH_ERR_INVALID_ARGUMENT;
+    }
+}
+
+// Device buffer accounted against the context's HBM budget.
+struct DevBuf {
+    nbvh_ctx* ctx = nullptr;
+    void* ptr     = nullptr;
+    size_t bytes  = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (ptr) {
+            hipSetDevice(ctx->device);
+            hipFree(ptr);
+            ctx->used -= bytes;
+        }
+        ptr   = nullptr;
+        bytes = 0;
+    }
+    void alloc(nbvh_ctx* c, size_t n)
+    {
+        release();
+        ctx = c;
+        if (n == 0) return;
+        if (ctx->used.load() + n > ctx->budget)
+            fail(NBVH_ERR_OUT_OF_MEMORY, "HBM budget exceeded: " + std::to_string(ctx->used.load() + n) + " > " +
+                                             std::to_string(ctx->budget) + " bytes");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (hipMalloc(&ptr, n) != hipSuccess) {
+            ptr = nullptr;
+            (void)hipGetLastError();
+            fail(NBVH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+        ctx->used += n;
+    }
+    template <typename T>
+    void upload(nbvh_ctx* c, const T* src, size_t count)
+    {
+        alloc(c, count * sizeof(T));
+        if (count) hip_check(hipMemcpy(ptr, src, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace
+}  // namespace nbvh
+
+using namespace nbvh;
+
+struct nbvh_scene {
+    nbvh_ctx* ctx = nullptr;
+    // host copies
+    std::vector<float> verts;
+    std::vector<uint32_t> tris;
+    std::vector<nbvh_node> bvh;
+    std::vector<uint32_t> prims;
+    std::vector<nbvh_node> nbvh_nodes;
+    float bbox_min[3] = {0, 0, 0}, bbox_max[3] = {0, 0, 0};
+    bool has_bbox = false, has_net = false;
+    nbvh_network_config cfg{};
+    std::vector<float> tables, mlp;
+    // device
+    DevBuf d_verts, d_tris, d_bvh, d_prims, d_nbvh, d_tables32, d_tables16, d_mlp32, d_frag, d_frag8;
+    NetArgs args{};
+    float table_scale = 1.f;
+    float fp8_wscale[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+};
+
+namespace nbvh {
+namespace {
+
+void require(bool cond, const char* msg)
+{
+    if (!cond) fail(NBVH_ERR_INVALID_ARGUMENT, msg);
+}
+
+// MLP layout consumed by the FP32 kernels: layers 1, 3, 5 row-major, layers 2, 4 transposed.
+std::vector<float> fp32_kernel_layout(const nbvh_network_config& cfg, const std::vector<float>& mlp)
+{
+    const uint32_t K0 = cfg.n_points * cfg.grid.n_levels * cfg.grid.n_features_per_level;
+    std::vector<float> out(mlp.size());
+    size_t off = 0;
+    uint32_t in = K0;
+    for (uint32_t layer = 0; layer <= cfg.n_hidden_layers; layer++) {
+        const uint32_t o = layer == cfg.n_hidden_layers ? cfg.n_outputs : cfg.n_neurons;
+        const bool transposed = (layer % 2 == 1) && layer != cfg.n_hidden_layers;
+        for (uint32_t j = 0; j < o; j++)
+            for (uint32_t k = 0; k < in; k++) {
+                const float w = mlp[off + (size_t)j * in + k];
+                if (transposed)
+                    out[off + (size_t)k * o + j] = w;
+                else
+                    out[off + (size_t)j * in + k] = w;
+            }
+        off += (size_t)o * in;
+        in = cfg.n_neurons;
+    }
+    return out;
+}
+
+void refresh_args(nbvh_scene* s)
+{
+    NetArgs& a = s->args;
+    a = NetArgs{};
+    a.nodes   = s->d_nbvh.as<nbvh_node>();
+    a.n_nodes = (uint32_t)s->nbvh_nodes.size();
+    const float dil = (1.f + kNodeExtentDilation) + kFixedNodeDilation;
+    for (int c = 0; c < 3; c++) {
+        a.enc_center[c] = (s->bbox_max[c] + s->bbox_min[c]) * 0.5f;
+        a.enc_extent[c] = (s->bbox_max[c] - s->bbox_min[c]) * dil;
+    }
+    a.thr_logit = 0.f;
+    a.max_steps = kDefaultMaxSteps;
+    if (!s->has_net) return;
+    const auto& cfg = s->cfg;
+    a.n_levels        = cfg.grid.n_levels;
+    a.n_features      = cfg.grid.n_features_per_level;
+    a.n_points        = cfg.n_points;
+    a.n_inputs        = cfg.n_points * cfg.grid.n_levels * cfg.grid.n_features_per_level;
+    a.n_neurons       = cfg.n_neurons;
+    a.n_hidden_layers = cfg.n_hidden_layers;
+    a.n_outputs       = cfg.n_outputs;
+    const uint32_t P  = cfg.n_points;
+    float delta, offset;
+    if (cfg.segment_distribution == NBVH_SEGMENT_EQUIDISTANT_EXTERIOR) {
+        delta  = (P == 1) ? 1.f : 1.f / (float)(P - 1);
+        offset = 0.f;
+    } else {
+        delta  = 1.f / (float)P;
+        offset = 0.5f;
+    }
+    for (uint32_t k = 0; k < 4; k++) a.seg_t[k] = k < P ? delta * ((float)k + offset) : 0.f;
+    const auto lv = grid_levels(cfg.grid);
+    for (size_t l = 0; l < lv.size(); l++) a.level[l] = lv[l];
+    a.tables_f32      = s->d_tables32.as<float>();
+    a.tables_f16      = s->d_tables16.as<uint16_t>();
+    a.table_scale     = s->table_scale;
+    a.inv_table_scale = 1.f / s->table_scale;
+    a.mlp_f32         = s->d_mlp32.as<float>();
+    a.mlp_frag        = s->d_frag.as<uint16_t>();
+    a.mlp_frag_fp8    = s->d_frag8.as<uint8_t>();
+    for (int i = 0; i < 8; i++) a.fp8_wscale[i] = s->fp8_wscale[i];
+}
+
+void upload_network(nbvh_scene* s)
+{
+    nbvh_ctx* ctx = s->ctx;
+    s->d_tables32.upload(ctx, s->tables.data(), s->tables.size());
+    // FP16 tables with a power-of-two scale that brings the largest magnitude to [0.5, 1): the
+    // tiny seeded values (+-1e-4) would otherwise sit next to the FP16 subnormal range. The MLP
+    // is positively homogeneous (no bias, ReLU), so the scale is undone exactly on the logits.
+    float amax = 0.f;
+    for (float v : s->tables) amax = std::max(amax, std::fabs(v));
+    int e = 0;
+    if (amax > 0.f && std::isfinite(amax)) {
+        std::frexp(amax, &e);  // amax = m * 2^e, m in [0.5, 1)
+        e = -e;
+        e = std::max(-24, std::min(24, e));
+    }
+    s->table_scale = std::ldexp(1.f, e);
+    std::vector<uint16_t> t16(s->tables.size());
+    for (size_t i = 0; i < t16.size(); i++) t16[i] = float_to_half(s->tables[i] * s->table_scale);
+    s->d_tables16.upload(ctx, t16.data(), t16.size());
+    const auto k32 = fp32_kernel_layout(s->cfg, s->mlp);
+    s->d_mlp32.upload(ctx, k32.data(), k32.size());
+    MfmaWeights mw = mfma_weight_layout(s->cfg, s->mlp);
+    s->d_frag.upload(ctx, mw.frag16.data(), mw.frag16.size());
+    s->d_frag8.upload(ctx, mw.frag8.data(), mw.frag8.size());
+    for (int i = 0; i < 8; i++) s->fp8_wscale[i] = mw.fp8_wscale[i];
+}
+
+uint32_t counter_slot(nbvh_ctx* ctx) { return ctx->counter_next.fetch_add(1) % nbvh_ctx::kCounters; }
+
+void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays, uint32_t n,
+               nbvh_hit* d_hits, hipStream_t stream)
+{
+    require(p != nullptr, "query params are NULL");
+    if (n == 0) return;
+    require(d_rays && d_hits, "ray or hit buffer is NULL");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (p->kind == NBVH_QUERY_TRIANGLE_CLOSEST || p->kind == NBVH_QUERY_TRIANGLE_ANY) {
+        if (s->bvh.empty()) fail(NBVH_ERR_STATE, "triangle query on a scene without triangles");
+        TriArgs t{s->d_verts.as<float>(), s->d_tris.as<uint32_t>(), s->d_bvh.as<nbvh_node>(), s->d_prims.as<uint32_t>(),
+                  p->kind == NBVH_QUERY_TRIANGLE_ANY ? 1u : 0u};
+        hip_check(query_triangles(t, d_rays, n, d_hits, stream), "triangle query launch");
+        return;
+    }
+    require(p->kind <= NBVH_QUERY_TRIANGLE_ANY, "unknown query kind");
+    if (s->nbvh_nodes.empty()) fail(NBVH_ERR_STATE, "neural query on a scene without an N-BVH");
+    if (!s->has_net) fail(NBVH_ERR_STATE, "neural query on a scene without network weights");
+    require(p->lod < NBVH_NUM_LODS, "lod must be 0..6");
+    require(p->precision <= NBVH_PREC_FP8, "unknown precision");
+    NetArgs a = s->args;
+    a.lod       = p->lod;
+    a.max_steps = p->max_traversal_steps ? p->max_traversal_steps : kDefaultMaxSteps;
+    a.any_hit   = p->kind == NBVH_QUERY_NEURAL_ANY ? 1u : 0u;
+    const float thr = p->visibility_threshold == 0.f ? 0.5f : p->visibility_threshold;
+    require(thr > 0.f && thr < 1.f, "visibility_threshold must be in (0, 1)");
+    a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
+    uint32_t* counter = ctx->d_counters + counter_slot(ctx);
+    hip_check(zero_u32(counter, 1, stream), "counter reset");
+    bool supported = false;
+    if (p->precision == NBVH_PREC_FP32)
+        hip_check(query_fp32(a, d_rays, n, d_hits, counter, (uint32_t)ctx->n_cus, stream, &supported), "fp32 query");
+    else
+        hip_check(query_mfma(a, p->precision, d_rays, n, d_hits, counter, (uint32_t)ctx->n_cus, stream, &supported),
+                  "mfma query");
+    if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no query kernel for this network shape / precision");
+}
+
+void run_network(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision, const float* d_in, uint32_t n, float* d_out,
+                 float* d_logits, hipStream_t stream)
+{
+    if (n == 0) return;
+    require(d_in != nullptr, "input buffer is NULL");
+    if (!s->has_net) fail(NBVH_ERR_STATE, "network inference on a scene without weights");
+    require(precision <= NBVH_PREC_FP8, "unknown precision");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    bool supported = false;
+    if (precision == NBVH_PREC_FP32)
+        hip_check(network_fp32(s->args, d_in, n, d_out, d_logits, stream, &supported), "fp32 network");
+    else
+        hip_check(network_mfma(s->args, precision, d_in, n, d_out, d_logits, stream, &supported), "mfma network");
+    if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no network kernel for this shape / precision");
+}
+
+}  // namespace
+}  // namespace nbvh
+
+// =============================================================================================
+// extern "C"
+// =============================================================================================
+
+NBVH_API uint32_t nbvh_abi_version(void) { return NBVH_ABI_VERSION; }
+
+NBVH_API const char* nbvh_status_string(int32_t s)
+{
+    switch (s) {
+    case NBVH_OK: return "ok";
+    case NBVH_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case NBVH_ERR_HIP: return "HIP error";
+    case NBVH_ERR_NO_DEVICE: return "no device";
+    case NBVH_ERR_OUT_OF_MEMORY: return "out of memory";
+    case NBVH_ERR_FORMAT: return "format error";
+    case NBVH_ERR_STATE: return "invalid state";
+    case NBVH_ERR_UNSUPPORTED: return "unsupported";
+    case NBVH_ERR_TIMEOUT: return "timeout";
+    case NBVH_ERR_CAPACITY: return "capacity";
+    default: return "unknown status";
+    }
+}
+
+NBVH_API const char* nbvh_last_error(const nbvh_ctx* ctx)
+{
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        return ctx->last_error.c_str();
+    }
+    return t_last_error.c_str();
+}
+
+NBVH_API int32_t nbvh_create(const nbvh_create_info* info, nbvh_ctx** out)
+{
+    return guarded(nullptr, [&] {
+        require(out != nullptr, "out_ctx is NULL");
+        *out = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            fail(NBVH_ERR_NO_DEVICE, "no HIP device available");
+        }
+        const int dev = info ? info->device : 0;
+        require(dev >= 0 && dev < count, "device ordinal out of range");
+        hip_check(hipSetDevice(dev), "hipSetDevice");
+        std::unique_ptr<nbvh_ctx> ctx(new nbvh_ctx);
+        ctx->device = dev;
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        ctx->n_cus = prop.multiProcessorCount;
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        ctx->budget = (info && info->hbm_budget_bytes) ? info->hbm_budget_bytes : (uint64_t)(0.9 * (double)free_b);
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, nbvh_ctx::kCounters * sizeof(uint32_t)), "hipMalloc counters");
+        ctx->d_counters = (uint32_t*)p;
+        hip_check(hipMemset(p, 0, nbvh_ctx::kCounters * sizeof(uint32_t)), "hipMemset counters");
+        *out = ctx.release();
+    });
+}
+
+NBVH_API int32_t nbvh_destroy(nbvh_ctx* ctx)
+{
+    if (!ctx) return NBVH_OK;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    if (ctx->d_counters) hipFree(ctx->d_counters);
+    delete ctx;
+    return NBVH_OK;
+}
+
+NBVH_API int32_t nbvh_memory_usage(const nbvh_ctx* ctx, uint64_t* used, uint64_t* budget)
+{
+    return guarded(ctx, [&] {
+        require(ctx != nullptr, "ctx is NULL");
+        if (used) *used = ctx->used.load();
+        if (budget) *budget = ctx->budget;
+    });
+}
+
+// ----- host builders -----
+
+NBVH_API int32_t nbvh_mesh_icosphere(uint32_t subdivisions, uint64_t seed, float amplitude, const float center[3],
+                                     float radius, float* verts, uint32_t* n_verts, uint32_t* indices,
+                                     uint32_t* n_tris)
+{
+    return guarded(nullptr, [&] {
+        require(subdivisions <= 10, "subdivisions must be <= 10");
+        require(n_verts && n_tris, "count outputs are NULL");
+        const uint64_t p4 = 1ull << (2 * subdivisions);
+        const uint64_t nt = 20 * p4, nv = 10 * p4 + 2;
+        *n_verts = (uint32_t)nv;
+        *n_tris  = (uint32_t)nt;
+        if (!verts && !indices) return;
+        require(verts && indices, "verts and indices must both be given");
+        const float zero[3] = {0, 0, 0};
+        std::vector<float> v;
+        std::vector<uint32_t> t;
+        icosphere(subdivisions, seed, amplitude, center ? center : zero, radius, v, t);
+        if (v.size() != nv * 3 || t.size() != nt * 3) fail(NBVH_ERR_INVALID_ARGUMENT, "icosphere size mismatch");
+        std::memcpy(verts, v.data(), v.size() * sizeof(float));
+        std::memcpy(indices, t.data(), t.size() * sizeof(uint32_t));
+    });
+}
+
+NBVH_API int32_t nbvh_bvh_build(const float* verts, uint32_t n_verts, const uint32_t* indices, uint32_t n_tris,
+                                uint32_t max_leaf_size, nbvh_node* nodes, uint32_t node_capacity, uint32_t* n_nodes,
+                                uint32_t* prim_indices)
+{
+    return guarded(nullptr, [&] {
+        require(verts && indices && nodes && n_nodes && prim_indices, "NULL argument");
+        std::vector<nbvh_node> nd;
+        std::vector<uint32_t> pr;
+        bvh_build(verts, n_verts, indices, n_tris, max_leaf_size, nd, pr);
+        *n_nodes = (uint32_t)nd.size();
+        if (nd.size() > node_capacity) fail(NBVH_ERR_CAPACITY, "node capacity too small");
+        std::memcpy(nodes, nd.data(), nd.size() * sizeof(nbvh_node));
+        std::memcpy(prim_indices, pr.data(), pr.size() * sizeof(uint32_t));
+    });
+}
+
+NBVH_API int32_t nbvh_nbvh_cut(const nbvh_node* bvh_nodes, uint32_t n_bvh_nodes, uint32_t n_leaves, nbvh_node* out,
+                               uint32_t capacity, uint32_t* n_out)
+{
+    return guarded(nullptr, [&] {
+        require(bvh_nodes && out && n_out, "NULL argument");
+        std::vector<nbvh_node> nd;
+        nbvh_cut(bvh_nodes, n_bvh_nodes, n_leaves, nd);
+        *n_out = (uint32_t)nd.size();
+        if (nd.size() > capacity) fail(NBVH_ERR_CAPACITY, "output capacity too small");
+        std::memcpy(out, nd.data(), nd.size() * sizeof(nbvh_node));
+    });
+}
+
+NBVH_API int32_t nbvh_nbvh_parse(const void* bytes, size_t size, nbvh_node* out, uint32_t capacity, uint32_t* n_nodes)
+{
+    return guarded(nullptr, [&] {
+        require(bytes && n_nodes, "NULL argument");
+        std::vector<nbvh_node> nd;
+        nbvh_parse(bytes, size, nd);
+        *n_nodes = (uint32_t)nd.size();
+        if (!out) return;
+        if (nd.size() > capacity) fail(NBVH_ERR_CAPACITY, "node capacity too small");
+        std::memcpy(out, nd.data(), nd.size() * sizeof(nbvh_node));
+    });
+}
+
+NBVH_API int32_t nbvh_nbvh_serialize(const nbvh_node* nodes, uint32_t n_nodes, void* out, size_t capacity,
+                                     size_t* size)
+{
+    return guarded(nullptr, [&] {
+        require(nodes && size, "NULL argument");
+        const auto buf = nbvh_serialize(nodes, n_nodes);
+        *size = buf.size();
+        if (!out) return;
+        if (buf.size() > capacity) fail(NBVH_ERR_CAPACITY, "output capacity too small");
+        std::memcpy(out, buf.data(), buf.size());
+    });
+}
+
+NBVH_API int32_t nbvh_network_param_counts(const nbvh_network_config* cfg, uint64_t* n_table, uint64_t* n_mlp)
+{
+    return guarded(nullptr, [&] {
+        require(cfg != nullptr, "cfg is NULL");
+        network_validate(*cfg);
+        if (n_table) *n_table = table_param_count(*cfg);
+        if (n_mlp) *n_mlp = mlp_param_count(*cfg);
+    });
+}
+
+NBVH_API int32_t nbvh_network_init_params(const nbvh_network_config* cfg, uint64_t seed, float table_range,
+                                          float* tables, float* mlp)
+{
+    return guarded(nullptr, [&] {
+        require(cfg && tables && mlp, "NULL argument");
+        network_validate(*cfg);
+        network_init(*cfg, seed, table_range, tables, mlp);
+    });
+}
+
+NBVH_API int32_t nbvh_camera_look_at(const float eye[3], const float target[3], const float up[3], float fovy,
+                                     uint32_t width, uint32_t height, nbvh_camera* out)
+{
+    return guarded(nullptr, [&] {
+        require(eye && target && up && out, "NULL argument");
+        require(width > 0 && height > 0, "image size must be positive");
+        camera_look_at(eye, target, up, fovy, width, height, *out);
+    });
+}
+
+// ----- scenes -----
+
+NBVH_API int32_t nbvh_scene_create(nbvh_ctx* ctx, const float* verts, uint32_t n_verts, const uint32_t* indices,
+                                   uint32_t n_tris, nbvh_scene** out)
+{
+    return guarded(ctx, [&] {
+        require(ctx && out, "NULL argument");
+        require(verts && indices && n_tris > 0, "empty mesh");
+        *out = nullptr;
+        std::unique_ptr<nbvh_scene> s(new nbvh_scene);
+        s->ctx = ctx;
+        s->verts.assign(verts, verts + (size_t)n_verts * 3);
+        s->tris.assign(indices, indices + (size_t)n_tris * 3);
+        bvh_build(verts, n_verts, indices, n_tris, 2, s->bvh, s->prims);
+        for (int c = 0; c < 3; c++) {
+            s->bbox_min[c] = s->bvh[0].aabb_min[c];
+            s->bbox_max[c] = s->bvh[0].aabb_max[c];
+        }
+        s->has_bbox = true;
+        s->d_verts.upload(ctx, s->verts.data(), s->verts.size());
+        s->d_tris.upload(ctx, s->tris.data(), s->tris.size());
+        s->d_bvh.upload(ctx, s->bvh.data(), s->bvh.size());
+        s->d_prims.upload(ctx, s->prims.data(), s->prims.size());
+        refresh_args(s.get());
+        *out = s.release();
+    });
+}
+
+NBVH_API int32_t nbvh_scene_create_empty(nbvh_ctx* ctx, nbvh_scene** out)
+{
+    return guarded(ctx, [&] {
+        require(ctx && out, "NULL argument");
+        std::unique_ptr<nbvh_scene> s(new nbvh_scene);
+        s->ctx = ctx;
+        refresh_args(s.get());
+        *out = s.release();
+    });
+}
+
+NBVH_API int32_t nbvh_scene_destroy(nbvh_scene* s)
+{
+    if (!s) return NBVH_OK;
+    hipSetDevice(s->ctx->device);
+    hipDeviceSynchronize();
+    delete s;
+    return NBVH_OK;
+}
+
+NBVH_API int32_t nbvh_scene_get_bvh(const nbvh_scene* s, nbvh_node* nodes, uint32_t node_capacity, uint32_t* n_nodes,
+                                    uint32_t* prim_indices, uint32_t prim_capacity, uint32_t* n_prims)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s != nullptr, "scene is NULL");
+        if (n_nodes) *n_nodes = (uint32_t)s->bvh.size();
+        if (n_prims) *n_prims = (uint32_t)s->prims.size();
+        if (nodes) {
+            if (s->bvh.size() > node_capacity) fail(NBVH_ERR_CAPACITY, "node capacity too small");
+            std::memcpy(nodes, s->bvh.data(), s->bvh.size() * sizeof(nbvh_node));
+        }
+        if (prim_indices) {
+            if (s->prims.size() > prim_capacity) fail(NBVH_ERR_CAPACITY, "prim capacity too small");
+            std::memcpy(prim_indices, s->prims.data(), s->prims.size() * sizeof(uint32_t));
+        }
+    });
+}
+
+static void set_nbvh(nbvh_scene* s, std::vector<nbvh_node> nodes)
+{
+    nbvh_validate(nodes.data(), (uint32_t)nodes.size());
+    const auto dil = nbvh_dilate(nodes, kFixedNodeDilation);
+    s->d_nbvh.upload(s->ctx, dil.data(), dil.size());
+    s->nbvh_nodes = std::move(nodes);
+    if (!s->has_bbox) {
+        // Without a mesh the encoding domain is the N-BVH root box (the reference uses the neural
+        // mesh's BVH root box, neural_bvh_renderer_accel.cpp:248-249, which equals it for a cut).
+        for (int c = 0; c < 3; c++) {
+            s->bbox_min[c] = s->nbvh_nodes[0].aabb_min[c];
+            s->bbox_max[c] = s->nbvh_nodes[0].aabb_max[c];
+        }
+        s->has_bbox = true;
+    }
+    refresh_args(s);
+}
+
+NBVH_API int32_t nbvh_scene_build_nbvh(nbvh_scene* s, uint32_t n_leaves)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s != nullptr, "scene is NULL");
+        if (s->bvh.empty()) fail(NBVH_ERR_STATE, "scene has no triangle BVH to cut");
+        std::vector<nbvh_node> nd;
+        nbvh_cut(s->bvh.data(), (uint32_t)s->bvh.size(), n_leaves, nd);
+        set_nbvh(s, std::move(nd));
+    });
+}
+
+NBVH_API int32_t nbvh_scene_load_nbvh(nbvh_scene* s, const void* bytes, size_t size)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s != nullptr, "scene is NULL");
+        std::vector<nbvh_node> nd;
+        nbvh_parse(bytes, size, nd);
+        set_nbvh(s, std::move(nd));
+    });
+}
+
+NBVH_API int32_t nbvh_scene_set_nbvh(nbvh_scene* s, const nbvh_node* nodes, uint32_t n_nodes)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && nodes && n_nodes > 0, "NULL argument");
+        set_nbvh(s, std::vector<nbvh_node>(nodes, nodes + n_nodes));
+    });
+}
+
+NBVH_API int32_t nbvh_scene_get_nbvh(const nbvh_scene* s, nbvh_node* nodes, uint32_t capacity, uint32_t* n_nodes)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && n_nodes, "NULL argument");
+        *n_nodes = (uint32_t)s->nbvh_nodes.size();
+        if (!nodes) return;
+        if (s->nbvh_nodes.size() > capacity) fail(NBVH_ERR_CAPACITY, "node capacity too small");
+        std::memcpy(nodes, s->nbvh_nodes.data(), s->nbvh_nodes.size() * sizeof(nbvh_node));
+    });
+}
+
+NBVH_API int32_t nbvh_scene_set_bbox(nbvh_scene* s, const float bmin[3], const float bmax[3])
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && bmin && bmax, "NULL argument");
+        for (int c = 0; c < 3; c++) require(bmin[c] < bmax[c], "bbox must have positive extent");
+        std::memcpy(s->bbox_min, bmin, sizeof(s->bbox_min));
+        std::memcpy(s->bbox_max, bmax, sizeof(s->bbox_max));
+        s->has_bbox = true;
+        refresh_args(s);
+    });
+}
+
+NBVH_API int32_t nbvh_scene_get_bbox(const nbvh_scene* s, float bmin[3], float bmax[3])
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && bmin && bmax, "NULL argument");
+        std::memcpy(bmin, s->bbox_min, sizeof(s->bbox_min));
+        std::memcpy(bmax, s->bbox_max, sizeof(s->bbox_max));
+    });
+}
+
+NBVH_API int32_t nbvh_scene_load_network(nbvh_scene* s, const nbvh_network_config* cfg, const float* tables,
+                                         uint64_t n_table, const float* mlp, uint64_t n_mlp)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && cfg && tables && mlp, "NULL argument");
+        network_validate(*cfg);
+        if (n_table != table_param_count(*cfg)) fail(NBVH_ERR_FORMAT, "table parameter count does not match config");
+        if (n_mlp != mlp_param_count(*cfg)) fail(NBVH_ERR_FORMAT, "MLP parameter count does not match config");
+        for (uint64_t i = 0; i < n_mlp; i++)
+            if (!std::isfinite(mlp[i])) fail(NBVH_ERR_FORMAT, "MLP weights contain NaN/inf");
+        s->cfg = *cfg;
+        s->tables.assign(tables, tables + n_table);
+        s->mlp.assign(mlp, mlp + n_mlp);
+        upload_network(s);
+        s->has_net = true;
+        refresh_args(s);
+    });
+}
+
+NBVH_API int32_t nbvh_scene_init_network(nbvh_scene* s, const nbvh_network_config* cfg, uint64_t seed,
+                                         float table_range)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && cfg, "NULL argument");
+        network_validate(*cfg);
+        std::vector<float> t(table_param_count(*cfg)), m(mlp_param_count(*cfg));
+        network_init(*cfg, seed, table_range, t.data(), m.data());
+        s->cfg = *cfg;
+        s->tables.swap(t);
+        s->mlp.swap(m);
+        upload_network(s);
+        s->has_net = true;
+        refresh_args(s);
+    });
+}
+
+NBVH_API int32_t nbvh_scene_get_network(const nbvh_scene* s, nbvh_network_config* cfg, float* tables,
+                                        uint64_t table_capacity, float* mlp, uint64_t mlp_capacity)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s != nullptr, "scene is NULL");
+        if (!s->has_net) fail(NBVH_ERR_STATE, "scene has no network");
+        if (cfg) *cfg = s->cfg;
+        if (tables) {
+            if (s->tables.size() > table_capacity) fail(NBVH_ERR_CAPACITY, "table capacity too small");
+            std::memcpy(tables, s->tables.data(), s->tables.size() * sizeof(float));
+        }
+        if (mlp) {
+            if (s->mlp.size() > mlp_capacity) fail(NBVH_ERR_CAPACITY, "MLP capacity too small");
+            std::memcpy(mlp, s->mlp.data(), s->mlp.size() * sizeof(float));
+        }
+    });
+}
+
+// ----- queries -----
+
+NBVH_API int32_t nbvh_raygen(nbvh_ctx* ctx, const nbvh_camera* cam, uint32_t w, uint32_t h, uint32_t sample,
+                             uint32_t frame, nbvh_ray* d_rays, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && cam && d_rays, "NULL argument");
+        require((uint64_t)w * h <= 0xFFFFFFFFull, "image too large");
+        if ((uint64_t)w * h == 0) return;
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        hip_check(raygen(*cam, w, h, sample, frame, d_rays, (hipStream_t)stream), "raygen launch");
+    });
+}
+
+NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays,
+                            uint32_t n, nbvh_hit* d_hits, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        run_query(ctx, s, p, d_rays, n, d_hits, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* rays,
+                                 uint32_t n, nbvh_hit* hits)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        if (n == 0) return;
+        require(rays && hits, "NULL buffer");
+        DevBuf dr, dh;
+        dr.upload(ctx, rays, n);
+        dh.alloc(ctx, (size_t)n * sizeof(nbvh_hit));
+        run_query(ctx, s, p, dr.as<nbvh_ray>(), n, dh.as<nbvh_hit>(), nullptr);
+        hip_check(hipMemcpy(hits, dh.ptr, (size_t)n * sizeof(nbvh_hit), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    });
+}
+
+NBVH_API int32_t nbvh_network_inference(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision, const float* d_in,
+                                        uint32_t n, float* d_out, float* d_logits, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        run_network(ctx, s, precision, d_in, n, d_out, d_logits, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_network_inference_host(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision,
+                                             const float* in, uint32_t n, float* out, float* logits)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        if (n == 0) return;
+        require(in != nullptr, "NULL input");
+        if (!s->has_net) fail(NBVH_ERR_STATE, "network inference on a scene without weights");
+        const size_t D = 3 * (size_t)s->cfg.n_points, O = s->cfg.n_outputs;
+        DevBuf di, dout, dlog;
+        di.upload(ctx, in, (size_t)n * D);
+        if (out) dout.alloc(ctx, (size_t)n * O * sizeof(float));
+        if (logits) dlog.alloc(ctx, (size_t)n * O * sizeof(float));
+        run_network(ctx, s, precision, di.as<float>(), n, dout.as<float>(), dlog.as<float>(), nullptr);
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (out) hip_check(hipMemcpy(out, dout.ptr, (size_t)n * O * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+        if (logits) hip_check(hipMemcpy(logits, dlog.ptr, (size_t)n * O * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    });
+}

@@ -1,0 +1,161 @@
+// basic.hip -- primary ray generation, counter reset and the triangle BVH fallback.
+#include <hip/hip_runtime.h>
+
+#include "../device/common.h"
+#include "launch.h"
+
+namespace nbvh {
+namespace dev {
+
+// get_lcg_rng + lcg_randomf (include/cuda/utils/lcg_rng.cuh:12-72).
+__device__ __forceinline__ uint32_t murmur_mix(uint32_t hash, uint32_t k)
+{
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    hash ^= k;
+    return ((hash << 13) | (hash >> 19)) * 5u + 0xe6546b64u;
+}
+__device__ __forceinline__ uint32_t murmur_fmix(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ float lcg_float(uint32_t& state)
+{
+    state = state * 1664525u + 1013904223u;
+    return ldexpf((float)state, -32);
+}
+
+// camera_raygen_and_intersect_neural_bvh's ray (neural_bvh_module_common.cuh:230-236) and
+// camera_raygen (neural_bvh_renderer_kernels.cuh:74-79).
+__global__ void raygen_kernel(nbvh_camera cam, uint32_t w, uint32_t h, uint32_t sample, uint32_t frame,
+                              nbvh_ray* __restrict__ rays)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t px = i % w, py = i / w;
+    uint32_t st = murmur_fmix(murmur_mix(murmur_mix(0u, i), 512u * frame + sample));
+    const float xf = ((float)px + lcg_float(st)) / (float)w;
+    const float yf = ((float)py + lcg_float(st)) / (float)h;
+    const float dx = (xf * cam.du[0] + yf * cam.dv[0]) + cam.dir_top_left[0];
+    const float dy = (xf * cam.du[1] + yf * cam.dv[1]) + cam.dir_top_left[1];
+    const float dz = (xf * cam.du[2] + yf * cam.dv[2]) + cam.dir_top_left[2];
+    const float inv = 1.f / sqrtf((dx * dx + dy * dy) + dz * dz);
+    float4* p = reinterpret_cast<float4*>(rays + i);
+    p[0] = make_float4(cam.origin[0], cam.origin[1], cam.origin[2], 1e-4f);
+    p[1] = make_float4(dx * inv, dy * inv, dz * inv, FLT_MAX);
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// CudaBVH2::intersect<ShadowRay> (include/cuda/accel/bvh2.cuh:13-88): nearer child first, the
+// farther pushed; leaf primitives tested in order, closest hit shrinks tmax; any hit returns.
+__global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, const nbvh_ray* __restrict__ rays, uint32_t n,
+                                                        nbvh_hit* __restrict__ hits)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Ray ray = load_ray(rays, i);
+    const TriSetup s = tri_setup(ray);
+    float tmax = ray.tmax;
+    uint32_t stack[kTriStackSize];
+    int ss = 0;
+    uint32_t ni = 0, flags = 0, prim = 0xFFFFFFFFu;
+    float best_t = FLT_MAX, bu = 0.f, bv = 0.f;
+    bool found = false;
+    while (true) {
+        const Box nd = load_node(a.nodes, ni);
+        if (nd.prim_count > 0u) {
+            for (uint32_t p = nd.left; p < nd.left + nd.prim_count; p++) {
+                const uint32_t tri = a.prims[p];
+                const uint32_t i0 = a.indices[3 * tri], i1 = a.indices[3 * tri + 1], i2 = a.indices[3 * tri + 2];
+                float t, u, v;
+                if (tri_intersect(ray, s, a.verts + 3 * (size_t)i0, a.verts + 3 * (size_t)i1, a.verts + 3 * (size_t)i2,
+                                  tmax, t, u, v)) {
+                    found  = true;
+                    tmax   = t;
+                    best_t = t;
+                    prim   = tri;
+                    bu     = u;
+                    bv     = v;
+                    if (a.any_hit) goto done;
+                }
+            }
+            if (ss == 0) break;
+            ni = stack[--ss];
+            continue;
+        }
+        {
+            uint32_t c1 = nd.left, c2 = nd.left + 1u;
+            float m1 = box_enter(load_node(a.nodes, c1), ray, tmax);
+            float m2 = box_enter(load_node(a.nodes, c2), ray, tmax);
+            if (m1 > m2) {
+                const float tm = m1;
+                m1 = m2;
+                m2 = tm;
+                const uint32_t tc = c1;
+                c1 = c2;
+                c2 = tc;
+            }
+            if (m1 == FLT_MAX) {
+                if (ss == 0) break;
+                ni = stack[--ss];
+            } else {
+                ni = c1;
+                if (m2 != FLT_MAX) {
+                    if (ss == kTriStackSize) {
+                        flags |= NBVH_FLAG_STACK_OVERFLOW;
+                        break;
+                    }
+                    stack[ss++] = c2;
+                }
+            }
+        }
+    }
+done:
+    nbvh_hit h;
+    h.t       = best_t;
+    h.prim_id = prim;
+    h.node_id = 0xFFFFFFFFu;
+    h.info    = (found ? (uint32_t)NBVH_HIT_TRIANGLE : 0u) | (flags << 8);
+    h.normal_or_uv[0] = bu;
+    h.normal_or_uv[1] = bv;
+    h.normal_or_uv[2] = 0.f;
+    h.logit     = 0.f;
+    h.albedo[0] = h.albedo[1] = h.albedo[2] = 0.f;
+    h.reserved  = 0.f;
+    store_hit(hits, i, h);
+}
+
+}  // namespace dev
+
+hipError_t raygen(const nbvh_camera& cam, uint32_t w, uint32_t h, uint32_t sample, uint32_t frame, nbvh_ray* rays,
+                  hipStream_t stream)
+{
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(dev::raygen_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, cam, w, h, sample, frame, rays);
+    return hipGetLastError();
+}
+
+hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::zero_u32_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t query_triangles(const TriArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::tri_query_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, rays, n, hits);
+    return hipGetLastError();
+}
+
+}  // namespace nbvh

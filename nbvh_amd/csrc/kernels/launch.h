@@ -1,0 +1,33 @@
+// launch.h -- host-callable launchers of the HIP kernels (defined in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "../nbvh_internal.h"
+
+namespace nbvh {
+
+// Primary rays (camera_raygen with LCG jitter).
+hipError_t raygen(const nbvh_camera& cam, uint32_t w, uint32_t h, uint32_t sample, uint32_t frame, nbvh_ray* rays,
+                  hipStream_t stream);
+
+// Zero a small device counter array (a kernel, not hipMemsetAsync, so the call can be captured).
+hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
+
+// Triangle BVH closest / any hit.
+hipError_t query_triangles(const TriArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, hipStream_t stream);
+
+// Neural queries (persistent grid sized from n_cus and the kernel's occupancy).
+// *supported = false when no kernel instance matches the network shape.
+hipError_t query_fp32(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, uint32_t* queue,
+                      uint32_t n_cus, hipStream_t stream, bool* supported);
+hipError_t network_fp32(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, hipStream_t stream,
+                        bool* supported);
+
+// MFMA paths: precision = NBVH_PREC_FP16 / BF16 / FP8.
+hipError_t query_mfma(const NetArgs& a, uint32_t precision, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits,
+                      uint32_t* queue, uint32_t n_cus, hipStream_t stream, bool* supported);
+hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
+                        hipStream_t stream, bool* supported);
+
+}  // namespace nbvh

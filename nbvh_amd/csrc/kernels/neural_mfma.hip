@@ -1,0 +1,334 @@
+// neural_mfma.hip -- FP16 / BF16 neural query and network inference on CDNA4 matrix cores.
+//
+// One wave evaluates 64 segments per iteration (the slots of the persistent wave scheduler):
+//   * FP16 hash tables (scaled by a power of two), trilinear interpolation in FP32;
+//   * the MLP as H^T = W * X^T with v_mfma_f32_32x32x16_{f16,bf16}: two 32-neuron row blocks x
+//     two 32-segment column blocks per layer, FP32 accumulation, weights staged once per block
+//     in LDS as ready-made A fragments (one ds_read_b128 per lane and MFMA);
+//   * the accumulators of a layer become the B operand of the next after ReLU + conversion,
+//     in registers (weights are pre-permuted on the host, host/mfma_layout.cpp);
+//   * layer-1 B operands: each lane gathers the features of its own segment, one
+//     v_permlane32_swap per register pair moves the halves into the operand layout.
+#include <hip/hip_runtime.h>
+
+#include "../device/wave_query.h"
+#include "launch.h"
+
+namespace nbvh {
+namespace dev {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <bool BF16>
+struct Frag;
+template <>
+struct Frag<false> {
+    typedef half8_t vec;
+    typedef _Float16 elem;
+    static __device__ __forceinline__ floatx16 mfma(vec a, vec b, floatx16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Frag<true> {
+    typedef bf16x8_t vec;
+    typedef __bf16 elem;
+    static __device__ __forceinline__ floatx16 mfma(vec a, vec b, floatx16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Number of fragments for K0 inputs, 64 hidden, 4 hidden layers, 8 outputs.
+template <int K0>
+struct MlpShape {
+    static constexpr int KS1    = K0 / 16;        // layer-1 k-steps
+    static constexpr int F_L1   = 2 * KS1;        // layer-1 fragments
+    static constexpr int F_HID  = 2 * 4;          // fragments per hidden layer
+    static constexpr int N_FRAG = F_L1 + 3 * F_HID + 4;
+};
+
+// v_permlane32_swap(x, y): lanes 32..63 of x <-> lanes 0..31 of y, on 16-byte fragments.
+template <typename V>
+__device__ __forceinline__ void swap_halves(V& x, V& y)
+{
+    uint4 a = __builtin_bit_cast(uint4, x), b = __builtin_bit_cast(uint4, y);
+    const auto r0 = __builtin_amdgcn_permlane32_swap(a.x, b.x, false, false);
+    const auto r1 = __builtin_amdgcn_permlane32_swap(a.y, b.y, false, false);
+    const auto r2 = __builtin_amdgcn_permlane32_swap(a.z, b.z, false, false);
+    const auto r3 = __builtin_amdgcn_permlane32_swap(a.w, b.w, false, false);
+    a = make_uint4(r0[0], r1[0], r2[0], r3[0]);
+    b = make_uint4(r0[1], r1[1], r2[1], r3[1]);
+    x = __builtin_bit_cast(V, a);
+    y = __builtin_bit_cast(V, b);
+}
+
+// Layer-1 B operands. Every lane computes all K0 features of its own segment (wave-uniform
+// gather code); lo[s] holds features 8s..8s+7, hi[s] features K0/2+8s..K0/2+8s+7. Column block 0
+// (segments 0..31) needs lo from lanes 0..31 and hi of the same segments in lanes 32..63;
+// column block 1 (segments 32..63) needs lo of those segments in lanes 0..31 and hi in lanes
+// 32..63. Swapping lanes 32..63 of lo with lanes 0..31 of hi produces exactly these operands
+// (lo becomes column block 0, hi column block 1), matching the weight order
+// K(s, g, e) = g*K0/2 + 8s + e of host/mfma_layout.cpp.
+template <int P, int L, int F, bool BF16>
+__device__ __forceinline__ void layer1_operands(const NetArgs& a, const float* pts,
+                                                typename Frag<BF16>::vec lo[P * L * F / 16],
+                                                typename Frag<BF16>::vec hi[P * L * F / 16])
+{
+    constexpr int K0 = P * L * F, HALF = K0 / 2;
+    typedef typename Frag<BF16>::elem E;
+    float x[K0];
+    grid_features<P, L, F, uint16_t>(a, a.tables_f16, pts, x);
+#pragma unroll
+    for (int k = 0; k < K0; k++) {
+        if (k < HALF)
+            lo[k / 8][k % 8] = (E)x[k];
+        else
+            hi[(k - HALF) / 8][(k - HALF) % 8] = (E)x[k];
+    }
+#pragma unroll
+    for (int s = 0; s < K0 / 16; s++) swap_halves(lo[s], hi[s]);
+}
+
+// ReLU + conversion of the accumulators acc[rb][cb] into the B operands of the next layer.
+template <bool BF16>
+__device__ __forceinline__ void acc_to_b(const floatx16 acc[2][2], typename Frag<BF16>::vec bh[2][4])
+{
+    typedef typename Frag<BF16>::elem E;
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float v = acc[s / 2][cb][8 * (s % 2) + e];
+                bh[cb][s][e] = (E)(v > 0.f ? v : 0.f);
+            }
+}
+
+// The whole network for the 64 segments of a wave. own[3P]: this lane's segment points.
+// Returns this lane's 8 logits (descaled). wfrag: LDS fragments (16 B per lane each).
+template <int P, int L, int F, bool BF16>
+__device__ __forceinline__ void mfma_network(const NetArgs& a, const uint4* __restrict__ wfrag, const float* own,
+                                             float z[8])
+{
+    constexpr int K0 = P * L * F;
+    typedef MlpShape<K0> S;
+    typedef Frag<BF16> FR;
+    typedef typename FR::vec V;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t g    = lane >> 5;
+    V b1[2][S::KS1];
+    layer1_operands<P, L, F, BF16>(a, own, b1[0], b1[1]);
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int s = 0; s < S::KS1; s++) {
+            const V w = __builtin_bit_cast(V, wfrag[(rb * S::KS1 + s) * 64 + lane]);
+            acc[rb][0] = FR::mfma(w, b1[0][s], acc[rb][0]);
+            acc[rb][1] = FR::mfma(w, b1[1][s], acc[rb][1]);
+        }
+    int fbase = S::F_L1;
+#pragma unroll
+    for (int layer = 0; layer < 3; layer++) {
+        V bh[2][4];
+        acc_to_b<BF16>(acc, bh);
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const V w = __builtin_bit_cast(V, wfrag[(fbase + rb * 4 + s) * 64 + lane]);
+                acc[rb][0] = FR::mfma(w, bh[0][s], acc[rb][0]);
+                acc[rb][1] = FR::mfma(w, bh[1][s], acc[rb][1]);
+            }
+        fbase += S::F_HID;
+    }
+    floatx16 out[2];
+    {
+        V bh[2][4];
+        acc_to_b<BF16>(acc, bh);
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) out[cb][r] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const V w = __builtin_bit_cast(V, wfrag[(fbase + s) * 64 + lane]);
+            out[0] = FR::mfma(w, bh[0][s], out[0]);
+            out[1] = FR::mfma(w, bh[1][s], out[1]);
+        }
+    }
+    // Rows 0..3 of a column block sit in lane group 0, rows 4..7 in lane group 1 (registers 0..3).
+    float send[4], recv[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        send[r] = g ? out[0][r] : out[1][r];
+        recv[r] = __shfl_xor(send[r], 32);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        z[r]     = (g ? recv[r] : out[0][r]) * a.inv_table_scale;
+        z[4 + r] = (g ? out[1][r] : recv[r]) * a.inv_table_scale;
+    }
+}
+
+template <int P, int L, int F, bool BF16>
+__device__ __forceinline__ void stage_weights(const NetArgs& a, uint4* wfrag)
+{
+    constexpr int NF = MlpShape<P * L * F>::N_FRAG;
+    const uint4* src = reinterpret_cast<const uint4*>(a.mlp_frag) + (BF16 ? (size_t)NF * 64 : 0);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)NF * 64; i += blockDim.x) wfrag[i] = src[i];
+    __syncthreads();
+}
+
+constexpr int kWavesPerBlock = 4;
+
+template <int P, int L, int F, bool BF16>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void neural_query_mfma_kernel(NetArgs a,
+                                                                                const nbvh_ray* __restrict__ rays,
+                                                                                uint32_t n_rays,
+                                                                                nbvh_hit* __restrict__ hits,
+                                                                                uint32_t* __restrict__ queue)
+{
+    constexpr int NF = MlpShape<P * L * F>::N_FRAG;
+    __shared__ uint4 wfrag[NF * 64];
+    __shared__ uint32_t stack_lds[kWavesPerBlock * kNeuralStackSize * 64];
+    stage_weights<P, L, F, BF16>(a, wfrag);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    LdsStack st{stack_lds + wave * kNeuralStackSize * 64 + lane, 0};
+    Slot s;
+    s.state = SLOT_EMPTY;
+    while (fill_slots(a, s, st, rays, n_rays, hits, queue)) {
+        float own[3 * P];
+        if (s.state == SLOT_SEG) {
+            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;
+        }
+        float z[8];
+        mfma_network<P, L, F, BF16>(a, wfrag, own, z);
+        if (s.state == SLOT_SEG) consume_segment(a, s, z, hits);
+    }
+}
+
+template <int P, int L, int F, bool BF16>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetArgs a, const float* __restrict__ in,
+                                                                           uint32_t n, float* __restrict__ out,
+                                                                           float* __restrict__ logits)
+{
+    constexpr int NF = MlpShape<P * L * F>::N_FRAG;
+    __shared__ uint4 wfrag[NF * 64];
+    stage_weights<P, L, F, BF16>(a, wfrag);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float own[3 * P];
+#pragma unroll
+    for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
+    float z[8];
+    mfma_network<P, L, F, BF16>(a, wfrag, own, z);
+    if (i < n) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
+            if (logits) logits[(size_t)i * 8 + j] = z[j];
+        }
+    }
+}
+
+}  // namespace dev
+
+namespace {
+
+template <typename K>
+uint32_t blocks_per_cu(K kernel, uint32_t threads)
+{
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, (int)threads, 0) != hipSuccess || b <= 0) {
+        (void)hipGetLastError();
+        b = 1;
+    }
+    return (uint32_t)b;
+}
+
+template <int P, int L, int F, bool BF16>
+hipError_t launch_query(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, uint32_t* queue,
+                        uint32_t n_cus, hipStream_t stream)
+{
+    auto k = dev::neural_query_mfma_kernel<P, L, F, BF16>;
+    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
+    const uint32_t per_block = 64 * dev::kWavesPerBlock;
+    uint32_t blocks = (n + per_block - 1) / per_block;
+    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rays, n, hits, queue);
+    return hipGetLastError();
+}
+
+template <int P, int L, int F, bool BF16>
+hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits,
+                          hipStream_t stream)
+{
+    const uint32_t per_block = 64 * dev::kWavesPerBlock;
+    hipLaunchKernelGGL((dev::network_mfma_kernel<P, L, F, BF16>), dim3((n + per_block - 1) / per_block),
+                       dim3(per_block), 0, stream, a, in, n, out, logits);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+#define NBVH_MFMA_SHAPES(X) X(3, 16, 2) X(3, 8, 4)
+
+hipError_t query_mfma(const NetArgs& a, uint32_t precision, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits,
+                      uint32_t* queue, uint32_t n_cus, hipStream_t stream, bool* supported)
+{
+    *supported = true;
+    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr) {
+#define X(P_, L_, F_)                                                                 \
+    if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
+        if (precision == NBVH_PREC_FP16)                                             \
+            return launch_query<P_, L_, F_, false>(a, rays, n, hits, queue, n_cus, stream); \
+        if (precision == NBVH_PREC_BF16)                                             \
+            return launch_query<P_, L_, F_, true>(a, rays, n, hits, queue, n_cus, stream);  \
+    }
+        NBVH_MFMA_SHAPES(X)
+#undef X
+    }
+    *supported = false;
+    return hipSuccess;
+}
+
+hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
+                        hipStream_t stream, bool* supported)
+{
+    *supported = true;
+    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr) {
+#define X(P_, L_, F_)                                                                 \
+    if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
+        if (precision == NBVH_PREC_FP16)                                             \
+            return launch_network<P_, L_, F_, false>(a, in, n, out, logits, stream);  \
+        if (precision == NBVH_PREC_BF16)                                             \
+            return launch_network<P_, L_, F_, true>(a, in, n, out, logits, stream);   \
+    }
+        NBVH_MFMA_SHAPES(X)
+#undef X
+    }
+    *supported = false;
+    return hipSuccess;
+}
+
+}  // namespace nbvh

@@ -1,0 +1,62 @@
+// nbvh_internal.h -- plain structures shared by the host library (g++) and the HIP kernels (hipcc).
+#pragma once
+
+#include <stdint.h>
+#include "../../include/nbvh.h"
+
+namespace nbvh {
+
+constexpr int      kMaxLevels        = 32;
+constexpr int      kNeuralStackSize  = 24;   // NEURAL_STACK_SIZE, include/cuda/neural/accel/neural_bvh.cuh:22
+constexpr int      kTriStackSize     = 64;   // reference: 32 without overflow check (bvh2.cuh:25)
+constexpr float    kFixedNodeDilation  = 1e-5f;  // neural_bvh_renderer_constants.cuh:30
+constexpr float    kNodeExtentDilation = 1e-4f;  // neural_bvh_renderer_constants.cuh:31
+constexpr float    kInferenceMintEps   = 1e-4f;  // neural_bvh_renderer_constants.cuh:25
+constexpr float    kInferenceMaxtEps   = 1e-3f;  // neural_bvh_renderer_constants.cuh:26
+constexpr float    kDepthEpsilon       = 0.2f;   // neural_bvh_module.h:270
+constexpr float    kFixedDepthEpsilon  = 1e-5f;  // neural_bvh_module.h:271
+constexpr uint32_t kDefaultMaxSteps    = 1000;   // m_max_traversal_steps, neural_bvh_module.h:200
+
+// One level of the multiresolution grid (tcnn GridEncoding, public convention).
+struct GridLevel {
+    float    scale;   // exp2(l*log2(b)) * N_min - 1
+    uint32_t res;     // ceil(scale) + 1
+    uint32_t res2;    // res * res
+    uint32_t size;    // entries in the level (min(next_multiple(res^3, 8), T)); a power of two when hashed
+    uint32_t offset;  // first entry of the level
+    uint32_t dense;   // 1: index = x + res*y + res^2*z; 0: coherent-prime hash
+};
+
+// Everything a neural kernel needs, passed by value (no __constant__ singletons, so several
+// scenes can be resident and queried concurrently).
+struct NetArgs {
+    uint32_t n_levels, n_features, n_points, n_inputs;  // n_inputs = P * L * F
+    uint32_t n_neurons, n_hidden_layers, n_outputs, lod;
+    float    seg_t[4];                 // segment point parameters t_k
+    float    enc_center[3];            // neural scene bbox centre, (max + min) * 0.5
+    float    enc_extent[3];            // bbox extent * (1 + node_extent_dilation + fixed_node_dilation)
+    float    thr_logit;                // visibility threshold on the logit (0 for 0.5)
+    uint32_t max_steps;                // max traversal steps (segments <= max_steps + 1)
+    const nbvh_node* nodes;            // N-BVH nodes with DILATED boxes (1 + 1e-5 about the centre)
+    uint32_t n_nodes;
+    uint32_t any_hit;
+    const float*    tables_f32;        // [entries x F]
+    const uint16_t* tables_f16;        // [entries x F] IEEE half of tables * table_scale
+    float    table_scale;              // power of two
+    float    inv_table_scale;
+    const float*    mlp_f32;           // row-major matrices, concatenated
+    const uint16_t* mlp_frag;          // MFMA fragment-ordered half/bf16 weights
+    const uint8_t*  mlp_frag_fp8;      // FP8 e4m3 hidden-layer fragments
+    float    fp8_wscale[8];            // per-layer weight dequantisation scales (FP8 variant)
+    GridLevel level[kMaxLevels];
+};
+
+struct TriArgs {
+    const float*     verts;    // [nv x 3]
+    const uint32_t*  indices;  // [nt x 3]
+    const nbvh_node* nodes;
+    const uint32_t*  prims;
+    uint32_t         any_hit;
+};
+
+}  // namespace nbvh

@@ -1,0 +1,223 @@
+"""CPU oracle of the N-BVH neural ray query -- TEST INFRASTRUCTURE ONLY.
+
+A restatement of the reference's algorithm (oracle/nbvh_oracle.cpp, citing the reference file
+and line of every function). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg may use it, and only as the checker / CPU baseline; the product library never calls it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+NODE_DTYPE = np.dtype([("aabb_min", "<f4", 3), ("prim_count", "<u4"), ("aabb_max", "<f4", 3),
+                       ("left_or_prim_offset", "<u4")])
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<u4"), ("node_id", "<u4"), ("info", "<u4"),
+                      ("normal_or_uv", "<f4", 3), ("logit", "<f4"), ("albedo", "<f4", 3), ("reserved", "<f4")])
+SEG_DTYPE = np.dtype([("ray", "<u4"), ("node", "<u4"), ("mint", "<f4"), ("maxt", "<f4"), ("logit", "<f4", 8)])
+
+FP32, FP64 = 0, 1
+
+
+class OracleNet(ctypes.Structure):
+    _fields_ = [("n_levels", ctypes.c_uint32), ("n_features", ctypes.c_uint32),
+                ("log2_hashmap_size", ctypes.c_uint32), ("base_resolution", ctypes.c_uint32),
+                ("per_level_scale", ctypes.c_float), ("n_points", ctypes.c_uint32),
+                ("distribution", ctypes.c_uint32), ("n_neurons", ctypes.c_uint32),
+                ("n_hidden_layers", ctypes.c_uint32), ("n_outputs", ctypes.c_uint32),
+                ("tables", ctypes.c_void_p), ("mlp", ctypes.c_void_p),
+                ("bbox_min", ctypes.c_float * 3), ("bbox_max", ctypes.c_float * 3)]
+
+
+def build():
+    """Compile liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float
+        sig = {
+            "oracle_murmur_mix": ([u32, u32], u32), "oracle_murmur_fmix": ([u32], u32),
+            "oracle_lcg_seed": ([u32, u32, u32], u32), "oracle_lcg_next": ([vp], u32),
+            "oracle_lcg_float": ([vp], f32), "oracle_expf": ([f32], f32), "oracle_sigmoidf": ([f32], f32),
+            "oracle_num_threads": ([], ctypes.c_int),
+            "oracle_raygen": ([vp, u32, u32, u32, u32, vp], None),
+            "oracle_slab": ([vp, vp, vp, vp, vp], ctypes.c_int),
+            "oracle_dilate": ([vp, f32, vp, vp], None),
+            "oracle_nbvh_leaf_sequence": ([vp, u32, vp, f32, vp, vp, vp, u32], ctypes.c_int),
+            "oracle_grid_levels": ([vp, vp, vp, vp, vp, vp], u64),
+            "oracle_encode": ([vp, vp, vp, vp, u32, u32, vp], None),
+            "oracle_grid": ([vp, vp, u32, u32, vp], None),
+            "oracle_network": ([vp, vp, u32, u32, vp, vp], None),
+            "oracle_neural_query": ([vp, vp, vp, u32, u32, u32, u32, u32, f32, vp, vp, u64, vp], None),
+            "oracle_tri_query": ([vp, vp, vp, vp, vp, u32, u32, vp], None),
+            "oracle_tri_brute": ([vp, vp, u32, vp, u32, vp], None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Net:
+    """Network + encoding box for the oracle (keeps the arrays alive)."""
+
+    def __init__(self, cfg, tables, mlp, bbox_min, bbox_max):
+        self.tables = np.ascontiguousarray(tables, np.float32)
+        self.mlp = np.ascontiguousarray(mlp, np.float32)
+        g = cfg.grid
+        self.s = OracleNet(g.n_levels, g.n_features_per_level, g.log2_hashmap_size, g.base_resolution,
+                           g.per_level_scale, cfg.n_points, cfg.segment_distribution, cfg.n_neurons,
+                           cfg.n_hidden_layers, cfg.n_outputs, self.tables.ctypes.data, self.mlp.ctypes.data,
+                           (ctypes.c_float * 3)(*bbox_min), (ctypes.c_float * 3)(*bbox_max))
+        self.cfg = cfg
+
+    @property
+    def n_points(self):
+        return self.s.n_points
+
+    def levels(self):
+        L = self.s.n_levels
+        scale = np.zeros(L, np.float32)
+        res, size, off, dense = (np.zeros(L, np.uint32) for _ in range(4))
+        total = lib().oracle_grid_levels(ctypes.byref(self.s), _p(scale), _p(res), _p(size), _p(off), _p(dense))
+        return dict(scale=scale, res=res, size=size, offset=off, dense=dense, n_params=total)
+
+    def encode(self, rays, mint, maxt, mode=FP32):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        mint = np.ascontiguousarray(mint, np.float32)
+        maxt = np.ascontiguousarray(maxt, np.float32)
+        out = np.zeros((len(rays), 3 * self.n_points), np.float32 if mode == FP32 else np.float64)
+        lib().oracle_encode(ctypes.byref(self.s), _p(rays), _p(mint), _p(maxt), len(rays), mode, _p(out))
+        return out
+
+    def grid(self, x, mode=FP32):
+        x = np.ascontiguousarray(x, np.float32 if mode == FP32 else np.float64).reshape(-1, 3)
+        LF = self.s.n_levels * self.s.n_features
+        out = np.zeros((len(x), LF), x.dtype)
+        lib().oracle_grid(ctypes.byref(self.s), _p(x), len(x), mode, _p(out))
+        return out
+
+    def network(self, inputs, mode=FP64):
+        x = np.ascontiguousarray(inputs, np.float32)
+        n = x.shape[0]
+        logits = np.zeros((n, self.s.n_outputs), np.float64)
+        outs = np.zeros((n, self.s.n_outputs), np.float64)
+        lib().oracle_network(ctypes.byref(self.s), _p(x), n, mode, _p(logits), _p(outs))
+        return outs, logits
+
+    def neural_query(self, nodes, rays, any_hit=False, mode=FP32, lod=0, max_steps=1000, threshold=0.5,
+                     log_segments=False):
+        nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        if not log_segments:
+            lib().oracle_neural_query(ctypes.byref(self.s), _p(nodes), _p(rays), len(rays), int(any_hit), mode, lod,
+                                      max_steps, threshold, _p(hits), None, 0, None)
+            return hits
+        n_seg = ctypes.c_uint64()
+        lib().oracle_neural_query(ctypes.byref(self.s), _p(nodes), _p(rays), len(rays), int(any_hit), mode, lod,
+                                  max_steps, threshold, _p(hits), None, 0, ctypes.byref(n_seg))
+        log = np.zeros(n_seg.value, SEG_DTYPE)
+        lib().oracle_neural_query(ctypes.byref(self.s), _p(nodes), _p(rays), len(rays), int(any_hit), mode, lod,
+                                  max_steps, threshold, _p(hits), _p(log), len(log), ctypes.byref(n_seg))
+        return hits, log
+
+
+def raygen(cam12, w, h, sample=0, frame=0):
+    rays = np.zeros(w * h, RAY_DTYPE)
+    c = np.ascontiguousarray(cam12, np.float32)
+    lib().oracle_raygen(_p(c), w, h, sample, frame, _p(rays))
+    return rays
+
+
+def leaf_sequence(nodes, ray, lod=0, current_maxt=np.float32(3.4028234663852886e38), cap=4096):
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    r = np.ascontiguousarray(np.asarray([ray], RAY_DTYPE))
+    on = np.zeros(cap, np.uint32)
+    omn = np.zeros(cap, np.float32)
+    omx = np.zeros(cap, np.float32)
+    n = lib().oracle_nbvh_leaf_sequence(_p(nodes), lod, _p(r), current_maxt, _p(on), _p(omn), _p(omx), cap)
+    if n < 0:
+        return None
+    n = min(n, cap)
+    return on[:n].copy(), omn[:n].copy(), omx[:n].copy()
+
+
+def slab(bmin, bmax, ray):
+    a, b = np.asarray(bmin, np.float32), np.asarray(bmax, np.float32)
+    r = np.ascontiguousarray(np.asarray([ray], RAY_DTYPE))
+    tn, tf = ctypes.c_float(), ctypes.c_float()
+    hit = lib().oracle_slab(_p(a), _p(b), _p(r), ctypes.byref(tn), ctypes.byref(tf))
+    return bool(hit), tn.value, tf.value
+
+
+def dilate(node, dilation=1e-5):
+    nd = np.ascontiguousarray(np.asarray([node], NODE_DTYPE))
+    a, b = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().oracle_dilate(_p(nd), dilation, _p(a), _p(b))
+    return a, b
+
+
+def tri_query(verts, tris, nodes, prims, rays, any_hit=False):
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    prims = np.ascontiguousarray(prims, np.uint32)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(len(rays), HIT_DTYPE)
+    lib().oracle_tri_query(_p(verts), _p(tris), _p(nodes), _p(prims), _p(rays), len(rays), int(any_hit), _p(hits))
+    return hits
+
+
+def tri_brute(verts, tris, rays):
+    verts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(len(rays), HIT_DTYPE)
+    lib().oracle_tri_brute(_p(verts), _p(tris), len(tris), _p(rays), len(rays), _p(hits))
+    return hits
+
+
+def murmur_mix(h, k):
+    return lib().oracle_murmur_mix(h, k)
+
+
+def murmur_fmix(h):
+    return lib().oracle_murmur_fmix(h)
+
+
+def lcg_seed(index, frame, linear):
+    return lib().oracle_lcg_seed(index, frame, linear)
+
+
+def lcg_floats(state, n):
+    s = ctypes.c_uint32(state)
+    return np.array([lib().oracle_lcg_float(ctypes.byref(s)) for _ in range(n)], np.float32)
+
+
+def expf(x):
+    return np.array([lib().oracle_expf(float(v)) for v in np.ravel(x)], np.float32).reshape(np.shape(x))
+
+
+def num_threads():
+    return lib().oracle_num_threads()

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the CPU oracle.
+
+Bars: bit-exact for rays, triangle ids/t, N-BVH leaf sequences and the whole FP32 neural query
+(the FP32 path evaluates the oracle's FP32 operation sequence); FP16/BF16 network outputs within
+stated tolerances of the FP64 oracle, visibility flips <= 0.5 %.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, bench_scene, camera_rays, random_rays
+
+pytestmark = pytest.mark.gpu
+
+CHESS = os.path.join(GOLDEN, "chess_nodes_32940_hash_18_lvl_8.nbvh")
+
+
+def hits_equal(a, b):
+    """Bitwise equality of hit records (NaN-safe)."""
+    return np.array_equal(a.view(np.uint32).reshape(len(a), -1), b.view(np.uint32).reshape(len(b), -1))
+
+
+@pytest.fixture(scope="module")
+def ctx(nb):
+    c = nb.Context()
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def bench(nb, orc, ctx):
+    v, t, nodes, prims, nbn = bench_scene(nb)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    cfg = nb.network_config()  # 16 levels x 2 features, 2^19, base 16, scale 1.5
+    scene.init_network(cfg, seed=1, table_range=1e-4)
+    cfg2, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg2, tables, mlp, bmin, bmax)
+    yield dict(v=v, t=t, nodes=nodes, prims=prims, nbvh=scene.nbvh(), scene=scene, net=net,
+               module=nb.NeuralBVHModule(ctx, scene))
+    scene.close()
+
+
+def test_scene_matches_host_builders(nb, bench):
+    nodes, prims = bench["scene"].bvh()
+    assert np.array_equal(nodes.view(np.uint32), bench["nodes"].view(np.uint32))
+    assert np.array_equal(prims, bench["prims"])
+    assert np.array_equal(bench["nbvh"].view(np.uint32), nb.nbvh_cut(nodes, 256).view(np.uint32))
+
+
+def test_raygen_bit_exact(nb, orc, ctx):
+    import torch
+    w, h = 512, 384
+    cam, expect = camera_rays(nb, orc, w, h, sample=3, frame=2)
+    d = torch.empty(w * h * 8, dtype=torch.float32, device="cuda")
+    ctx.raygen(cam, w, h, d.data_ptr(), sample=3, frame=2)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().view(orc.RAY_DTYPE)
+    assert np.array_equal(got.view(np.uint32), expect.view(np.uint32))
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_triangle_query_bit_exact(nb, orc, bench, any_hit):
+    _, rays = camera_rays(nb, orc, 256, 256)
+    rays = np.concatenate([rays, random_rays(20000, 4, inside_frac=0.3)])
+    kind = nb.QUERY_TRIANGLE_ANY if any_hit else nb.QUERY_TRIANGLE_CLOSEST
+    got = bench["module"].neural_bvh_traversal(rays, kind=kind)
+    exp = orc.tri_query(bench["v"], bench["t"], bench["nodes"], bench["prims"], rays, any_hit=any_hit)
+    if not any_hit:
+        assert hits_equal(got, exp)
+    else:
+        assert np.array_equal(got["info"], exp["info"])
+    assert (nb.hit_kind(got) == nb.HIT_TRIANGLE).mean() > 0.3
+
+
+def test_network_fp32_bit_exact(nb, orc, bench):
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(0, 1, (3000, 9)), rng.uniform(-0.2, 1.2, (500, 9))]).astype(np.float32)
+    out, logits = bench["module"].network_inference(x, precision=nb.PREC_FP32)
+    o32, z32 = bench["net"].network(x, mode=orc.FP32)
+    assert np.array_equal(logits, z32.astype(np.float32))
+    assert np.array_equal(out, o32.astype(np.float32))
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_neural_query_fp32_bit_exact(nb, orc, bench, any_hit):
+    _, rays = camera_rays(nb, orc, 128, 128)
+    rays = np.concatenate([rays, random_rays(6000, 7, inside_frac=0.4)])
+    kind = nb.QUERY_NEURAL_ANY if any_hit else nb.QUERY_NEURAL_CLOSEST
+    got = bench["module"].neural_bvh_traversal(rays, kind=kind, precision=nb.PREC_FP32)
+    exp = bench["net"].neural_query(bench["nbvh"], rays, any_hit=any_hit, mode=orc.FP32)
+    assert (nb.hit_kind(exp) == nb.HIT_NEURAL).mean() > 0.3
+    assert np.array_equal(got["info"], exp["info"])
+    assert hits_equal(got, exp)
+
+
+def test_neural_query_step_cap_bit_exact(nb, orc, bench):
+    rays = random_rays(4000, 8, inside_frac=0.5)
+    got = bench["module"].neural_bvh_traversal(rays, max_traversal_steps=1)
+    exp = bench["net"].neural_query(bench["nbvh"], rays, max_steps=1, mode=orc.FP32)
+    assert hits_equal(got, exp)
+    assert ((nb.hit_flags(got) & nb.FLAG_STEP_CAP) != 0).sum() > 0
+    assert nb.hit_segments(got).max() == 2
+
+
+def test_neural_query_threshold_bit_exact(nb, orc, bench):
+    rays = random_rays(3000, 10, inside_frac=0.2)
+    got = bench["module"].neural_bvh_traversal(rays, visibility_threshold=0.50001)
+    exp = bench["net"].neural_query(bench["nbvh"], rays, threshold=0.50001, mode=orc.FP32)
+    assert hits_equal(got, exp)
+
+
+@pytest.mark.parametrize("lod", [0, 6])
+def test_chess_nbvh_fp32_bit_exact(nb, orc, ctx, lod):
+    cfg = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=18, base_resolution=8,
+                            per_level_scale=2.0)
+    scene = nb.Scene(ctx)
+    scene.load_neural_bvh(CHESS)
+    scene.init_network(cfg, seed=9, table_range=1e-4)
+    _, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    nodes = scene.nbvh()
+    assert len(nodes) == 25249
+    net = orc.Net(cfg, tables, mlp, bmin, bmax)
+    rng = np.random.default_rng(lod)
+    n = 4000
+    rays = np.zeros(n, orc.RAY_DTYPE)
+    o = rng.uniform(bmin - 0.2, bmax + 0.2, (n, 3))
+    tgt = rng.uniform(bmin, bmax, (n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
+    got = nb.NeuralBVHModule(ctx, scene).neural_bvh_traversal(rays, lod=lod)
+    exp = net.neural_query(nodes, rays, lod=lod, mode=orc.FP32)
+    assert hits_equal(got, exp)
+    assert nb.hit_segments(got).mean() > 1.0
+    scene.close()
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_network_mfma_vs_fp64(nb, orc, bench, precision):
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16}[precision]
+    # Segment inputs of real queries (the oracle's segment log), encoded in FP32.
+    _, rays = camera_rays(nb, orc, 96, 96)
+    _, log = bench["net"].neural_query(bench["nbvh"], rays, mode=orc.FP64, log_segments=True)
+    x = bench["net"].encode(rays[log["ray"]], log["mint"], log["maxt"], mode=orc.FP32)
+    out, logits = bench["module"].network_inference(x, precision=prec)
+    o64, z64 = bench["net"].network(x, mode=orc.FP64)
+    scale = np.abs(z64).max(axis=0)
+    err = np.abs(logits - z64) / scale
+    # Relative to each output's range: FP16 keeps ~11 bits, BF16 ~8 bits through 5 layers.
+    tol = {"fp16": 2e-2, "bf16": 8e-2}[precision]
+    assert err.max() < tol, err.max(axis=0)
+    flips = np.mean(np.sign(logits[:, 0]) != np.sign(z64[:, 0]))
+    assert flips <= 0.005, flips
+    assert len(x) > 5000
+
+
+def test_neural_query_fp16_vs_fp64(nb, orc, bench):
+    _, rays = camera_rays(nb, orc, 128, 128)
+    rays = np.concatenate([rays, random_rays(4000, 3, inside_frac=0.3)])
+    got = bench["module"].neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    exp = bench["net"].neural_query(bench["nbvh"], rays, mode=orc.FP64)
+    kg, ke = nb.hit_kind(got), nb.hit_kind(exp)
+    assert (kg != ke).mean() <= 0.005
+    same = (kg == 1) & (ke == 1) & (got["node_id"] == exp["node_id"])
+    assert same.mean() > 0.3
+    rel = np.abs(got["t"][same] - exp["t"][same]) / np.maximum(exp["t"][same], 1e-3)
+    assert np.percentile(rel, 99) < 1e-2
+
+
+def test_errors_through_abi(nb, ctx):
+    scene = nb.Scene(ctx)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    rays = random_rays(10, 1)
+    with pytest.raises(nb.NbvhError) as e:
+        mod.neural_bvh_traversal(rays)
+    assert e.value.status == 6  # no N-BVH
+    with pytest.raises(nb.NbvhError) as e:
+        mod.neural_bvh_traversal(rays, kind=nb.QUERY_TRIANGLE_CLOSEST)
+    assert e.value.status == 6  # no triangles
+    scene.set_nbvh(nb.nbvh_cut(nb.bvh_build(*nb.icosphere(2))[0], 8))
+    with pytest.raises(nb.NbvhError) as e:
+        mod.neural_bvh_traversal(rays)
+    assert e.value.status == 6  # no weights
+    scene.init_network(nb.network_config(n_levels=2, log2_hashmap_size=10), seed=1)
+    mod.neural_bvh_traversal(rays)  # FP32 kernel exists for this shape
+    with pytest.raises(nb.NbvhError) as e:
+        mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    assert e.value.status == 7  # no MFMA kernel for K0 = 12
+    scene.close()

@@ -1,0 +1,188 @@
+"""Host side of libnbvh_hip.so (no GPU): ABI exports, builders, formats, error handling."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, GOLDEN
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "nbvh.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|uint32_t|const char\*)\s+(nbvh_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol(nb):
+    from nbvh_amd import _ffi
+    names = header_functions()
+    assert len(names) >= 30
+    L = _ffi.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", _ffi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (nbvh_\w+)", out))
+    assert set(names) <= exported
+    # Python binding covers the whole header.
+    assert set(names) == set(_ffi.SIGNATURES)
+    assert L.nbvh_abi_version() == 1
+
+
+def test_status_strings_and_null_args(nb):
+    from nbvh_amd import _ffi
+    L = _ffi.lib()
+    assert L.nbvh_status_string(0) == b"ok"
+    assert L.nbvh_status_string(5) == b"format error"
+    assert L.nbvh_bvh_build(None, 0, None, 0, 2, None, 0, None, None) == _ffi.ERR_INVALID_ARGUMENT
+    assert b"NULL" in L.nbvh_last_error(None)
+
+
+def test_icosphere_counts_and_displacement(nb):
+    for s in range(4):
+        v, t = nb.icosphere(s, seed=1, amplitude=0.0)
+        assert len(t) == 20 * 4 ** s and len(v) == 10 * 4 ** s + 2
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+    v, t = nb.icosphere(6, seed=7, amplitude=0.1)
+    assert len(t) == 81920 and len(v) == 40962
+    r = np.linalg.norm(v, axis=1)
+    assert 0.9 <= r.min() < 0.97 and 1.03 < r.max() <= 1.1
+    v2, _ = nb.icosphere(6, seed=7, amplitude=0.1)
+    v3, _ = nb.icosphere(6, seed=8, amplitude=0.1)
+    assert np.array_equal(v, v2) and not np.array_equal(v, v3)
+    # Every edge is shared by exactly two triangles (closed, manifold).
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(e, axis=0, return_counts=True)
+    assert np.all(counts == 2)
+
+
+def bvh_invariants(nodes, prims, v, t, max_leaf):
+    n = len(nodes)
+    assert sorted(prims.tolist()) == list(range(len(t)))
+    seen = np.zeros(len(t), int)
+    depth = np.zeros(n, int)
+    stack = [0]
+    visited = 0
+    while stack:
+        i = stack.pop()
+        visited += 1
+        nd = nodes[i]
+        if nd["prim_count"] > 0:
+            assert nd["prim_count"] <= max_leaf
+            for p in prims[nd["left_or_prim_offset"]:nd["left_or_prim_offset"] + nd["prim_count"]]:
+                seen[p] += 1
+                tv = v[t[p]]
+                assert np.all(tv >= nd["aabb_min"]) and np.all(tv <= nd["aabb_max"])
+        else:
+            l = nd["left_or_prim_offset"]
+            assert i < l and l + 1 < n
+            for c in (l, l + 1):
+                assert np.all(nodes[c]["aabb_min"] >= nd["aabb_min"]) and np.all(nodes[c]["aabb_max"] <= nd["aabb_max"])
+                depth[c] = depth[i] + 1
+                stack.append(c)
+    assert visited == n and np.all(seen == 1)
+    return depth.max()
+
+
+def test_bvh_build_invariants(nb):
+    v, t = nb.icosphere(4, seed=2, amplitude=0.1)
+    nodes, prims = nb.bvh_build(v, t)
+    assert bvh_invariants(nodes, prims, v, t, 2) < 64
+    # Degenerate input: all triangles identical (forces median splits).
+    tt = np.tile(t[:1], (100, 1))
+    nodes2, prims2 = nb.bvh_build(v, tt, max_leaf_size=2)
+    bvh_invariants(nodes2, prims2, v, tt, 2)
+
+
+def test_bvh_build_rejects_bad_input(nb):
+    v, t = nb.icosphere(1)
+    bad = t.copy()
+    bad[3, 1] = 10 ** 6
+    with pytest.raises(nb.NbvhError) as e:
+        nb.bvh_build(v, bad)
+    assert e.value.status == 1
+
+
+def test_nbvh_cut_invariants(nb):
+    v, t = nb.icosphere(5, seed=4, amplitude=0.1)
+    nodes, prims = nb.bvh_build(v, t)
+    for k in (1, 2, 64, 256):
+        cut = nb.nbvh_cut(nodes, k)
+        leaves = cut["left_or_prim_offset"] == nb.NO_NEURAL_CHILD
+        assert leaves.sum() == k and len(cut) == 2 * k - 1
+        assert set(cut["prim_count"][leaves].tolist()) <= {1, 2}
+        assert np.all(cut["prim_count"][~leaves] == 0)
+        assert np.array_equal(cut[0]["aabb_min"], nodes[0]["aabb_min"])
+        # Every triangle lies inside at least one leaf box.
+        c = v[t].mean(axis=1)
+        inside = np.zeros(len(t), bool)
+        for lf in cut[leaves]:
+            inside |= np.all((c >= lf["aabb_min"]) & (c <= lf["aabb_max"]), axis=1)
+        assert inside.all()
+        # Children adjacent, after the parent; boxes nested.
+        for i in np.nonzero(~leaves)[0]:
+            l = cut[i]["left_or_prim_offset"]
+            assert l > i and l + 1 < len(cut)
+    # Asking for more leaves than the BVH has gives the whole tree (mesh leaves: LEAF_FINAL).
+    tiny_v, tiny_t = nb.icosphere(0)
+    tn, _ = nb.bvh_build(tiny_v, tiny_t)
+    full = nb.nbvh_cut(tn, 1000)
+    assert len(full) == len(tn) and np.all(full["prim_count"][full["left_or_prim_offset"] == nb.NO_NEURAL_CHILD] == 1)
+
+
+def test_nbvh_roundtrip_and_corruption(nb):
+    data = open(os.path.join(GOLDEN, "chess_nodes_32940_hash_18_lvl_8.nbvh"), "rb").read()
+    nodes = nb.nbvh_parse(data)
+    assert nb.nbvh_serialize(nodes) == data
+    for bad in (data[:-1], data[:15], data + b"\0", b""):
+        with pytest.raises(nb.NbvhError) as e:
+            nb.nbvh_parse(bad)
+        assert e.value.status == 5
+    # A child index pointing outside the array, or a cycle, is rejected.
+    arr = bytearray(data)
+    arr[16 + 24:16 + 32] = (10 ** 6).to_bytes(8, "little")
+    with pytest.raises(nb.NbvhError):
+        nb.nbvh_parse(bytes(arr))
+    arr = bytearray(data)
+    arr[16 + 40 + 24:16 + 40 + 32] = (0).to_bytes(8, "little")  # node 1 -> child 0 (cycle)
+    with pytest.raises(nb.NbvhError):
+        nb.nbvh_parse(bytes(arr))
+
+
+def test_network_params(nb, orc):
+    cfg = nb.network_config()
+    nt, nm = nb.network_param_counts(cfg)
+    assert nm == 18944 and nt == 2 * (4096 + 13824 + 46656 + 157464 + 12 * 2 ** 19)
+    t, m = nb.network_init_params(cfg, 3, 1e-4)
+    assert np.abs(t).max() <= 1e-4 and abs(t.mean()) < 2e-6
+    assert np.abs(m[:64 * 96]).max() <= np.sqrt(6 / 96) and np.abs(m[64 * 96:64 * 96 + 4096]).max() <= np.sqrt(6 / 64)
+    t2, m2 = nb.network_init_params(cfg, 3, 1e-4)
+    assert np.array_equal(t, t2) and np.array_equal(m, m2)
+    bad = nb.network_config(n_neurons=32)
+    with pytest.raises(nb.NbvhError) as e:
+        nb.network_param_counts(bad)
+    assert e.value.status == 7
+    # Chess configuration (8 levels x 4 features, base 8, scale 2, 2^18).
+    chess = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=18, base_resolution=8,
+                              per_level_scale=2.0)
+    tables, mlp = nb.network_init_params(chess, 1)
+    assert orc.Net(chess, tables, mlp, (0, 0, 0), (1, 1, 1)).levels()["n_params"] == len(tables)
+
+
+def test_camera_look_at(nb):
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, 200, 100)
+    du, dv, tl = np.array(cam.du), np.array(cam.dv), np.array(cam.dir_top_left)
+    h = 2 * np.tan(np.radians(22.5))
+    assert np.allclose(np.linalg.norm(dv), h, rtol=1e-6) and np.allclose(np.linalg.norm(du), 2 * h, rtol=1e-6)
+    assert np.allclose(tl + 0.5 * du + 0.5 * dv, [0, 0, -1], atol=1e-6)
+    assert dv[1] < 0  # image y goes down
+
+
+def test_context_without_gpu_reports_no_device(nb):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(nb.NbvhError) as e:
+        nb.Context()
+    assert e.value.status == 3

@@ -1,0 +1,187 @@
+"""Pins of the oracle's geometry: slab test, node dilation, N-BVH traversal (leaf sets against a
+brute-force cut of the tree), the shipped chess .nbvh, and the triangle BVH against brute force."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, camera_rays, random_rays
+
+CHESS = os.path.join(GOLDEN, "chess_nodes_32940_hash_18_lvl_8.nbvh")
+FLT_MAX = np.float32(3.4028234663852886e38)
+
+
+def np_slab(bmin, bmax, o, d):
+    """Majercik slab test in float32 numpy with glm::min/max NaN semantics."""
+    with np.errstate(all="ignore"):
+        t0 = (np.float32(bmin) - o) / d
+        t1 = (np.float32(bmax) - o) / d
+    gmin = lambda x, y: np.where(y < x, y, x)
+    gmax = lambda x, y: np.where(x < y, y, x)
+    s, b = gmin(t0, t1), gmax(t0, t1)
+    tn = gmax(gmax(s[..., 0], s[..., 1]), s[..., 2])
+    tf = gmin(gmin(b[..., 0], b[..., 1]), b[..., 2])
+    return tn, tf
+
+
+def test_slab_matches_numpy_including_axis_aligned(orc):
+    rng = np.random.default_rng(1)
+    rays = random_rays(300, 2)
+    # Axis-aligned directions (zero components: +-inf and 0/0 = NaN slabs) and origins on planes.
+    rays["d"][:60] = np.float32([[1, 0, 0], [0, -1, 0], [0, 0, 1], [1, 1, 0], [0, 1, -1], [-1, 0, 0]] * 10)
+    rays["o"][30:60, 0] = np.float32(-0.5)
+    for i in range(len(rays)):
+        bmin = rng.uniform(-1, 0, 3).astype(np.float32)
+        bmax = (bmin + rng.uniform(0.1, 1.5, 3)).astype(np.float32)
+        if i < 60:
+            bmin[0] = np.float32(-0.5)
+        hit, tn, tf = orc.slab(bmin, bmax, rays[i])
+        etn, etf = np_slab(bmin, bmax, rays["o"][i], rays["d"][i])
+        assert np.array_equal(np.float32(tn), etn, equal_nan=True)
+        assert np.array_equal(np.float32(tf), etf, equal_nan=True)
+        assert hit == bool(etn <= etf)
+
+
+def test_slab_interval_brute_force(orc):
+    # The returned [tn, tf] contains exactly the sampled ray parameters whose points lie in the box.
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        o = rng.uniform(-3, 3, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        ray = np.zeros(1, orc.RAY_DTYPE)[0]
+        ray["o"], ray["d"] = o, d
+        bmin = rng.uniform(-1, 0, 3).astype(np.float32)
+        bmax = (bmin + rng.uniform(0.2, 1.2, 3)).astype(np.float32)
+        hit, tn, tf = orc.slab(bmin, bmax, ray)
+        ts = np.linspace(-8, 8, 4001)
+        pts = o[None, :].astype(np.float64) + ts[:, None] * d[None, :].astype(np.float64)
+        inside = np.all((pts >= bmin - 1e-6) & (pts <= bmax + 1e-6), axis=1)
+        strict = np.all((pts > bmin + 1e-4) & (pts < bmax - 1e-4), axis=1)
+        if strict.any():
+            assert hit and tn <= ts[strict].min() + 1e-5 and tf >= ts[strict].max() - 1e-5
+        if hit and tf - tn > 0.01:
+            mid = 0.5 * (tn + tf)
+            assert inside[np.argmin(np.abs(ts - mid))]
+        if not inside.any():
+            assert (not hit) or (tf - tn) < 1e-3
+
+
+def test_dilation_closed_form(orc, nb):
+    node = np.zeros(1, nb.NODE_DTYPE)[0]
+    node["aabb_min"] = np.float32([-1.0, 0.25, 3.0])
+    node["aabb_max"] = np.float32([1.0, 0.5, 7.0])
+    a, b = orc.dilate(node, 1e-5)
+    ext = node["aabb_max"] - node["aabb_min"]
+    c = node["aabb_min"] + ext * np.float32(0.5)
+    e2 = ext * np.float32(1.0 + np.float32(1e-5))
+    assert np.array_equal(a, c - e2 * np.float32(0.5)) and np.array_equal(b, c + e2 * np.float32(0.5))
+    # Dilation grows the box symmetrically by ext * 0.5e-5 (up to rounding).
+    assert np.allclose(node["aabb_min"] - a, ext * 0.5e-5, rtol=1e-2, atol=1e-7)
+
+
+def brute_leaf_set(orc, nodes, ray, lod):
+    """Leaves of the cut at `lod` whose dilated box passes the inference acceptance rule; the cut
+    is found by a plain recursive walk over the tree (no stack discipline, no culling)."""
+    out = set()
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        nd = nodes[i]
+        pc, left = int(nd["prim_count"]), int(nd["left_or_prim_offset"])
+        leaf = (0 < pc <= 2 + lod) or left == 0xFFFFFFFF
+        a, b = orc.dilate(nd, 1e-5)
+        hit, tn, tf = orc.slab(a, b, ray)
+        if not leaf:
+            if hit:
+                stack += [left, left + 1]
+            continue
+        if not hit:
+            continue
+        ok = tn < FLT_MAX and ((tf > np.float32(1e-3)) if tn < 0 else (tn > np.float32(1e-4)))
+        if ok:
+            out.add(i)
+    return out
+
+
+def check_leaf_sets(orc, nodes, rays, lod):
+    n_nonempty = 0
+    for r in rays:
+        seq = orc.leaf_sequence(nodes, r, lod=lod)
+        assert seq is not None, "stack overflow"
+        ids = list(seq[0])
+        assert len(ids) == len(set(ids)), "a leaf visited twice"
+        assert set(ids) == brute_leaf_set(orc, nodes, r, lod)
+        n_nonempty += len(ids) > 0
+        # Segments are the dilated slab interval of the visited leaf.
+        for node, mn, mx in zip(*seq):
+            a, b = orc.dilate(nodes[node], 1e-5)
+            _, tn, tf = orc.slab(a, b, r)
+            assert mn == tn and mx == tf
+    return n_nonempty
+
+
+def test_leaf_sets_bench_cut(nb, orc, small_scene):
+    v, t, nodes, prims, nbn = small_scene
+    rays = random_rays(150, 5, inside_frac=0.3)
+    assert check_leaf_sets(orc, nbn, rays, 0) > 60
+
+
+def test_chess_fixture_format(nb):
+    data = open(CHESS, "rb").read()
+    assert len(data) == 1009984
+    nodes = nb.nbvh_parse(data)
+    assert len(nodes) == 25249
+    pc = nodes["prim_count"]
+    hist = {int(k): int(c) for k, c in zip(*np.unique(pc, return_counts=True))}
+    assert hist == {0: 12624, 1: 106, 8: 12519}
+    leaves = nodes["left_or_prim_offset"] == 0xFFFFFFFF
+    assert leaves.sum() == 12625
+    assert np.allclose(nodes[0]["aabb_min"], [-0.2457, 0.0149, -0.2491], atol=1e-4)
+    assert np.allclose(nodes[0]["aabb_max"], [0.2469, 0.169, 0.2481], atol=1e-4)
+    # Serialisation round trip is byte-exact.
+    assert nb.nbvh_serialize(nodes) == data
+
+
+def test_leaf_sets_chess_fixture(nb, orc):
+    nodes = nb.nbvh_parse(open(CHESS, "rb").read())
+    lo, hi = nodes[0]["aabb_min"], nodes[0]["aabb_max"]
+    rng = np.random.default_rng(11)
+    rays = np.zeros(60, orc.RAY_DTYPE)
+    o = rng.uniform(lo - 0.3, hi + 0.3, (60, 3))
+    tgt = rng.uniform(lo, hi, (60, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, FLT_MAX
+    for lod in (0, 6):
+        assert check_leaf_sets(orc, nodes, rays, lod) > 20
+
+
+def test_triangle_bvh_matches_brute_force(nb, orc, small_scene):
+    v, t, nodes, prims, nbn = small_scene
+    _, rays = camera_rays(nb, orc, 48, 48)
+    rays = np.concatenate([rays, random_rays(400, 9, inside_frac=0.5)])
+    h = orc.tri_query(v, t, nodes, prims, rays)
+    b = orc.tri_brute(v, t, rays)
+    assert np.array_equal(h["info"] & 0xFF, b["info"] & 0xFF)
+    assert np.array_equal(h["t"], b["t"])
+    assert np.array_equal(h["prim_id"], b["prim_id"])
+    assert (h["info"] & 0xFF == 2).mean() > 0.3
+    # Any-hit finds a hit exactly when closest-hit does.
+    a = orc.tri_query(v, t, nodes, prims, rays, any_hit=True)
+    assert np.array_equal(a["info"] & 0xFF, h["info"] & 0xFF)
+
+
+def test_watertight_shared_edges(nb, orc):
+    # Rays through shared vertices/edges of a closed mesh never slip through (watertightness):
+    # every ray from outside aimed at the centre of a closed sphere hits.
+    v, t = nb.icosphere(2, seed=0, amplitude=0.0)
+    nodes, prims = nb.bvh_build(v, t)
+    rays = np.zeros(len(v), orc.RAY_DTYPE)
+    rays["o"] = (v * 3.0).astype(np.float32)
+    d = -v / np.linalg.norm(v, axis=1, keepdims=True)
+    rays["d"] = d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, FLT_MAX
+    h = orc.tri_query(v, t, nodes, prims, rays)
+    assert np.all(h["info"] & 0xFF == 2)
