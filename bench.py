@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: Mrays/s of the N-BVH neural ray query on one MI355X.
+
+Workload (one step = one query over the whole ray batch, rays resident in HBM):
+  displaced icosphere (6 subdivisions = 81,920 triangles, 3-octave value noise, amplitude 0.1),
+  N-BVH of 256 leaves cut from its binned-SAH BVH, 1024 x 1024 jittered primary rays from a
+  pinhole camera at distance 3 (fovy 45), hash grid 16 levels x 2 features, 2^19 entries per
+  level, base resolution 16, scale 1.5, tables U(+-1e-4), MLP 96 -> 64 x 4 -> 8 (ReLU, Sigmoid),
+  Kaiming-uniform seeded weights.
+The headline value is the FP16-MFMA path; the FP32 plain path is timed beside it, the FP16
+visibility decisions are compared with the FP64 oracle on a sample, and the oracle's FP32 CPU
+path is timed on the host cores (cpu_baseline).
+
+  python bench.py [--steps K] [--warmup W] [--precision fp16|bf16|fp32] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s neural query, FP16 MFMA vs FP32 plain @<=0.5% vis flip; p99 ms"
+HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+FLOP_PER_SEGMENT = 2 * (96 * 64 + 3 * 64 * 64 + 64 * 8)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "fp32"])
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the neural query is not sharded across GPUs in this build "
+              "(rays are independent; see DESIGN.md for how the batch would shard). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nbvh_amd as nb
+    from nbvh_amd import hip
+
+    W, H = args.width, args.height
+    n_rays = W * H
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp32": nb.PREC_FP32}[args.precision]
+
+    log("building scene")
+    v, t = nb.icosphere(6, seed=7, amplitude=0.1)
+    ctx = nb.Context()
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    cfg = nb.network_config()
+    scene.init_network(cfg, seed=1, table_range=1e-4)
+    module = nb.NeuralBVHModule(ctx, scene)
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+
+    stream = hip.Stream()
+    d_rays = hip.Buffer(n_rays * 32)
+    d_hits = hip.Buffer(n_rays * 48)
+    ctx.raygen(cam, W, H, d_rays.ptr, stream=stream.handle)
+    stream.synchronize()
+
+    def timed(precision, steps, warmup):
+        p = module.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=precision)
+        for _ in range(warmup):
+            module.query_device(p, d_rays.ptr, n_rays, d_hits.ptr, stream=stream.handle)
+        stream.synchronize()
+        hip.synchronize()
+        evs = [hip.Event() for _ in range(steps + 1)]
+        t0 = time.perf_counter()
+        evs[0].record(stream)
+        for i in range(steps):
+            module.query_device(p, d_rays.ptr, n_rays, d_hits.ptr, stream=stream.handle)
+            evs[i + 1].record(stream)
+        stream.synchronize()
+        hip.synchronize()
+        wall = time.perf_counter() - t0
+        per = np.array([evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)])
+        return per, wall
+
+    log(f"timing {args.precision}: warmup {args.warmup}, steps {args.steps}")
+    per, wall = timed(prec, args.steps, args.warmup)
+    hits = d_hits.download(nb.HIT_DTYPE)
+    segs = int(nb.hit_segments(hits).astype(np.int64).sum())
+    ms = float(per.mean())
+    value = n_rays / (ms * 1e-3) / 1e6
+    log(f"{args.precision}: {ms:.3f} ms/step, {value:.1f} Mrays/s, {segs} segments")
+
+    result = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic: procedural displaced icosphere, seeded hash tables and MLP weights",
+        "config": {"workload": f"neural closest-hit query, {W}x{H} primary rays, icosphere 81920 tris, "
+                               "N-BVH 256 leaves, hash grid 16x2 2^19 base 16 scale 1.5, MLP 96-64x4-8",
+                   "rays": n_rays, "segments": segs, "precision": args.precision},
+        "latency_ms": {"p50": round(float(np.percentile(per, 50)), 4),
+                       "p99": round(float(np.percentile(per, 99)), 4)},
+        "wall_s_timed": round(wall, 4),
+    }
+
+    # Roofline of the dominant kernel (the query kernel is the only kernel of a step besides a
+    # one-thread counter reset). Algorithmic bytes: hash-grid gathers 3 points x 16 levels x
+    # 8 corners x 2 features x table bytes per segment, plus 32 B ray in + 48 B hit out per ray.
+    tb = 4 if prec == nb.PREC_FP32 else 2
+    alg_bytes = segs * 3 * 16 * 8 * 2 * tb + n_rays * (32 + 48)
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    result["roofline"] = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                          "bytes_per_launch": alg_bytes,
+                          "mfma_tflops": round(segs * FLOP_PER_SEGMENT / (ms * 1e-3) / 1e12, 3)}
+
+    if not args.no_fp32 and prec != nb.PREC_FP32:
+        per32, _ = timed(nb.PREC_FP32, max(3, args.steps // 4), 1)
+        ms32 = float(per32.mean())
+        v32 = n_rays / (ms32 * 1e-3) / 1e6
+        result["fp32_plain"] = {"value": round(v32, 3), "ms_per_step": round(ms32, 4)}
+        result["speedup_vs_fp32_plain"] = round(value / v32, 3)
+        log(f"fp32 plain: {ms32:.3f} ms/step, {v32:.1f} Mrays/s -> speedup {value / v32:.2f}x")
+
+    # Visibility agreement with the FP64 oracle on a sample of the workload's segments (the
+    # oracle's segment log of every 16th row), evaluated by the same network kernel.
+    try:
+        import oracle
+        cfg2, tables, mlp = scene.network()
+        bmin, bmax = scene.bbox()
+        net = oracle.Net(cfg2, tables, mlp, bmin, bmax)
+        rays = d_rays.download(nb.RAY_DTYPE)
+        sample = rays.reshape(H, W)[::16].reshape(-1)
+        _, seglog = net.neural_query(scene.nbvh(), sample, mode=oracle.FP64, log_segments=True)
+        x = net.encode(sample[seglog["ray"]], seglog["mint"], seglog["maxt"], mode=oracle.FP32)
+        _, z = module.network_inference(x, precision=prec)
+        flips = float(np.mean((z[:, 0] > 0) != (seglog["logit"][:, 0] > 0)))
+        result["vis_flip_rate"] = flips
+        result["vis_flip_sample_segments"] = int(len(x))
+        log(f"visibility flips vs FP64 oracle: {flips:.5f} over {len(x)} segments")
+
+        if not args.no_cpu_baseline:
+            # CPU baseline: the oracle's FP32 path (same traversal + hash grid + FP32 MLP), all
+            # host threads OpenMP-parallel, on the full frame; repeated to >= 10 s of CPU time.
+            nodes = scene.nbvh()
+            net.neural_query(nodes, rays[:4096], mode=oracle.FP32)
+            reps, t0 = 0, time.perf_counter()
+            while True:
+                net.neural_query(nodes, rays, mode=oracle.FP32)
+                reps += 1
+                el = time.perf_counter() - t0
+                if el * oracle.num_threads() >= 10.0 or el > 30.0:
+                    break
+            cpu = reps * n_rays / el / 1e6
+            result["cpu_baseline"] = {"value": round(cpu, 4), "unit": "Mrays/s", "cores": oracle.num_threads(),
+                                      "kind": "port",
+                                      "sample": f"{reps} x full {W}x{H} frame, oracle FP32 path, {el:.1f} s wall"}
+            log(f"cpu baseline: {cpu:.3f} Mrays/s on {oracle.num_threads()} threads")
+    except ImportError as e:
+        log(f"oracle unavailable ({e}); no flip rate / cpu baseline")
+
+    print(json.dumps(result), flush=True)
+    scene.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
