@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device-to-device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the neural query is not sharded across GPUs in this build "
@@ -75,7 +77,7 @@ def main():
     stream.synchronize()
 
     def timed(precision, steps, warmup):
-        p = module.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=precision)
+        p = module.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=precision, image_width=W)
         for _ in range(warmup):
             module.query_device(p, d_rays.ptr, n_rays, d_hits.ptr, stream=stream.handle)
         stream.synchronize()
@@ -168,6 +170,16 @@ def main():
             log(f"cpu baseline: {cpu:.3f} Mrays/s on {oracle.num_threads()} threads")
     except ImportError as e:
         log(f"oracle unavailable ({e}); no flip rate / cpu baseline")
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = hip.Buffer(nbytes), hip.Buffer(nbytes)
+        src.zero()
+        hip.synchronize()
+        hip.check(hip._runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, hip.D2D), "hipMemcpy D2D")
+        hip.synchronize()
+        result["calibration_copy_bytes"] = nbytes
+        log(f"calibration copy: {nbytes} bytes device to device")
 
     print(json.dumps(result), flush=True)
     scene.close()

@@ -150,7 +150,11 @@ typedef struct nbvh_query_params {
     uint32_t lod;                 /* rendered_lod, 0..6 */
     uint32_t max_traversal_steps; /* reference m_max_traversal_steps; 0 => 1000 */
     float    visibility_threshold;/* primary_visibility_threshold on the sigmoid output; 0 => 0.5 */
-    uint32_t reserved[3];
+    uint32_t image_width;         /* scheduling hint: the rays are an image of this width in scanline
+                                     order (width % 8 == 0, n_rays % (8*width) == 0); they are then
+                                     traced in 8x8 pixel tiles. 0 => array order. Results do not
+                                     depend on it. */
+    uint32_t reserved[2];
 } nbvh_query_params;
 
 /* Pinhole camera in the reference's CameraParams form (origin, du, dv, dir_top_left). */

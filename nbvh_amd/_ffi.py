@@ -55,7 +55,7 @@ class NetworkConfig(ctypes.Structure):
 class QueryParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("precision", ctypes.c_uint32), ("lod", ctypes.c_uint32),
                 ("max_traversal_steps", ctypes.c_uint32), ("visibility_threshold", ctypes.c_float),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("image_width", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Camera(ctypes.Structure):

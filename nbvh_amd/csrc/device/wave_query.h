@@ -39,7 +39,9 @@ struct LdsStack {
     __device__ __forceinline__ int size() const { return n; }
 };
 
-enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DONE = 3 };
+// EMPTY: needs a ray; NEW: ray claimed, not loaded; RAY: traversing to the next leaf;
+// SEG: holds a leaf segment to evaluate; DONE: the queue is drained.
+enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DONE = 3, SLOT_NEW = 4 };
 
 struct Slot {
     uint32_t state;
@@ -72,36 +74,100 @@ __device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, nbvh_hit* 
     s.state = SLOT_EMPTY;
 }
 
+// XCD (accelerator complex die) the wave runs on: HW_REG_XCC_ID bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;
+}
+
+// Ray queue split into kQueueBands contiguous bands of the ray array, one counter per band.
+// A wave starts on the band of its XCD, so rays that are neighbours in the array (neighbouring
+// pixels for image-ordered rays) are traced on the same XCD and share its L2; a wave moves on
+// to the next band only when its band is drained (work stealing for the tail).
+#ifndef NBVH_QUEUE_BANDS
+#define NBVH_QUEUE_BANDS 8
+#endif
+constexpr uint32_t kQueueBands   = NBVH_QUEUE_BANDS;
+constexpr uint32_t kQueueStride  = 4;                             // counters 16 B apart
+constexpr uint32_t kQueueWords   = kQueueBands * kQueueStride;    // u32 words per query
+
+struct RayQueue {
+    uint32_t* heads;
+    uint32_t n_rays;
+    uint32_t image_width;  // 0: claim order = array order; else 8x8-tile order of a scanline image
+    uint32_t band;         // wave-uniform
+    uint32_t tried;        // wave-uniform: bands found drained
+    __device__ __forceinline__ void init(uint32_t* h, uint32_t n, uint32_t width)
+    {
+        heads = h;
+        n_rays = n;
+        image_width = width;
+        band = xcc_id() % kQueueBands;
+        tried = 0;
+    }
+    __device__ __forceinline__ uint32_t band_lo(uint32_t b) const
+    {
+        return (uint32_t)(((uint64_t)b * n_rays) / kQueueBands);
+    }
+    // Claim index -> ray index. With an image width W (W % 8 == 0, n % (8W) == 0, checked on the
+    // host) consecutive claims walk 8x8 pixel tiles, tile rows of 8 scanlines in order, so the
+    // 64 slots of a wave trace a compact pixel block (gathers of neighbouring rays share cache
+    // lines at the coarse and middle grid levels). A bijection on [0, n).
+    __device__ __forceinline__ uint32_t ray_of(uint32_t i) const
+    {
+        if (image_width == 0u) return i;
+        const uint32_t block = i / (8u * image_width), j = i - block * 8u * image_width;
+        const uint32_t tile = j >> 6, w = j & 63u;
+        return (block * 8u + (w >> 3)) * image_width + tile * 8u + (w & 7u);
+    }
+};
+
 // Phase A. Returns false when the wave has no segment left to evaluate (all slots DONE).
 __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& st, const nbvh_ray* __restrict__ rays,
-                                           uint32_t n_rays, nbvh_hit* __restrict__ hits, uint32_t* __restrict__ queue)
+                                           RayQueue& q, nbvh_hit* __restrict__ hits)
 {
     const uint32_t lane = threadIdx.x & 63u;
     while (true) {
-        if (s.state == SLOT_EMPTY) {
-            const uint64_t m = __ballot(1);
-            const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (r < n_rays) {
-                s.state    = SLOT_RAY;
-                s.ray_id   = r;
-                s.ray      = load_ray(rays, r);
-                s.best_t   = s.ray.tmax;
-                s.segments = 0;
-                s.flags    = 0;
-                s.kind     = 0;
-                s.hit_node = 0xFFFFFFFFu;
-                s.hit_logit = 0.f;
-                s.nrm[0] = s.nrm[1] = s.nrm[2] = 0.f;
-                s.alb[0] = s.alb[1] = s.alb[2] = 0.f;
-                st.n = 0;
-                st.push(0u);
+        // Claim rays for the empty slots (convergent: every lane takes part in the ballot).
+        const uint64_t m = __ballot(s.state == SLOT_EMPTY);
+        if (m != 0ull) {
+            if (q.tried >= kQueueBands) {
+                if (s.state == SLOT_EMPTY) s.state = SLOT_DONE;
             } else {
-                s.state = SLOT_DONE;
+                const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(q.heads + q.band * kQueueStride, (uint32_t)__popcll(m));
+                base = __shfl(base, leader);
+                const uint32_t lo = q.band_lo(q.band), hi = q.band_lo(q.band + 1u);
+                bool failed = false;
+                if (s.state == SLOT_EMPTY) {
+                    const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    if (r < hi - lo) {
+                        s.state  = SLOT_NEW;
+                        s.ray_id = q.ray_of(lo + r);
+                    } else {
+                        failed = true;
+                    }
+                }
+                if (__any(failed)) {
+                    q.band = (q.band + 1u) % kQueueBands;
+                    q.tried++;
+                }
             }
+        }
+        if (s.state == SLOT_NEW) {
+            s.state    = SLOT_RAY;
+            s.ray      = load_ray(rays, s.ray_id);
+            s.best_t   = s.ray.tmax;
+            s.segments = 0;
+            s.flags    = 0;
+            s.kind     = 0;
+            s.hit_node = 0xFFFFFFFFu;
+            s.hit_logit = 0.f;
+            s.nrm[0] = s.nrm[1] = s.nrm[2] = 0.f;
+            s.alb[0] = s.alb[1] = s.alb[2] = 0.f;
+            st.n = 0;
+            st.push(0u);
         }
         if (s.state == SLOT_RAY) {
             if (st.empty()) {

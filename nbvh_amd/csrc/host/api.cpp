@@ -33,7 +33,8 @@ struct nbvh_ctx {
     std::string last_error;
     uint32_t* d_counters = nullptr;  // ring of queue counters for concurrent launches
     std::atomic<uint32_t> counter_next{0};
-    static constexpr uint32_t kCounters = 4096;
+    static constexpr uint32_t kCounterWords = 32;            // per query: 8 band heads, 16 B apart
+    static constexpr uint32_t kCounters     = 2048 * kCounterWords;
 };
 
 namespace nbvh {
@@ -211,6 +212,9 @@ void refresh_args(nbvh_scene* s)
     for (size_t l = 0; l < lv.size(); l++) a.level[l] = lv[l];
     a.tables_f32      = s->d_tables32.as<float>();
     a.tables_f16      = s->d_tables16.as<uint16_t>();
+    // The MFMA kernels address the FP16 tables through a buffer resource (31-bit byte range);
+    // larger tables leave tables_f16_bytes at 0 and those kernels report UNSUPPORTED.
+    a.tables_f16_bytes = s->d_tables16.bytes < (1ull << 31) ? (uint32_t)s->d_tables16.bytes : 0u;
     a.table_scale     = s->table_scale;
     a.inv_table_scale = 1.f / s->table_scale;
     a.mlp_f32         = s->d_mlp32.as<float>();
@@ -246,7 +250,13 @@ void upload_network(nbvh_scene* s)
     for (int i = 0; i < 8; i++) s->fp8_wscale[i] = mw.fp8_wscale[i];
 }
 
-uint32_t counter_slot(nbvh_ctx* ctx) { return ctx->counter_next.fetch_add(1) % nbvh_ctx::kCounters; }
+// Queue counters of one query; a ring of 2048 slots, so at most 2048 queries of one context may
+// be in flight at once.
+uint32_t counter_slot(nbvh_ctx* ctx)
+{
+    constexpr uint32_t slots = nbvh_ctx::kCounters / nbvh_ctx::kCounterWords;
+    return (ctx->counter_next.fetch_add(1) % slots) * nbvh_ctx::kCounterWords;
+}
 
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays, uint32_t n,
                nbvh_hit* d_hits, hipStream_t stream)
@@ -274,8 +284,11 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     const float thr = p->visibility_threshold == 0.f ? 0.5f : p->visibility_threshold;
     require(thr > 0.f && thr < 1.f, "visibility_threshold must be in (0, 1)");
     a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
+    // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
+    const uint64_t w = p->image_width;
+    a.image_width = (w != 0 && w % 8 == 0 && (uint64_t)n % (8 * w) == 0) ? (uint32_t)w : 0u;
     uint32_t* counter = ctx->d_counters + counter_slot(ctx);
-    hip_check(zero_u32(counter, 1, stream), "counter reset");
+    hip_check(zero_u32(counter, nbvh_ctx::kCounterWords, stream), "counter reset");
     bool supported = false;
     if (p->precision == NBVH_PREC_FP32)
         hip_check(query_fp32(a, d_rays, n, d_hits, counter, (uint32_t)ctx->n_cus, stream, &supported), "fp32 query");

@@ -15,13 +15,19 @@ namespace dev {
 // MLP with 4 hidden layers of 64 (5 matrices). Weight layout of a.mlp_f32 for this kernel:
 // layers 1, 3, 5 row-major (out x in), layers 2, 4 transposed (in x out), see
 // host/api.cpp:fp32_kernel_layout. Accumulation: acc = fmaf(W[j][k], x[k], acc), k ascending.
+// Weights are read through the constant address space: they are read-only for the whole launch,
+// so wave-uniform reads become scalar loads into SGPRs (v_fma with an SGPR operand) instead of
+// one VGPR per weight.
+typedef const float __attribute__((address_space(4))) cfloat;
+
 template <int K0>
-__device__ __forceinline__ void mlp_f32(const float* __restrict__ W, const float* x, float* h_lds, float z[8])
+__device__ __forceinline__ void mlp_f32(const float* W_generic, const float* x, float* h_lds, float z[8])
 {
+    const cfloat* __restrict__ W = (const cfloat*)W_generic;
     const uint32_t lane = threadIdx.x & 63u;
     // Layer 1: j rolled, inputs in registers, output to LDS.
     for (int j = 0; j < 64; j++) {
-        const float* w = W + j * K0;
+        const cfloat* w = W +j * K0;
         float acc = 0.f;
 #pragma unroll
         for (int k = 0; k < K0; k++) acc = fmaf(w[k], x[k], acc);
@@ -34,7 +40,7 @@ __device__ __forceinline__ void mlp_f32(const float* __restrict__ W, const float
     for (int j = 0; j < 64; j++) r[j] = 0.f;
     for (int k = 0; k < 64; k++) {
         const float hk = h_lds[k * 64 + lane];
-        const float* w = W + k * 64;
+        const cfloat* w = W +k * 64;
 #pragma unroll
         for (int j = 0; j < 64; j++) r[j] = fmaf(w[j], hk, r[j]);
     }
@@ -43,7 +49,7 @@ __device__ __forceinline__ void mlp_f32(const float* __restrict__ W, const float
     W += 64 * 64;
     // Layer 3: j rolled, inputs in registers, output to LDS.
     for (int j = 0; j < 64; j++) {
-        const float* w = W + j * 64;
+        const cfloat* w = W +j * 64;
         float acc = 0.f;
 #pragma unroll
         for (int k = 0; k < 64; k++) acc = fmaf(w[k], r[k], acc);
@@ -55,7 +61,7 @@ __device__ __forceinline__ void mlp_f32(const float* __restrict__ W, const float
     for (int j = 0; j < 64; j++) r[j] = 0.f;
     for (int k = 0; k < 64; k++) {
         const float hk = h_lds[k * 64 + lane];
-        const float* w = W + k * 64;
+        const cfloat* w = W +k * 64;
 #pragma unroll
         for (int j = 0; j < 64; j++) r[j] = fmaf(w[j], hk, r[j]);
     }
@@ -73,7 +79,12 @@ __device__ __forceinline__ void mlp_f32(const float* __restrict__ W, const float
 }
 
 template <int P, int L, int F>
-__global__ __launch_bounds__(64) void neural_query_fp32_kernel(NetArgs a, const nbvh_ray* __restrict__ rays,
+// Minimum waves per SIMD for the register allocation. Measured: forcing 2 halves the FP32
+// query's throughput (the allocator then starves the scalar weight prefetch), so 1 (no bound).
+#ifndef NBVH_FP32_WAVES_PER_EU
+#define NBVH_FP32_WAVES_PER_EU 1
+#endif
+__global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_kernel(NetArgs a, const nbvh_ray* __restrict__ rays,
                                                                uint32_t n_rays, nbvh_hit* __restrict__ hits,
                                                                uint32_t* __restrict__ queue)
 {
@@ -84,7 +95,9 @@ __global__ __launch_bounds__(64) void neural_query_fp32_kernel(NetArgs a, const 
     LdsStack st{stack_lds + lane, 0};
     Slot s;
     s.state = SLOT_EMPTY;
-    while (fill_slots(a, s, st, rays, n_rays, hits, queue)) {
+    RayQueue q;
+    q.init(queue, n_rays, a.image_width);
+    while (fill_slots(a, s, st, rays, q, hits)) {
         float x[K0];
         if (s.state == SLOT_SEG) {
             float pts[3 * P];

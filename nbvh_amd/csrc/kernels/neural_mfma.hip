@@ -73,22 +73,127 @@ __device__ __forceinline__ void swap_halves(V& x, V& y)
 // 32..63. Swapping lanes 32..63 of lo with lanes 0..31 of hi produces exactly these operands
 // (lo becomes column block 0, hi column block 1), matching the weight order
 // K(s, g, e) = g*K0/2 + 8s + e of host/mfma_layout.cpp.
+// Trilinear interpolation of one level from the FP16 tables through a buffer resource: 32-bit
+// byte offsets (one VGPR per gather instead of a 64-bit address), and the hardware range check
+// returns 0 for any offset past the table instead of faulting.
+template <int F, bool EXACT>
+__device__ __forceinline__ void interp_level_f16(const GridLevel& L, __amdgpu_buffer_rsrc_t tab, float x, float y,
+                                                 float z, float acc[F])
+{
+    uint32_t g[3];
+    float f[3];
+    grid_cell(L, x, y, z, g, f);
+    uint32_t idx[8];
+    grid_corners<EXACT>(L, g, idx);
+    const uint32_t base = L.offset * (uint32_t)(2 * F);
+#pragma unroll
+    for (int k = 0; k < F; k++) acc[k] = 0.f;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; c++) {
+        const float w = corner_weight(c, f);
+        const uint32_t off = base + idx[c] * (uint32_t)(2 * F);
+        if constexpr (F == 2) {
+            const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(tab, off, 0, 0);
+            acc[0] = fmaf(w, half_bits_to_float(q & 0xFFFFu), acc[0]);
+            acc[1] = fmaf(w, half_bits_to_float(q >> 16), acc[1]);
+        } else {
+            static_assert(F == 4, "FP16 gather implemented for F = 2 and F = 4");
+            const auto q = __builtin_amdgcn_raw_buffer_load_b64(tab, off, 0, 0);
+            acc[0] = fmaf(w, half_bits_to_float(q[0] & 0xFFFFu), acc[0]);
+            acc[1] = fmaf(w, half_bits_to_float(q[0] >> 16), acc[1]);
+            acc[2] = fmaf(w, half_bits_to_float(q[1] & 0xFFFFu), acc[2]);
+            acc[3] = fmaf(w, half_bits_to_float(q[1] >> 16), acc[3]);
+        }
+    }
+}
+
+// Levels whose gathers are issued together (G x 8 loads in flight per lane before the first use).
+#ifndef NBVH_GATHER_GROUP
+#define NBVH_GATHER_GROUP 4
+#endif
+
+// Features of all P points of this lane's segment, converted to 16 bits as each level completes:
+// feature k = p*L*F + l*F + f goes to lo[k/8][k%8] (k < K0/2) or hi[(k-K0/2)/8][..].
+// Levels are processed in groups of G: all corner offsets of the group, then all its loads,
+// then the weights and the accumulation (same per-level arithmetic as interp_level_f16).
+template <int P, int L, int F, bool BF16, bool EXACT>
+__device__ __forceinline__ void gather_operands(const NetArgs& a, __amdgpu_buffer_rsrc_t tab, const float* pts,
+                                                typename Frag<BF16>::vec* lo, typename Frag<BF16>::vec* hi)
+{
+    constexpr int HALF = P * L * F / 2;
+    constexpr int G = (L % NBVH_GATHER_GROUP == 0) ? NBVH_GATHER_GROUP : 1;
+    typedef typename Frag<BF16>::elem E;
+    static_assert(F == 2 || F == 4, "FP16 gather implemented for F = 2 and F = 4");
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+#pragma unroll
+        for (int l0 = 0; l0 < L; l0 += G) {
+            uint32_t off[G][8];
+            float fr[G][3];
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                const GridLevel& lv = a.level[l0 + i];
+                uint32_t g[3], idx[8];
+                grid_cell(lv, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], g, fr[i]);
+                grid_corners<EXACT>(lv, g, idx);
+                const uint32_t base = lv.offset * (uint32_t)(2 * F);
+#pragma unroll
+                for (int c = 0; c < 8; c++) off[i][c] = base + idx[c] * (uint32_t)(2 * F);
+            }
+            // One load per corner. (Measured: serving x-neighbour corners with one 16-byte load
+            // plus a range-masked second load halves the throughput, so corners stay separate.)
+            uint32_t q[G][8][F / 2];
+#pragma unroll
+            for (int i = 0; i < G; i++)
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    if constexpr (F == 2) {
+                        q[i][c][0] = __builtin_amdgcn_raw_buffer_load_b32(tab, off[i][c], 0, 0);
+                    } else {
+                        const auto v = __builtin_amdgcn_raw_buffer_load_b64(tab, off[i][c], 0, 0);
+                        q[i][c][0] = v[0];
+                        q[i][c][1] = v[1];
+                    }
+                }
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                float acc[F];
+#pragma unroll
+                for (int k = 0; k < F; k++) acc[k] = 0.f;
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const float w = corner_weight((uint32_t)c, fr[i]);
+#pragma unroll
+                    for (int k = 0; k < F; k++) {
+                        const uint32_t bits = (k & 1) ? (q[i][c][k / 2] >> 16) : (q[i][c][k / 2] & 0xFFFFu);
+                        acc[k] = fmaf(w, half_bits_to_float(bits), acc[k]);
+                    }
+                }
+#pragma unroll
+                for (int f = 0; f < F; f++) {
+                    const int k = p * L * F + (l0 + i) * F + f;
+                    if (k < HALF)
+                        lo[k / 8][k % 8] = (E)acc[f];
+                    else
+                        hi[(k - HALF) / 8][(k - HALF) % 8] = (E)acc[f];
+                }
+            }
+        }
+    }
+}
+
 template <int P, int L, int F, bool BF16>
 __device__ __forceinline__ void layer1_operands(const NetArgs& a, const float* pts,
                                                 typename Frag<BF16>::vec lo[P * L * F / 16],
                                                 typename Frag<BF16>::vec hi[P * L * F / 16])
 {
-    constexpr int K0 = P * L * F, HALF = K0 / 2;
-    typedef typename Frag<BF16>::elem E;
-    float x[K0];
-    grid_features<P, L, F, uint16_t>(a, a.tables_f16, pts, x);
-#pragma unroll
-    for (int k = 0; k < K0; k++) {
-        if (k < HALF)
-            lo[k / 8][k % 8] = (E)x[k];
-        else
-            hi[(k - HALF) / 8][(k - HALF) % 8] = (E)x[k];
-    }
+    constexpr int K0 = P * L * F;
+    const __amdgpu_buffer_rsrc_t tab =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.tables_f16, (short)0, (int)a.tables_f16_bytes, 0x00020000);
+    if (__all(in_unit_cube<3 * P>(pts)))
+        gather_operands<P, L, F, BF16, false>(a, tab, pts, lo, hi);
+    else
+        gather_operands<P, L, F, BF16, true>(a, tab, pts, lo, hi);
 #pragma unroll
     for (int s = 0; s < K0 / 16; s++) swap_halves(lo[s], hi[s]);
 }
@@ -198,9 +303,13 @@ __device__ __forceinline__ void stage_weights(const NetArgs& a, uint4* wfrag)
 }
 
 constexpr int kWavesPerBlock = 4;
+// Minimum waves per SIMD the register allocation must allow (latency hiding of the gathers).
+#ifndef NBVH_MFMA_WAVES_PER_EU
+#define NBVH_MFMA_WAVES_PER_EU 2
+#endif
 
 template <int P, int L, int F, bool BF16>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void neural_query_mfma_kernel(NetArgs a,
+__global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void neural_query_mfma_kernel(NetArgs a,
                                                                                 const nbvh_ray* __restrict__ rays,
                                                                                 uint32_t n_rays,
                                                                                 nbvh_hit* __restrict__ hits,
@@ -214,7 +323,9 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void neural_query_mfma_kernel(
     LdsStack st{stack_lds + wave * kNeuralStackSize * 64 + lane, 0};
     Slot s;
     s.state = SLOT_EMPTY;
-    while (fill_slots(a, s, st, rays, n_rays, hits, queue)) {
+    RayQueue q;
+    q.init(queue, n_rays, a.image_width);
+    while (fill_slots(a, s, st, rays, q, hits)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
             encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
@@ -297,7 +408,7 @@ hipError_t query_mfma(const NetArgs& a, uint32_t precision, const nbvh_ray* rays
                       uint32_t* queue, uint32_t n_cus, hipStream_t stream, bool* supported)
 {
     *supported = true;
-    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr) {
+    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr && a.tables_f16_bytes != 0) {
 #define X(P_, L_, F_)                                                                 \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
         if (precision == NBVH_PREC_FP16)                                             \
@@ -316,7 +427,7 @@ hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, u
                         hipStream_t stream, bool* supported)
 {
     *supported = true;
-    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr) {
+    if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr && a.tables_f16_bytes != 0) {
 #define X(P_, L_, F_)                                                                 \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
         if (precision == NBVH_PREC_FP16)                                             \

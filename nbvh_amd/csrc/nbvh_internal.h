@@ -37,11 +37,13 @@ struct NetArgs {
     float    enc_extent[3];            // bbox extent * (1 + node_extent_dilation + fixed_node_dilation)
     float    thr_logit;                // visibility threshold on the logit (0 for 0.5)
     uint32_t max_steps;                // max traversal steps (segments <= max_steps + 1)
+    uint32_t image_width;              // tiled claim order of a scanline image (0: array order)
     const nbvh_node* nodes;            // N-BVH nodes with DILATED boxes (1 + 1e-5 about the centre)
     uint32_t n_nodes;
     uint32_t any_hit;
     const float*    tables_f32;        // [entries x F]
     const uint16_t* tables_f16;        // [entries x F] IEEE half of tables * table_scale
+    uint32_t tables_f16_bytes;         // size of tables_f16 (< 2^31: buffer-resource range)
     float    table_scale;              // power of two
     float    inv_table_scale;
     const float*    mlp_f32;           // row-major matrices, concatenated

@@ -75,6 +75,18 @@ def test_device_query_on_stream_matches_host_query(nb, orc, bench):
     s.destroy()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_tiled_claim_order_same_results(nb, orc, bench, precision):
+    # The image-width hint only changes which wave traces which ray.
+    prec = {"fp32": nb.PREC_FP32, "fp16": nb.PREC_FP16}[precision]
+    _, rays = camera_rays(nb, orc, 96, 64)
+    a = bench["module"].neural_bvh_traversal(rays, precision=prec)
+    b = bench["module"].neural_bvh_traversal(rays, precision=prec, image_width=96)
+    assert hits_equal(a, b)
+    c = bench["module"].neural_bvh_traversal(rays[:96 * 60], precision=prec, image_width=96)  # 60 % 8: ignored
+    assert hits_equal(a[:96 * 60], c)
+
+
 @pytest.mark.parametrize("any_hit", [False, True])
 def test_triangle_query_bit_exact(nb, orc, bench, any_hit):
     _, rays = camera_rays(nb, orc, 256, 256)
