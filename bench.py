@@ -34,6 +34,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(precision, width, height):
+    """HBM bytes per launch of the query kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (counter readings scaled by the same counters' readings for a
+    device copy of known size); None when no summary covers this precision and frame size."""
+    path = os.path.join(ROOT, "profiles", f"neural_query_{precision}_{width}x{height}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +135,8 @@ def main():
     alg_bytes = segs * 3 * 16 * 8 * 2 * tb + n_rays * (32 + 48)
     gbs = alg_bytes / (ms * 1e-3) / 1e9
     result["roofline"] = {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                          "frac": round(gbs / HBM_PEAK_GBS, 5),
+                          "traffic": measured_traffic(args.precision, W, H),
                           "bytes_per_launch": alg_bytes,
                           "mfma_tflops": round(segs * FLOP_PER_SEGMENT / (ms * 1e-3) / 1e12, 3)}
 
