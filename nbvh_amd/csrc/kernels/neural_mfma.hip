@@ -302,7 +302,11 @@ __device__ __forceinline__ void stage_weights(const NetArgs& a, uint4* wfrag)
     __syncthreads();
 }
 
-constexpr int kWavesPerBlock = 4;
+// Waves per workgroup of the query kernel; the workgroup shares one LDS copy of the weights.
+#ifndef NBVH_MFMA_BLOCK_WAVES
+#define NBVH_MFMA_BLOCK_WAVES 4
+#endif
+constexpr int kWavesPerBlock = NBVH_MFMA_BLOCK_WAVES;
 // Minimum waves per SIMD the register allocation must allow (latency hiding of the gathers).
 #ifndef NBVH_MFMA_WAVES_PER_EU
 #define NBVH_MFMA_WAVES_PER_EU 2
