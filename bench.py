@@ -47,6 +47,150 @@ def measured_traffic(precision, width, height):
     return None
 
 
+def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
+    """Continuous-batching serving run over three resident scenes.
+
+    Scenes: the 81,920-triangle icosphere (N-BVH 256), a field of 64 geodesic spheres (2.17M
+    triangles, N-BVH 4096) and one geodesic sphere of frequency 648 (8.4M triangles, N-BVH
+    16384), each with its own seeded FP16 network of the headline shape. Requests arrive as a
+    Poisson process; sizes are lognormal (median 65,536 rays, sigma 1) clamped to [4,096, 2M];
+    70 % closest-hit, 30 % any-hit (shadow) neural queries; the scene is drawn uniformly. Rays
+    are windows of a 2048x1024 camera frame of the request's scene, resident in HBM; hits go to
+    a device ring (zero-copy device requests). The arrival rate is set to `load` times the
+    measured single-launch capacity of the mix. Latency is submit -> hits complete, from the
+    server's own steady-clock timestamps."""
+    import threading
+    rng = np.random.default_rng(seed)
+    log("serving: building the 2.17M- and 8.4M-triangle scenes")
+    t0 = time.perf_counter()
+    scenes = [scene_a]
+    for verts_tris, leaves, net_seed in ((nb.sphere_field(64, seed=5), 4096, 2),
+                                         (nb.geosphere(648, seed=9, amplitude=0.1), 16384, 3)):
+        sc = nb.Scene(ctx, *verts_tris)
+        sc.build_nbvh(leaves)
+        sc.init_network(nb.network_config(), seed=net_seed, table_range=1e-4)
+        scenes.append(sc)
+    log(f"serving: scenes ready in {time.perf_counter() - t0:.1f} s")
+    W, H = 2048, 1024
+    pool = []
+    for sc in scenes:
+        lo, hi = sc.bbox()
+        c = (lo + hi) * 0.5
+        ext = float(np.max(hi - lo))
+        cam = nb.camera_look_at(tuple(c + np.array([0.3, 0.4, 1.0]) * 1.6 * ext), tuple(c), (0, 1, 0), 45.0, W, H)
+        d = hip.Buffer(W * H * 32)
+        ctx.raygen(cam, W, H, d.ptr)
+        pool.append(d)
+    hip.synchronize()
+    ring_rays = 16 << 20
+    d_hits = hip.Buffer(ring_rays * 48)
+
+    def params(kind):
+        return nb.NeuralBVHModule.params(kind=kind, precision=nb.PREC_FP16)
+
+    # Capacity of the mix: one 1M-ray launch per scene and kind, timed with events.
+    s = hip.Stream()
+    cap = []
+    for sc, d in zip(scenes, pool):
+        mod = nb.NeuralBVHModule(ctx, sc)
+        for kind in (nb.QUERY_NEURAL_CLOSEST, nb.QUERY_NEURAL_ANY):
+            p = params(kind)
+            mod.query_device(p, d.ptr, 1 << 20, d_hits.ptr, stream=s.handle)
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record(s)
+            mod.query_device(p, d.ptr, 1 << 20, d_hits.ptr, stream=s.handle)
+            e1.record(s)
+            s.synchronize()
+            cap.append((1 << 20) / (e0.elapsed_ms(e1) * 1e-3))
+    s.destroy()
+    # Mix weights: scene uniform, 70/30 kinds -> time per ray is the weighted mean of 1/capacity.
+    sec_per_ray = np.mean([0.7 / cap[2 * i] + 0.3 / cap[2 * i + 1] for i in range(3)])
+    capacity = 1.0 / sec_per_ray
+
+    sizes = np.clip(rng.lognormal(np.log(65536), 1.0, 200000), 4096, 2 << 20).astype(np.int64)
+    mean_size = float(sizes.mean())
+    rate = load * capacity / mean_size
+    n_req = int(rate * seconds)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    reqs = []
+    for i in range(n_req):
+        n = int(sizes[i])
+        sc = int(rng.integers(3))
+        kind = nb.QUERY_NEURAL_CLOSEST if rng.random() < 0.7 else nb.QUERY_NEURAL_ANY
+        off = int(rng.integers(0, W * H - n + 1))
+        reqs.append((sc, kind, off, n))
+    log(f"serving: capacity of the mix {capacity / 1e6:.0f} Mrays/s, mean request {mean_size:.0f} rays, "
+        f"{rate:.0f} requests/s for {seconds:.1f} s ({n_req} requests, load {load})")
+
+    srv = nb.Server(ctx, n_streams=4, max_batch_rays=1 << 21)
+    # Warm-up: every scene x kind once through the server.
+    for i in range(3):
+        for kind in (nb.QUERY_NEURAL_CLOSEST, nb.QUERY_NEURAL_ANY):
+            srv.wait(srv.submit_device(scenes[i], params(kind), pool[i].ptr, 65536, d_hits.ptr))
+    tickets = []
+    cond = threading.Condition()
+    done = [False]
+    infos = []
+
+    def collector():
+        i = 0
+        while True:
+            with cond:
+                while i >= len(tickets) and not done[0]:
+                    cond.wait()
+                if i >= len(tickets):
+                    return
+                t, n = tickets[i]
+            _, info = srv.wait(t)
+            infos.append((info, n))
+            i += 1
+
+    th = threading.Thread(target=collector)
+    th.start()
+    ring = 0
+    ps = {k: params(k) for k in (nb.QUERY_NEURAL_CLOSEST, nb.QUERY_NEURAL_ANY)}
+    start = time.perf_counter()
+    due = start
+    for (sc, kind, off, n), gap in zip(reqs, gaps):
+        due += gap
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            if due - now > 2e-4:
+                time.sleep(due - now - 1e-4)
+        if ring + n > ring_rays:
+            ring = 0
+        t = srv.submit_device(scenes[sc], ps[kind], pool[sc].ptr + off * 32, n, d_hits.ptr + ring * 48)
+        ring += n
+        with cond:
+            tickets.append((t, n))
+            cond.notify()
+    with cond:
+        done[0] = True
+        cond.notify()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([(i["complete_ns"] - i["submit_ns"]) * 1e-6 for i, _ in infos])
+    t_first = min(i["submit_ns"] for i, _ in infos)
+    t_last = max(i["complete_ns"] for i, _ in infos)
+    rays = sum(n for _, n in infos)
+    mrays = rays / ((t_last - t_first) * 1e-9) / 1e6
+    for sc in scenes[1:]:
+        sc.close()
+    return {
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3), "mrays_per_s": round(mrays, 1), "requests": len(infos),
+        "rays": int(rays), "offered_requests_per_s": round(rate, 1), "load": load,
+        "mix_capacity_mrays_per_s": round(capacity / 1e6, 1),
+        "requests_per_launch": round(st["n_batched_requests"] / max(1, st["n_launches"]), 2),
+        "workload": "3 resident scenes (81,920 / 2.17M / 8.4M triangles; N-BVH 256 / 4096 / 16384), FP16, "
+                    "Poisson arrivals, lognormal sizes median 64k in [4k, 2M], 70% closest / 30% shadow any-hit, "
+                    "device-resident rays, 4 streams",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +201,9 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (continuous batching) run")
+    ap.add_argument("--serving-seconds", type=float, default=4.0)
+    ap.add_argument("--serving-load", type=float, default=0.7, help="offered load / capacity of the mix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device-to-device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -184,6 +331,13 @@ def main():
             log(f"cpu baseline: {cpu:.3f} Mrays/s on {oracle.num_threads()} threads")
     except ImportError as e:
         log(f"oracle unavailable ({e}); no flip rate / cpu baseline")
+
+    if not args.no_serving and prec != nb.PREC_FP32:
+        sv = serving_bench(nb, hip, ctx, scene, args.serving_seconds, args.serving_load)
+        result["serving"] = sv
+        result["p99_ms"] = sv["p99_ms"]
+        log(f"serving: p50 {sv['p50_ms']} ms, p99 {sv['p99_ms']} ms, {sv['mrays_per_s']} Mrays/s, "
+            f"{sv['requests']} requests, {sv['requests_per_launch']} requests per launch")
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

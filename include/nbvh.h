@@ -197,6 +197,11 @@ int32_t nbvh_memory_usage(const nbvh_ctx* ctx, uint64_t* used_bytes, uint64_t* b
 int32_t nbvh_mesh_icosphere(uint32_t subdivisions, uint64_t seed, float amplitude, const float center[3],
                             float radius, float* verts, uint32_t* n_verts, uint32_t* indices, uint32_t* n_tris);
 
+/* Geodesic sphere of frequency k: each icosahedron face split into a k x k triangle grid,
+ * 20*k^2 triangles, 10*k^2+2 vertices, displaced like nbvh_mesh_icosphere. */
+int32_t nbvh_mesh_geosphere(uint32_t frequency, uint64_t seed, float amplitude, const float center[3],
+                            float radius, float* verts, uint32_t* n_verts, uint32_t* indices, uint32_t* n_tris);
+
 /* Binned-SAH BVH2 over triangles (max_leaf_size primitives per leaf; reference uses 2,
  * include/cuda/cuda_backend.h:232). nodes needs 2*n_tris-1 entries, prim_indices n_tris. */
 int32_t nbvh_bvh_build(const float* verts, uint32_t n_verts, const uint32_t* indices, uint32_t n_tris,
@@ -284,6 +289,68 @@ int32_t nbvh_network_inference(nbvh_ctx* ctx, const nbvh_scene* scene, uint32_t 
 /* Host-pointer variant of nbvh_network_inference; synchronous. */
 int32_t nbvh_network_inference_host(nbvh_ctx* ctx, const nbvh_scene* scene, uint32_t precision, const float* inputs,
                                     uint32_t n, float* outputs, float* logits);
+
+/* ---------------------------------------------------------------------------------------------
+ * Serving: continuous batching of ray-query requests over device-resident scenes.
+ *
+ * Replaces the reference's one-frame-at-a-time driver (NeuralPathModule::neural_bvh_traversal,
+ * src/cuda/neural/path/path_module.cu:54-137, a single host thread with blocking syncs) for
+ * callers that issue many independent ray batches. Any thread may submit; a scheduler thread
+ * keeps up to n_streams launches in flight. Each launch takes the oldest queued request and
+ * every later queued request with the same scene and query parameters (kind, precision, lod,
+ * max_traversal_steps, visibility_threshold), up to 64 requests and max_batch_rays rays, and
+ * runs them as one persistent kernel launch. A request's hits do not depend on what it was
+ * batched with. Requests with host buffers are staged through a device arena of staging_bytes
+ * (80 B per ray) and are admitted when their share fits; device-buffer requests are zero-copy.
+ * The scene must not be modified while requests on it are queued or running.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct nbvh_server nbvh_server;
+
+typedef struct nbvh_server_config {
+    uint32_t n_streams;      /* launches in flight at once, one HIP stream each, 1..16; 0 => 4 */
+    uint32_t max_batch_rays; /* rays one launch collects from queued requests (a single larger
+                                request still runs, alone); 0 => 2^21 */
+    uint64_t staging_bytes;  /* device arena for host-buffer requests; 0 => 256 MiB */
+    uint32_t reserved[4];
+} nbvh_server_config;
+
+enum nbvh_request_flags {
+    NBVH_REQUEST_DEVICE_BUFFERS = 1u /* rays / hits are device pointers (no staging, no copies) */
+};
+
+typedef struct nbvh_request_info {
+    int32_t  status;      /* NBVH_OK when the hits are complete */
+    uint32_t n_launches;  /* launches the request ran in (1) */
+    uint32_t batch_parts; /* requests in its launch, itself included */
+    uint32_t reserved;
+    uint64_t submit_ns;   /* steady-clock timestamps (ns): submitted, */
+    uint64_t launch_ns;   /* its launch queued on a stream, */
+    uint64_t complete_ns; /* hits complete (copied back for host buffers) */
+} nbvh_request_info;
+
+typedef struct nbvh_server_stats {
+    uint64_t n_submitted;
+    uint64_t n_completed;
+    uint64_t n_failed;
+    uint64_t n_launches;
+    uint64_t n_batched_requests; /* sum over launches of requests per launch */
+    uint64_t n_rays;             /* rays submitted */
+    uint32_t n_pending;          /* queued, not launched */
+    uint32_t n_inflight;         /* launches on the device */
+} nbvh_server_stats;
+
+int32_t nbvh_server_create(nbvh_ctx* ctx, const nbvh_server_config* cfg, nbvh_server** out_server);
+/* Runs every queued request to completion, then frees the server. */
+int32_t nbvh_server_destroy(nbvh_server* server);
+/* Queue a request; *ticket identifies it for nbvh_request_wait. The buffers must stay valid until
+ * the request completes. NBVH_ERR_CAPACITY: a host-buffer request larger than the arena. */
+int32_t nbvh_request_submit(nbvh_server* server, const nbvh_scene* scene, const nbvh_query_params* params,
+                            const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits, uint32_t flags,
+                            uint64_t* ticket);
+/* Wait for a request (timeout_us < 0: no limit). NBVH_ERR_TIMEOUT leaves the ticket valid; any
+ * other return collects it. info (optional) receives its timestamps and outcome. */
+int32_t nbvh_request_wait(nbvh_server* server, uint64_t ticket, int64_t timeout_us, nbvh_request_info* info);
+int32_t nbvh_server_get_stats(nbvh_server* server, nbvh_server_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -17,8 +17,8 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    SEGMENT_EQUIDISTANT_INTERIOR)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
-__all__ = ["Context", "Scene", "NeuralBVHModule", "NbvhError", "network_config", "icosphere", "bvh_build",
-           "nbvh_cut", "nbvh_parse", "nbvh_serialize", "network_param_counts", "network_init_params",
+__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "NbvhError", "network_config", "icosphere", "bvh_build",
+           "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
            "camera_look_at", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
 
@@ -58,6 +58,37 @@ def icosphere(subdivisions, seed=0, amplitude=0.1, center=(0.0, 0.0, 0.0), radiu
     _check(L.nbvh_mesh_icosphere(subdivisions, seed, amplitude, _ptr(c), radius, _ptr(v), ctypes.byref(nv), _ptr(t),
                                  ctypes.byref(nt)))
     return v, t
+
+
+def geosphere(frequency, seed=0, amplitude=0.1, center=(0.0, 0.0, 0.0), radius=1.0):
+    """Displaced geodesic sphere of frequency k (20 k^2 triangles): (verts [n,3], tris [m,3])."""
+    L = _lib()
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    c = np.asarray(center, np.float32)
+    _check(L.nbvh_mesh_geosphere(frequency, seed, amplitude, _ptr(c), radius, None, ctypes.byref(nv), None,
+                                 ctypes.byref(nt)))
+    v = np.empty((nv.value, 3), np.float32)
+    t = np.empty((nt.value, 3), np.uint32)
+    _check(L.nbvh_mesh_geosphere(frequency, seed, amplitude, _ptr(c), radius, _ptr(v), ctypes.byref(nv), _ptr(t),
+                                 ctypes.byref(nt)))
+    return v, t
+
+
+def sphere_field(n_spheres=64, seed=0, freq_range=(29, 50), amplitude=0.1, radius_range=(0.04, 0.12)):
+    """Procedural multi-object scene: n displaced geodesic spheres with seeded frequencies
+    (20 k^2 triangles each), radii and centres in the unit cube, merged into one mesh.
+    The defaults give 64 spheres of 16.8k-50k triangles, 2.1M triangles in total."""
+    rng = np.random.default_rng(seed)
+    vs, ts, off = [], [], 0
+    for i in range(n_spheres):
+        k = int(rng.integers(freq_range[0], freq_range[1] + 1))
+        r = float(rng.uniform(*radius_range))
+        c = rng.uniform(r, 1.0 - r, 3)
+        v, t = geosphere(k, seed=int(rng.integers(1 << 62)), amplitude=amplitude, center=c, radius=r)
+        vs.append(v)
+        ts.append(t + np.uint32(off))
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
 
 
 def bvh_build(verts, tris, max_leaf_size=2):
@@ -287,3 +318,68 @@ class NeuralBVHModule:
         self.ctx.check(_lib().nbvh_network_inference(self.ctx.handle, self.scene.handle, precision,
                                                      ctypes.c_void_p(d_in_ptr), n, ctypes.c_void_p(d_out_ptr),
                                                      ctypes.c_void_p(d_logits_ptr or 0), ctypes.c_void_p(stream or 0)))
+
+
+class Server:
+    """Continuous-batching request loop (nbvh_server_*): any thread submits ray batches against
+    resident scenes and waits on tickets; queued requests with the same scene and parameters share
+    launches, up to n_streams launches overlap on the GPU, host-buffer requests are admitted by
+    the staging arena's capacity."""
+
+    def __init__(self, ctx, n_streams=4, max_batch_rays=1 << 21, staging_bytes=256 << 20):
+        self.ctx = ctx
+        cfg = _ffi.ServerConfig(n_streams, max_batch_rays, staging_bytes)
+        h = ctypes.c_void_p()
+        ctx.check(_lib().nbvh_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._keep = {}  # ticket -> host arrays kept alive until the request is collected
+
+    def close(self):
+        if self.handle:
+            _lib().nbvh_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, scene, params, rays, hits=None):
+        """Host request: rays (RAY_DTYPE array) -> a ticket; the hits array (allocated here unless
+        given) is filled when the request completes and returned by wait()."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        if hits is None:
+            hits = np.zeros(len(rays), HIT_DTYPE)
+        t = ctypes.c_uint64()
+        self.ctx.check(_lib().nbvh_request_submit(self.handle, scene.handle, ctypes.byref(params), _ptr(rays),
+                                                  len(rays), _ptr(hits), 0, ctypes.byref(t)))
+        self._keep[t.value] = (rays, hits)
+        return t.value
+
+    def submit_device(self, scene, params, d_rays_ptr, n, d_hits_ptr):
+        """Device request (zero-copy): n rays at d_rays_ptr, hit records to d_hits_ptr."""
+        t = ctypes.c_uint64()
+        self.ctx.check(_lib().nbvh_request_submit(self.handle, scene.handle, ctypes.byref(params),
+                                                  ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
+                                                  _ffi.REQUEST_DEVICE_BUFFERS, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_us=-1):
+        """Block until the request completes -> (hits or None for device requests, info dict).
+        Raises NbvhError (status 8) on timeout, leaving the ticket valid."""
+        info = _ffi.RequestInfo()
+        st = _lib().nbvh_request_wait(self.handle, ticket, timeout_us, ctypes.byref(info))
+        if st == _ffi.ERR_TIMEOUT:
+            raise NbvhError(st, "request not complete")
+        kept = self._keep.pop(ticket, None)
+        self.ctx.check(st)
+        d = {k: getattr(info, k) for k in ("status", "n_launches", "batch_parts", "submit_ns", "launch_ns",
+                                           "complete_ns")}
+        return (kept[1] if kept else None), d
+
+    def stats(self):
+        st = _ffi.ServerStats()
+        self.ctx.check(_lib().nbvh_server_get_stats(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}

@@ -67,6 +67,26 @@ class CreateInfo(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("reserved0", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("n_streams", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
+                ("staging_bytes", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("n_launches", ctypes.c_uint32), ("batch_parts", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("submit_ns", ctypes.c_uint64), ("launch_ns", ctypes.c_uint64),
+                ("complete_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("n_submitted", ctypes.c_uint64), ("n_completed", ctypes.c_uint64), ("n_failed", ctypes.c_uint64),
+                ("n_launches", ctypes.c_uint64), ("n_batched_requests", ctypes.c_uint64),
+                ("n_rays", ctypes.c_uint64), ("n_pending", ctypes.c_uint32), ("n_inflight", ctypes.c_uint32)]
+
+
+REQUEST_DEVICE_BUFFERS = 1
+
+
 class NbvhError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(f"nbvh status {status}: {message}")
@@ -93,6 +113,7 @@ SIGNATURES = {
     "nbvh_destroy": [P],
     "nbvh_memory_usage": [P, PU64, PU64],
     "nbvh_mesh_icosphere": [U32, U64, F32, P, F32, P, PU32, P, PU32],
+    "nbvh_mesh_geosphere": [U32, U64, F32, P, F32, P, PU32, P, PU32],
     "nbvh_bvh_build": [P, U32, P, U32, U32, P, U32, PU32, P],
     "nbvh_nbvh_cut": [P, U32, U32, P, U32, PU32],
     "nbvh_nbvh_parse": [P, SZ, P, U32, PU32],
@@ -118,6 +139,11 @@ SIGNATURES = {
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_network_inference": [P, P, U32, P, U32, P, P, P],
     "nbvh_network_inference_host": [P, P, U32, P, U32, P, P],
+    "nbvh_server_create": [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)],
+    "nbvh_server_destroy": [P],
+    "nbvh_request_submit": [P, P, ctypes.POINTER(QueryParams), P, U32, P, U32, PU64],
+    "nbvh_request_wait": [P, U64, ctypes.c_int64, ctypes.POINTER(RequestInfo)],
+    "nbvh_server_get_stats": [P, ctypes.POINTER(ServerStats)],
 }
 _RESTYPES = {"nbvh_abi_version": ctypes.c_uint32, "nbvh_status_string": ctypes.c_char_p,
              "nbvh_last_error": ctypes.c_char_p}

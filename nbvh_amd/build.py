@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnbvh_hip.so")
+DRIVER = os.path.join(HERE, "nbvh_driver")
+ROOT = os.path.dirname(HERE)
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
@@ -59,6 +61,13 @@ def build(verbose=False, jobs=None):
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # C++ host driver over include/nbvh.hpp, linked against the library just built.
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(ROCM, "include"), os.path.join(CSRC, "driver", "nbvh_driver.cpp"), "-o", tmp,
+          "-L" + HERE, "-lnbvh_hip", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, DRIVER)
     return LIB
 
 

@@ -418,5 +418,47 @@ __device__ __forceinline__ void store_hit(nbvh_hit* hits, uint32_t i, const nbvh
     p[2] = make_float4(h.albedo[0], h.albedo[1], h.albedo[2], h.reserved);
 }
 
+// Part of claim index i of a multi-part launch: the last part whose begin <= i (parts sorted).
+__device__ __forceinline__ uint32_t batch_part(const RayBatch& b, uint32_t i)
+{
+    uint32_t lo = 0, hi = b.n_parts;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.parts[mid].begin <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Claim index -> (part, index within the part). PARTS: the launch has a part table (a
+// compile-time switch, so single-array launches carry no part bookkeeping).
+template <bool PARTS>
+__device__ __forceinline__ void batch_locate(const RayBatch& b, uint32_t i, uint32_t& part, uint32_t& local)
+{
+    if constexpr (PARTS) {
+        part  = batch_part(b, i);
+        local = i - b.parts[part].begin;
+    } else {
+        part  = 0;
+        local = i;
+    }
+}
+
+template <bool PARTS>
+__device__ __forceinline__ const nbvh_ray* batch_rays(const RayBatch& b, uint32_t part)
+{
+    if constexpr (PARTS) return b.parts[part].rays;
+    return b.rays;
+}
+
+template <bool PARTS>
+__device__ __forceinline__ nbvh_hit* batch_hits(const RayBatch& b, uint32_t part)
+{
+    if constexpr (PARTS) return b.parts[part].hits;
+    return b.hits;
+}
+
 }  // namespace dev
 }  // namespace nbvh

@@ -45,7 +45,8 @@ enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DON
 
 struct Slot {
     uint32_t state;
-    uint32_t ray_id;
+    uint32_t part;    // part of a multi-part launch (0 for a single array)
+    uint32_t ray_id;  // index within the part
     Ray ray;
     float best_t;
     float seg_mint, seg_maxt;
@@ -55,7 +56,8 @@ struct Slot {
     float nrm[3], alb[3];
 };
 
-__device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, nbvh_hit* __restrict__ hits)
+template <bool PARTS>
+__device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, const RayBatch& b)
 {
     nbvh_hit h;
     h.t       = (s.kind != 0u && !a.any_hit) ? s.best_t : FLT_MAX;
@@ -70,7 +72,7 @@ __device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, nbvh_hit* 
     h.albedo[1] = s.alb[1];
     h.albedo[2] = s.alb[2];
     h.reserved  = 0.f;
-    store_hit(hits, s.ray_id, h);
+    store_hit(batch_hits<PARTS>(b, s.part), s.ray_id, h);
     s.state = SLOT_EMPTY;
 }
 
@@ -123,8 +125,8 @@ struct RayQueue {
 };
 
 // Phase A. Returns false when the wave has no segment left to evaluate (all slots DONE).
-__device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& st, const nbvh_ray* __restrict__ rays,
-                                           RayQueue& q, nbvh_hit* __restrict__ hits)
+template <bool PARTS>
+__device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& st, const RayBatch& b, RayQueue& q)
 {
     const uint32_t lane = threadIdx.x & 63u;
     while (true) {
@@ -143,8 +145,8 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
                 if (s.state == SLOT_EMPTY) {
                     const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                     if (r < hi - lo) {
-                        s.state  = SLOT_NEW;
-                        s.ray_id = q.ray_of(lo + r);
+                        s.state = SLOT_NEW;
+                        batch_locate<PARTS>(b, q.ray_of(lo + r), s.part, s.ray_id);
                     } else {
                         failed = true;
                     }
@@ -157,7 +159,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
         }
         if (s.state == SLOT_NEW) {
             s.state    = SLOT_RAY;
-            s.ray      = load_ray(rays, s.ray_id);
+            s.ray      = load_ray(batch_rays<PARTS>(b, s.part), s.ray_id);
             s.best_t   = s.ray.tmax;
             s.segments = 0;
             s.flags    = 0;
@@ -171,19 +173,19 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
         }
         if (s.state == SLOT_RAY) {
             if (st.empty()) {
-                finish_ray(a, s, hits);
+                finish_ray<PARTS>(a, s, b);
             } else {
                 const int r = nbvh_next_leaf(a.nodes, a.lod, s.ray, st, s.best_t, s.seg_mint, s.seg_maxt, s.seg_node);
                 if (r == 1) {
                     if (s.segments == a.max_steps + 1u) {
                         s.flags |= NBVH_FLAG_STEP_CAP;
-                        finish_ray(a, s, hits);
+                        finish_ray<PARTS>(a, s, b);
                     } else {
                         s.state = SLOT_SEG;
                     }
                 } else {
                     if (r < 0) s.flags |= NBVH_FLAG_STACK_OVERFLOW;
-                    finish_ray(a, s, hits);
+                    finish_ray<PARTS>(a, s, b);
                 }
             }
         }
@@ -193,7 +195,8 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
 }
 
 // Phase C: outputs z[0..7] (pre-sigmoid logits) of the slot's current segment.
-__device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const float z[8], nbvh_hit* __restrict__ hits)
+template <bool PARTS>
+__device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const float z[8], const RayBatch& b)
 {
     s.segments++;
     s.state = SLOT_RAY;
@@ -206,7 +209,7 @@ __device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const
             s.kind      = NBVH_HIT_NEURAL;
             s.hit_node  = s.seg_node;
             s.hit_logit = z[0];
-            finish_ray(a, s, hits);
+            finish_ray<PARTS>(a, s, b);
         }
         return;
     }

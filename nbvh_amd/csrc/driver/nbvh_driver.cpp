@@ -1,0 +1,162 @@
+// nbvh_driver -- C++ host driver of the neural ray query through include/nbvh.hpp.
+//
+// The reference drives its query from C++ (NeuralBVHRenderer -> NeuralPathModule); this is the
+// same path for a C++ caller of libnbvh_hip.so:
+//   nbvh_driver host                      host-side builders only (runs without a GPU)
+//   nbvh_driver query [options]           build the 1024x1024 icosphere workload, trace it, write
+//                                         the hit records (48 B each) to --out
+// options: --width W --height H --precision fp32|fp16|bf16 --kind closest|any|tri|tri-any
+//          --leaves K --seed S --out FILE --repeat R (prints ms per query)
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "nbvh.hpp"
+
+namespace {
+
+void hip_ok(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "%s: %s\n", what, hipGetErrorString(e));
+        std::exit(3);
+    }
+}
+
+int host_selftest()
+{
+    nbvh::Mesh m = nbvh::icosphere(3, 7, 0.1f);
+    std::vector<nbvh_node> nodes(2 * m.n_tris());
+    std::vector<uint32_t> prims(m.n_tris());
+    uint32_t nn = 0;
+    nbvh::check(nbvh_bvh_build(m.verts.data(), m.n_verts(), m.tris.data(), m.n_tris(), 2, nodes.data(),
+                               (uint32_t)nodes.size(), &nn, prims.data()));
+    std::vector<nbvh_node> cut(2 * 64);
+    uint32_t nc = 0;
+    nbvh::check(nbvh_nbvh_cut(nodes.data(), nn, 64, cut.data(), (uint32_t)cut.size(), &nc));
+    size_t size = 0;
+    nbvh::check(nbvh_nbvh_serialize(cut.data(), nc, nullptr, 0, &size));
+    std::vector<char> bytes(size);
+    nbvh::check(nbvh_nbvh_serialize(cut.data(), nc, bytes.data(), size, &size));
+    std::vector<nbvh_node> back(nc);
+    uint32_t nb = 0;
+    nbvh::check(nbvh_nbvh_parse(bytes.data(), size, back.data(), nc, &nb));
+    if (nb != nc || std::memcmp(back.data(), cut.data(), nc * sizeof(nbvh_node)) != 0) {
+        std::fprintf(stderr, "round trip mismatch\n");
+        return 1;
+    }
+    uint64_t nt = 0, nm = 0;
+    const nbvh_network_config cfg = nbvh::network_config();
+    nbvh::check(nbvh_network_param_counts(&cfg, &nt, &nm));
+    std::printf("host ok: %u triangles, %u BVH nodes, %u N-BVH nodes, %zu .nbvh bytes, %llu + %llu params\n",
+                m.n_tris(), nn, nc, size, (unsigned long long)nt, (unsigned long long)nm);
+    try {
+        nbvh::Context ctx;
+        std::printf("device: available\n");
+    } catch (const nbvh::Error& e) {
+        if (e.status() != NBVH_ERR_NO_DEVICE) throw;
+        std::printf("device: none (%s)\n", e.what());
+    }
+    return 0;
+}
+
+int query(int argc, char** argv)
+{
+    uint32_t W = 1024, H = 1024, leaves = 256, repeat = 0;
+    uint64_t seed = 1;
+    std::string prec = "fp16", kind = "closest", out;
+    for (int i = 2; i + 1 < argc; i += 2) {
+        const std::string a = argv[i], v = argv[i + 1];
+        if (a == "--width") W = (uint32_t)std::stoul(v);
+        else if (a == "--height") H = (uint32_t)std::stoul(v);
+        else if (a == "--precision") prec = v;
+        else if (a == "--kind") kind = v;
+        else if (a == "--leaves") leaves = (uint32_t)std::stoul(v);
+        else if (a == "--seed") seed = std::stoull(v);
+        else if (a == "--out") out = v;
+        else if (a == "--repeat") repeat = (uint32_t)std::stoul(v);
+        else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 2;
+        }
+    }
+    nbvh::QueryOptions o;
+    o.precision   = prec == "fp32" ? NBVH_PREC_FP32 : prec == "bf16" ? NBVH_PREC_BF16 : NBVH_PREC_FP16;
+    o.kind        = kind == "any" ? NBVH_QUERY_NEURAL_ANY
+                  : kind == "tri" ? NBVH_QUERY_TRIANGLE_CLOSEST
+                  : kind == "tri-any" ? NBVH_QUERY_TRIANGLE_ANY : NBVH_QUERY_NEURAL_CLOSEST;
+    o.image_width = W;
+
+    nbvh::Context ctx;
+    nbvh::Mesh m = nbvh::icosphere(6, 7, 0.1f);
+    nbvh::Scene scene(ctx, m);
+    scene.build_neural_bvh(leaves);
+    scene.init_network(nbvh::network_config(), seed, 1e-4f);
+    nbvh::NeuralBVHModule module(ctx, scene);
+
+    const float eye[3] = {0, 0, 3}, target[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    const nbvh_camera cam = nbvh::camera_look_at(eye, target, up, 45.f, W, H);
+    const uint32_t n = W * H;
+    nbvh_ray* d_rays = nullptr;
+    nbvh_hit* d_hits = nullptr;
+    hip_ok(hipMalloc((void**)&d_rays, (size_t)n * sizeof(nbvh_ray)), "hipMalloc rays");
+    hip_ok(hipMalloc((void**)&d_hits, (size_t)n * sizeof(nbvh_hit)), "hipMalloc hits");
+    hipStream_t stream;
+    hip_ok(hipStreamCreate(&stream), "hipStreamCreate");
+    ctx.raygen(cam, W, H, d_rays, 0, 0, stream);
+    module.neural_bvh_traversal(d_rays, n, d_hits, o, stream);
+    hip_ok(hipStreamSynchronize(stream), "sync");
+    if (repeat) {
+        hipEvent_t e0, e1;
+        hip_ok(hipEventCreate(&e0), "event");
+        hip_ok(hipEventCreate(&e1), "event");
+        hip_ok(hipEventRecord(e0, stream), "record");
+        for (uint32_t r = 0; r < repeat; r++) module.neural_bvh_traversal(d_rays, n, d_hits, o, stream);
+        hip_ok(hipEventRecord(e1, stream), "record");
+        hip_ok(hipEventSynchronize(e1), "sync");
+        float ms = 0.f;
+        hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        std::printf("%s %s: %.4f ms per query, %.1f Mrays/s\n", prec.c_str(), kind.c_str(), ms / repeat,
+                    n / (ms / repeat * 1e-3) / 1e6);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    std::vector<nbvh_hit> hits(n);
+    hip_ok(hipMemcpy(hits.data(), d_hits, (size_t)n * sizeof(nbvh_hit), hipMemcpyDeviceToHost), "D2H");
+    uint64_t neural = 0;
+    for (const auto& h : hits) neural += NBVH_HIT_KIND(h.info) != NBVH_HIT_NONE;
+    std::printf("%u rays, %llu hits\n", n, (unsigned long long)neural);
+    if (!out.empty()) {
+        std::ofstream f(out, std::ios::binary);
+        f.write(reinterpret_cast<const char*>(hits.data()), (std::streamsize)(hits.size() * sizeof(nbvh_hit)));
+    }
+    (void)hipStreamDestroy(stream);
+    (void)hipFree(d_rays);
+    (void)hipFree(d_hits);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: nbvh_driver host | query [options]\n");
+        return 2;
+    }
+    try {
+        if (std::strcmp(argv[1], "host") == 0) return host_selftest();
+        if (std::strcmp(argv[1], "query") == 0) return query(argc, argv);
+    } catch (const nbvh::Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    std::fprintf(stderr, "unknown command %s\n", argv[1]);
+    return 2;
+}

@@ -19,26 +19,10 @@
 #include <vector>
 
 #include "../kernels/launch.h"
-#include "host_common.h"
+#include "context.h"
 #include "mfma_layout.h"
 
-#define NBVH_API extern "C" __attribute__((visibility("default")))
-
-struct nbvh_ctx {
-    int device = 0;
-    int n_cus  = 0;
-    uint64_t budget = 0;
-    std::atomic<uint64_t> used{0};
-    mutable std::mutex mu;
-    std::string last_error;
-    uint32_t* d_counters = nullptr;  // ring of queue counters for concurrent launches
-    std::atomic<uint32_t> counter_next{0};
-    static constexpr uint32_t kCounterWords = 32;            // per query: 8 band heads, 16 B apart
-    static constexpr uint32_t kCounters     = 2048 * kCounterWords;
-};
-
 namespace nbvh {
-namespace {
 
 thread_local std::string t_last_error;
 
@@ -56,101 +40,41 @@ void hip_check(hipError_t e, const char* what)
     if (e != hipSuccess) fail(NBVH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-template <typename F>
-int32_t guarded(const nbvh_ctx* ctx, F&& f)
-{
-    try {
-        f();
-        return NBVH_OK;
-    } catch (const Error& e) {
-        set_error(ctx, e.what());
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        set_error(ctx, "host allocation failed");
-        return NBVH_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        set_error(ctx, e.what());
-        return NBVH_ERR_INVALID_ARGUMENT;
-    }
-}
-
-// Device buffer accounted against the context's HBM budget.
-struct DevBuf {
-    nbvh_ctx* ctx = nullptr;
-    void* ptr     = nullptr;
-    size_t bytes  = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (ptr) {
-            hipSetDevice(ctx->device);
-            hipFree(ptr);
-            ctx->used -= bytes;
-        }
-        ptr   = nullptr;
-        bytes = 0;
-    }
-    void alloc(nbvh_ctx* c, size_t n)
-    {
-        release();
-        ctx = c;
-        if (n == 0) return;
-        if (ctx->used.load() + n > ctx->budget)
-            fail(NBVH_ERR_OUT_OF_MEMORY, "HBM budget exceeded: " + std::to_string(ctx->used.load() + n) + " > " +
-                                             std::to_string(ctx->budget) + " bytes");
-        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        if (hipMalloc(&ptr, n) != hipSuccess) {
-            ptr = nullptr;
-            (void)hipGetLastError();
-            fail(NBVH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(n) + " bytes failed");
-        }
-        bytes = n;
-        ctx->used += n;
-    }
-    template <typename T>
-    void upload(nbvh_ctx* c, const T* src, size_t count)
-    {
-        alloc(c, count * sizeof(T));
-        if (count) hip_check(hipMemcpy(ptr, src, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(ptr); }
-};
-
-}  // namespace
-}  // namespace nbvh
-
-using namespace nbvh;
-
-struct nbvh_scene {
-    nbvh_ctx* ctx = nullptr;
-    // host copies
-    std::vector<float> verts;
-    std::vector<uint32_t> tris;
-    std::vector<nbvh_node> bvh;
-    std::vector<uint32_t> prims;
-    std::vector<nbvh_node> nbvh_nodes;
-    float bbox_min[3] = {0, 0, 0}, bbox_max[3] = {0, 0, 0};
-    bool has_bbox = false, has_net = false;
-    nbvh_network_config cfg{};
-    std::vector<float> tables, mlp;
-    // device
-    DevBuf d_verts, d_tris, d_bvh, d_prims, d_nbvh, d_tables32, d_tables16, d_mlp32, d_frag, d_frag8;
-    NetArgs args{};
-    float table_scale = 1.f;
-    float fp8_wscale[8] = {1, 1, 1, 1, 1, 1, 1, 1};
-};
-
-namespace nbvh {
-namespace {
-
 void require(bool cond, const char* msg)
 {
     if (!cond) fail(NBVH_ERR_INVALID_ARGUMENT, msg);
 }
+
+void DevBuf::release()
+{
+    if (ptr) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipFree(ptr);
+        ctx->used -= bytes;
+    }
+    ptr   = nullptr;
+    bytes = 0;
+}
+
+void DevBuf::alloc(nbvh_ctx* c, size_t n)
+{
+    release();
+    ctx = c;
+    if (n == 0) return;
+    if (ctx->used.load() + n > ctx->budget)
+        fail(NBVH_ERR_OUT_OF_MEMORY, "HBM budget exceeded: " + std::to_string(ctx->used.load() + n) + " > " +
+                                         std::to_string(ctx->budget) + " bytes");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (hipMalloc(&ptr, n) != hipSuccess) {
+        ptr = nullptr;
+        (void)hipGetLastError();
+        fail(NBVH_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+    ctx->used += n;
+}
+
+namespace {
 
 // MLP layout consumed by the FP32 kernels: layers 1, 3, 5 row-major, layers 2, 4 transposed.
 std::vector<float> fp32_kernel_layout(const nbvh_network_config& cfg, const std::vector<float>& mlp)
@@ -258,18 +182,23 @@ uint32_t counter_slot(nbvh_ctx* ctx)
     return (ctx->counter_next.fetch_add(1) % slots) * nbvh_ctx::kCounterWords;
 }
 
-void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays, uint32_t n,
-               nbvh_hit* d_hits, hipStream_t stream)
+}  // namespace
+
+void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+               hipStream_t stream)
 {
     require(p != nullptr, "query params are NULL");
+    const uint32_t n = batch.n;
     if (n == 0) return;
-    require(d_rays && d_hits, "ray or hit buffer is NULL");
+    require(batch.parts != nullptr || (batch.rays && batch.hits), "ray or hit buffer is NULL");
+    require(batch.parts == nullptr || (batch.n_parts >= 1 && batch.n_parts <= kMaxBatchParts),
+            "batch part count out of range");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (p->kind == NBVH_QUERY_TRIANGLE_CLOSEST || p->kind == NBVH_QUERY_TRIANGLE_ANY) {
         if (s->bvh.empty()) fail(NBVH_ERR_STATE, "triangle query on a scene without triangles");
         TriArgs t{s->d_verts.as<float>(), s->d_tris.as<uint32_t>(), s->d_bvh.as<nbvh_node>(), s->d_prims.as<uint32_t>(),
                   p->kind == NBVH_QUERY_TRIANGLE_ANY ? 1u : 0u};
-        hip_check(query_triangles(t, d_rays, n, d_hits, stream), "triangle query launch");
+        hip_check(query_triangles(t, batch, stream), "triangle query launch");
         return;
     }
     require(p->kind <= NBVH_QUERY_TRIANGLE_ANY, "unknown query kind");
@@ -286,17 +215,19 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
     // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
     const uint64_t w = p->image_width;
-    a.image_width = (w != 0 && w % 8 == 0 && (uint64_t)n % (8 * w) == 0) ? (uint32_t)w : 0u;
+    a.image_width = (batch.parts == nullptr && w != 0 && w % 8 == 0 && (uint64_t)n % (8 * w) == 0) ? (uint32_t)w : 0u;
     uint32_t* counter = ctx->d_counters + counter_slot(ctx);
     hip_check(zero_u32(counter, nbvh_ctx::kCounterWords, stream), "counter reset");
     bool supported = false;
     if (p->precision == NBVH_PREC_FP32)
-        hip_check(query_fp32(a, d_rays, n, d_hits, counter, (uint32_t)ctx->n_cus, stream, &supported), "fp32 query");
+        hip_check(query_fp32(a, batch, counter, (uint32_t)ctx->n_cus, stream, &supported), "fp32 query");
     else
-        hip_check(query_mfma(a, p->precision, d_rays, n, d_hits, counter, (uint32_t)ctx->n_cus, stream, &supported),
+        hip_check(query_mfma(a, p->precision, batch, counter, (uint32_t)ctx->n_cus, stream, &supported),
                   "mfma query");
     if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no query kernel for this network shape / precision");
 }
+
+namespace {
 
 void run_network(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision, const float* d_in, uint32_t n, float* d_out,
                  float* d_logits, hipStream_t stream)
@@ -316,6 +247,8 @@ void run_network(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision, const f
 
 }  // namespace
 }  // namespace nbvh
+
+using namespace nbvh;
 
 // =============================================================================================
 // extern "C"
@@ -381,9 +314,9 @@ NBVH_API int32_t nbvh_create(const nbvh_create_info* info, nbvh_ctx** out)
 NBVH_API int32_t nbvh_destroy(nbvh_ctx* ctx)
 {
     if (!ctx) return NBVH_OK;
-    hipSetDevice(ctx->device);
-    hipDeviceSynchronize();
-    if (ctx->d_counters) hipFree(ctx->d_counters);
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     delete ctx;
     return NBVH_OK;
 }
@@ -417,6 +350,29 @@ NBVH_API int32_t nbvh_mesh_icosphere(uint32_t subdivisions, uint64_t seed, float
         std::vector<uint32_t> t;
         icosphere(subdivisions, seed, amplitude, center ? center : zero, radius, v, t);
         if (v.size() != nv * 3 || t.size() != nt * 3) fail(NBVH_ERR_INVALID_ARGUMENT, "icosphere size mismatch");
+        std::memcpy(verts, v.data(), v.size() * sizeof(float));
+        std::memcpy(indices, t.data(), t.size() * sizeof(uint32_t));
+    });
+}
+
+NBVH_API int32_t nbvh_mesh_geosphere(uint32_t frequency, uint64_t seed, float amplitude, const float center[3],
+                                     float radius, float* verts, uint32_t* n_verts, uint32_t* indices,
+                                     uint32_t* n_tris)
+{
+    return guarded(nullptr, [&] {
+        require(frequency >= 1 && frequency <= 4096, "frequency must be in 1..4096");
+        require(n_verts && n_tris, "count outputs are NULL");
+        const uint64_t k2 = (uint64_t)frequency * frequency;
+        const uint64_t nt = 20 * k2, nv = 10 * k2 + 2;
+        *n_verts = (uint32_t)nv;
+        *n_tris  = (uint32_t)nt;
+        if (!verts && !indices) return;
+        require(verts && indices, "verts and indices must both be given");
+        const float zero[3] = {0, 0, 0};
+        std::vector<float> v;
+        std::vector<uint32_t> t;
+        geosphere(frequency, seed, amplitude, center ? center : zero, radius, v, t);
+        if (v.size() != nv * 3 || t.size() != nt * 3) fail(NBVH_ERR_INVALID_ARGUMENT, "geosphere size mismatch");
         std::memcpy(verts, v.data(), v.size() * sizeof(float));
         std::memcpy(indices, t.data(), t.size() * sizeof(uint32_t));
     });
@@ -549,8 +505,8 @@ NBVH_API int32_t nbvh_scene_create_empty(nbvh_ctx* ctx, nbvh_scene** out)
 NBVH_API int32_t nbvh_scene_destroy(nbvh_scene* s)
 {
     if (!s) return NBVH_OK;
-    hipSetDevice(s->ctx->device);
-    hipDeviceSynchronize();
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipDeviceSynchronize();
     delete s;
     return NBVH_OK;
 }
@@ -726,7 +682,7 @@ NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query
     return guarded(ctx, [&] {
         require(ctx && s, "NULL argument");
         require(s->ctx == ctx, "scene belongs to another context");
-        run_query(ctx, s, p, d_rays, n, d_hits, (hipStream_t)stream);
+        run_query(ctx, s, p, RayBatch{d_rays, d_hits, nullptr, 0, n}, (hipStream_t)stream);
     });
 }
 
@@ -741,7 +697,7 @@ NBVH_API int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_
         DevBuf dr, dh;
         dr.upload(ctx, rays, n);
         dh.alloc(ctx, (size_t)n * sizeof(nbvh_hit));
-        run_query(ctx, s, p, dr.as<nbvh_ray>(), n, dh.as<nbvh_hit>(), nullptr);
+        run_query(ctx, s, p, RayBatch{dr.as<nbvh_ray>(), dh.as<nbvh_hit>(), nullptr, 0, n}, nullptr);
         hip_check(hipMemcpy(hits, dh.ptr, (size_t)n * sizeof(nbvh_hit), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     });
 }

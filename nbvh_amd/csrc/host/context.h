@@ -1,0 +1,107 @@
+// context.h -- the library's context and scene objects and the internal helpers shared by the
+// translation units behind the C ABI (api.cpp: scenes and queries; server.cpp: serving loop).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../nbvh_internal.h"
+#include "host_common.h"
+
+#define NBVH_API extern "C" __attribute__((visibility("default")))
+
+struct nbvh_ctx {
+    int device = 0;
+    int n_cus  = 0;
+    uint64_t budget = 0;
+    std::atomic<uint64_t> used{0};
+    mutable std::mutex mu;
+    std::string last_error;
+    uint32_t* d_counters = nullptr;  // ring of queue counters for concurrent launches
+    std::atomic<uint32_t> counter_next{0};
+    static constexpr uint32_t kCounterWords = 32;            // per query: 8 band heads, 16 B apart
+    static constexpr uint32_t kCounters     = 2048 * kCounterWords;
+};
+
+namespace nbvh {
+
+// Record the failure message on ctx (if any) and on the calling thread.
+void set_error(const nbvh_ctx* ctx, const std::string& msg);
+// Throws Error(NBVH_ERR_HIP) with the runtime's message when e != hipSuccess.
+void hip_check(hipError_t e, const char* what);
+// Throws Error(NBVH_ERR_INVALID_ARGUMENT, msg) when cond is false.
+void require(bool cond, const char* msg);
+
+// Runs f, turning exceptions into a status and the context's last-error message.
+template <typename F>
+int32_t guarded(const nbvh_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return NBVH_OK;
+    } catch (const Error& e) {
+        set_error(ctx, e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return NBVH_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(ctx, e.what());
+        return NBVH_ERR_INVALID_ARGUMENT;
+    }
+}
+
+// Device buffer accounted against the context's HBM budget.
+struct DevBuf {
+    nbvh_ctx* ctx = nullptr;
+    void* ptr     = nullptr;
+    size_t bytes  = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release();
+    void alloc(nbvh_ctx* c, size_t n);
+    template <typename T>
+    void upload(nbvh_ctx* c, const T* src, size_t count)
+    {
+        alloc(c, count * sizeof(T));
+        if (count) hip_check(hipMemcpy(ptr, src, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace nbvh
+
+struct nbvh_scene {
+    nbvh_ctx* ctx = nullptr;
+    // host copies
+    std::vector<float> verts;
+    std::vector<uint32_t> tris;
+    std::vector<nbvh_node> bvh;
+    std::vector<uint32_t> prims;
+    std::vector<nbvh_node> nbvh_nodes;
+    float bbox_min[3] = {0, 0, 0}, bbox_max[3] = {0, 0, 0};
+    bool has_bbox = false, has_net = false;
+    nbvh_network_config cfg{};
+    std::vector<float> tables, mlp;
+    // device
+    nbvh::DevBuf d_verts, d_tris, d_bvh, d_prims, d_nbvh, d_tables32, d_tables16, d_mlp32, d_frag, d_frag8;
+    nbvh::NetArgs args{};
+    float table_scale = 1.f;
+    float fp8_wscale[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+};
+
+namespace nbvh {
+
+// Queue one query launch (counter reset + kernel) over the rays of `batch` on `stream`.
+// Validates the parameters against the scene; throws Error on failure.
+void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+               hipStream_t stream);
+
+}  // namespace nbvh

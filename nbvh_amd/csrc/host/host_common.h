@@ -18,6 +18,8 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(int32_t status, const std::string& msg) { throw Error(status, msg); }
 
+void geosphere(uint32_t frequency, uint64_t seed, float amplitude, const float center[3], float radius,
+               std::vector<float>& verts, std::vector<uint32_t>& tris);
 void icosphere(uint32_t subdivisions, uint64_t seed, float amplitude, const float center[3], float radius,
                std::vector<float>& verts, std::vector<uint32_t>& tris);
 void camera_look_at(const float eye[3], const float target[3], const float up[3], float fovy_deg, uint32_t w,

@@ -59,12 +59,14 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
 
 // CudaBVH2::intersect<ShadowRay> (include/cuda/accel/bvh2.cuh:13-88): nearer child first, the
 // farther pushed; leaf primitives tested in order, closest hit shrinks tmax; any hit returns.
-__global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, const nbvh_ray* __restrict__ rays, uint32_t n,
-                                                        nbvh_hit* __restrict__ hits)
+template <bool PARTS>
+__global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, RayBatch b)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Ray ray = load_ray(rays, i);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n) return;
+    uint32_t part, i;
+    batch_locate<PARTS>(b, g, part, i);
+    const Ray ray = load_ray(batch_rays<PARTS>(b, part), i);
     const TriSetup s = tri_setup(ray);
     float tmax = ray.tmax;
     uint32_t stack[kTriStackSize];
@@ -133,7 +135,7 @@ done:
     h.logit     = 0.f;
     h.albedo[0] = h.albedo[1] = h.albedo[2] = 0.f;
     h.reserved  = 0.f;
-    store_hit(hits, i, h);
+    store_hit(batch_hits<PARTS>(b, part), i, h);
 }
 
 }  // namespace dev
@@ -152,9 +154,12 @@ hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t query_triangles(const TriArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, hipStream_t stream)
+hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::tri_query_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, rays, n, hits);
+    if (rb.parts)
+        hipLaunchKernelGGL(dev::tri_query_kernel<true>, dim3((rb.n + 255) / 256), dim3(256), 0, stream, a, rb);
+    else
+        hipLaunchKernelGGL(dev::tri_query_kernel<false>, dim3((rb.n + 255) / 256), dim3(256), 0, stream, a, rb);
     return hipGetLastError();
 }
 

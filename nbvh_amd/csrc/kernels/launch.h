@@ -15,18 +15,18 @@ hipError_t raygen(const nbvh_camera& cam, uint32_t w, uint32_t h, uint32_t sampl
 hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 
 // Triangle BVH closest / any hit.
-hipError_t query_triangles(const TriArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, hipStream_t stream);
+hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 
 // Neural queries (persistent grid sized from n_cus and the kernel's occupancy).
 // *supported = false when no kernel instance matches the network shape.
-hipError_t query_fp32(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, uint32_t* queue,
-                      uint32_t n_cus, hipStream_t stream, bool* supported);
+hipError_t query_fp32(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream,
+                      bool* supported);
 hipError_t network_fp32(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, hipStream_t stream,
                         bool* supported);
 
 // MFMA paths: precision = NBVH_PREC_FP16 / BF16 / FP8.
-hipError_t query_mfma(const NetArgs& a, uint32_t precision, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits,
-                      uint32_t* queue, uint32_t n_cus, hipStream_t stream, bool* supported);
+hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                      hipStream_t stream, bool* supported);
 hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
                         hipStream_t stream, bool* supported);
 

@@ -78,14 +78,13 @@ __device__ __forceinline__ void mlp_f32(const float* W_generic, const float* x, 
     }
 }
 
-template <int P, int L, int F>
+template <int P, int L, int F, bool PARTS>
 // Minimum waves per SIMD for the register allocation. Measured: forcing 2 halves the FP32
 // query's throughput (the allocator then starves the scalar weight prefetch), so 1 (no bound).
 #ifndef NBVH_FP32_WAVES_PER_EU
 #define NBVH_FP32_WAVES_PER_EU 1
 #endif
-__global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_kernel(NetArgs a, const nbvh_ray* __restrict__ rays,
-                                                               uint32_t n_rays, nbvh_hit* __restrict__ hits,
+__global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_kernel(NetArgs a, RayBatch b,
                                                                uint32_t* __restrict__ queue)
 {
     constexpr int K0 = P * L * F;
@@ -96,8 +95,8 @@ __global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, n_rays, a.image_width);
-    while (fill_slots(a, s, st, rays, q, hits)) {
+    q.init(queue, b.n, a.image_width);
+    while (fill_slots<PARTS>(a, s, st, b, q)) {
         float x[K0];
         if (s.state == SLOT_SEG) {
             float pts[3 * P];
@@ -109,7 +108,7 @@ __global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_
         }
         float z[8];
         mlp_f32<K0>(a.mlp_f32, x, h_lds, z);
-        if (s.state == SLOT_SEG) consume_segment(a, s, z, hits);
+        if (s.state == SLOT_SEG) consume_segment<PARTS>(a, s, z, b);
     }
 }
 
@@ -144,11 +143,11 @@ __global__ __launch_bounds__(64) void network_fp32_kernel(NetArgs a, const float
 
 }  // namespace dev
 
-template <int P, int L, int F>
-static hipError_t launch_fp32_query(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits,
-                                    uint32_t* queue, uint32_t n_cus, hipStream_t stream)
+template <int P, int L, int F, bool PARTS>
+static hipError_t launch_fp32_query_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                                      hipStream_t stream)
 {
-    auto k = dev::neural_query_fp32_kernel<P, L, F>;
+    auto k = dev::neural_query_fp32_kernel<P, L, F, PARTS>;
     static const uint32_t bpc = [&] {
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, 64, 0) != hipSuccess || b <= 0) {
@@ -157,10 +156,18 @@ static hipError_t launch_fp32_query(const NetArgs& a, const nbvh_ray* rays, uint
         }
         return (uint32_t)b;
     }();
-    uint32_t blocks = (n + 63) / 64;
+    uint32_t blocks = (rb.n + 63) / 64;
     blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(64), 0, stream, a, rays, n, hits, queue);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(64), 0, stream, a, rb, queue);
     return hipGetLastError();
+}
+
+template <int P, int L, int F>
+static hipError_t launch_fp32_query(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                                    hipStream_t stream)
+{
+    return rb.parts ? launch_fp32_query_k<P, L, F, true>(a, rb, queue, n_cus, stream)
+                    : launch_fp32_query_k<P, L, F, false>(a, rb, queue, n_cus, stream);
 }
 
 template <int P, int L, int F>
@@ -174,13 +181,13 @@ static hipError_t launch_fp32_network(const NetArgs& a, const float* in, uint32_
 
 #define NBVH_FP32_SHAPES(X) X(3, 16, 2) X(3, 8, 4) X(3, 2, 2) X(3, 4, 2)
 
-hipError_t query_fp32(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, uint32_t* queue,
-                      uint32_t n_cus, hipStream_t stream, bool* supported)
+hipError_t query_fp32(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream,
+                      bool* supported)
 {
     *supported = true;
 #define X(P_, L_, F_)                                                                                    \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_ && a.n_hidden_layers == 4)            \
-        return launch_fp32_query<P_, L_, F_>(a, rays, n, hits, queue, n_cus, stream);
+        return launch_fp32_query<P_, L_, F_>(a, rb, queue, n_cus, stream);
     NBVH_FP32_SHAPES(X)
 #undef X
     *supported = false;

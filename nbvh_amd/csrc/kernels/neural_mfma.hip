@@ -312,11 +312,9 @@ constexpr int kWavesPerBlock = NBVH_MFMA_BLOCK_WAVES;
 #define NBVH_MFMA_WAVES_PER_EU 2
 #endif
 
-template <int P, int L, int F, bool BF16>
+template <int P, int L, int F, bool BF16, bool PARTS>
 __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void neural_query_mfma_kernel(NetArgs a,
-                                                                                const nbvh_ray* __restrict__ rays,
-                                                                                uint32_t n_rays,
-                                                                                nbvh_hit* __restrict__ hits,
+                                                                                RayBatch b,
                                                                                 uint32_t* __restrict__ queue)
 {
     constexpr int NF = MlpShape<P * L * F>::N_FRAG;
@@ -328,8 +326,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, n_rays, a.image_width);
-    while (fill_slots(a, s, st, rays, q, hits)) {
+    q.init(queue, b.n, a.image_width);
+    while (fill_slots<PARTS>(a, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
             encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
@@ -339,7 +337,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
         }
         float z[8];
         mfma_network<P, L, F, BF16>(a, wfrag, own, z);
-        if (s.state == SLOT_SEG) consume_segment(a, s, z, hits);
+        if (s.state == SLOT_SEG) consume_segment<PARTS>(a, s, z, b);
     }
 }
 
@@ -381,17 +379,23 @@ uint32_t blocks_per_cu(K kernel, uint32_t threads)
     return (uint32_t)b;
 }
 
-template <int P, int L, int F, bool BF16>
-hipError_t launch_query(const NetArgs& a, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits, uint32_t* queue,
-                        uint32_t n_cus, hipStream_t stream)
+template <int P, int L, int F, bool BF16, bool PARTS>
+hipError_t launch_query_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
 {
-    auto k = dev::neural_query_mfma_kernel<P, L, F, BF16>;
+    auto k = dev::neural_query_mfma_kernel<P, L, F, BF16, PARTS>;
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
-    uint32_t blocks = (n + per_block - 1) / per_block;
+    uint32_t blocks = (rb.n + per_block - 1) / per_block;
     blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rays, n, hits, queue);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rb, queue);
     return hipGetLastError();
+}
+
+template <int P, int L, int F, bool BF16>
+hipError_t launch_query(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
+{
+    return rb.parts ? launch_query_k<P, L, F, BF16, true>(a, rb, queue, n_cus, stream)
+                    : launch_query_k<P, L, F, BF16, false>(a, rb, queue, n_cus, stream);
 }
 
 template <int P, int L, int F, bool BF16>
@@ -408,17 +412,17 @@ hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* 
 
 #define NBVH_MFMA_SHAPES(X) X(3, 16, 2) X(3, 8, 4)
 
-hipError_t query_mfma(const NetArgs& a, uint32_t precision, const nbvh_ray* rays, uint32_t n, nbvh_hit* hits,
-                      uint32_t* queue, uint32_t n_cus, hipStream_t stream, bool* supported)
+hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                      hipStream_t stream, bool* supported)
 {
     *supported = true;
     if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr && a.tables_f16_bytes != 0) {
 #define X(P_, L_, F_)                                                                 \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
         if (precision == NBVH_PREC_FP16)                                             \
-            return launch_query<P_, L_, F_, false>(a, rays, n, hits, queue, n_cus, stream); \
+            return launch_query<P_, L_, F_, false>(a, rb, queue, n_cus, stream);          \
         if (precision == NBVH_PREC_BF16)                                             \
-            return launch_query<P_, L_, F_, true>(a, rays, n, hits, queue, n_cus, stream);  \
+            return launch_query<P_, L_, F_, true>(a, rb, queue, n_cus, stream);           \
     }
         NBVH_MFMA_SHAPES(X)
 #undef X

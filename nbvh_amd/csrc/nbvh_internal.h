@@ -61,4 +61,25 @@ struct TriArgs {
     uint32_t         any_hit;
 };
 
+// One part of a multi-request launch: n rays read from `rays`, hit records written to `hits`,
+// occupying claim indices [begin, begin + n) of the launch.
+struct BatchPart {
+    const nbvh_ray* rays;
+    nbvh_hit*       hits;
+    uint32_t        begin;
+    uint32_t        n;
+};
+constexpr uint32_t kMaxBatchParts = 64;
+
+// The rays of one query launch. parts == nullptr: one array of n rays (rays -> hits).
+// Otherwise n_parts (1..kMaxBatchParts) parts in device memory, sorted by begin, parts[0].begin
+// == 0, laid end to end over [0, n).
+struct RayBatch {
+    const nbvh_ray*  rays;
+    nbvh_hit*        hits;
+    const BatchPart* parts;
+    uint32_t         n_parts;
+    uint32_t         n;
+};
+
 }  // namespace nbvh

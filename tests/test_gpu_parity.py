@@ -218,3 +218,22 @@ def test_errors_through_abi(nb, ctx):
         mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
     assert e.value.status == 7  # no MFMA kernel for K0 = 12
     scene.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_cpp_driver_matches_python_path(nb, orc, bench, tmp_path, precision):
+    """The C++ host driver (include/nbvh.hpp -> C ABI) produces the same hit records as the
+    Python host path for the same scene, camera and precision."""
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "nbvh_amd", "nbvh_driver")
+    out = tmp_path / "hits.bin"
+    r = subprocess.run([exe, "query", "--width", "256", "--height", "192", "--precision", precision, "--out",
+                        str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, nb.HIT_DTYPE)
+    _, rays = camera_rays(nb, orc, 256, 192)
+    prec = {"fp32": nb.PREC_FP32, "fp16": nb.PREC_FP16}[precision]
+    exp = bench["module"].neural_bvh_traversal(rays, precision=prec)
+    assert hits_equal(got, exp)
+    assert (nb.hit_kind(got) == nb.HIT_NEURAL).mean() > 0.2

@@ -57,6 +57,35 @@ def test_icosphere_counts_and_displacement(nb):
     assert np.all(counts == 2)
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 16])
+def test_geosphere_closed_oriented_surface(nb, k):
+    v, t = nb.geosphere(k, seed=1, amplitude=0.0)
+    assert len(t) == 20 * k * k and len(v) == 10 * k * k + 2
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+    assert len(np.unique(t)) == len(v)  # every vertex is used (shared edge points)
+    # Each directed edge appears once and its reverse once: closed and consistently oriented.
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]).astype(np.int64)
+    key = e[:, 0] * len(v) + e[:, 1]
+    rev = e[:, 1] * len(v) + e[:, 0]
+    assert len(np.unique(key)) == len(key) and np.array_equal(np.sort(key), np.sort(rev))
+    p = v[t].astype(np.float64)
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    assert np.all(np.einsum("ij,ij->i", n, p.mean(axis=1)) > 0)  # outward
+    # k = 1 is the icosahedron itself.
+    if k == 1:
+        v0, t0 = nb.icosphere(0, amplitude=0.0)
+        assert np.array_equal(v, v0) and np.array_equal(t, t0)
+
+
+def test_sphere_field_workload(nb):
+    v, t = nb.sphere_field(n_spheres=8, seed=3, freq_range=(4, 6))
+    assert len(t) % 20 == 0 and 8 * 20 * 16 <= len(t) <= 8 * 20 * 36
+    assert v.min() >= 0.0 and v.max() <= 1.0 + 1e-6
+    assert t.max() == len(v) - 1
+    v2, t2 = nb.sphere_field(n_spheres=8, seed=3, freq_range=(4, 6))
+    assert np.array_equal(v, v2) and np.array_equal(t, t2)
+
+
 def bvh_invariants(nodes, prims, v, t, max_leaf):
     n = len(nodes)
     assert sorted(prims.tolist()) == list(range(len(t)))
@@ -186,3 +215,14 @@ def test_context_without_gpu_reports_no_device(nb):
     with pytest.raises(nb.NbvhError) as e:
         nb.Context()
     assert e.value.status == 3
+
+
+def test_cpp_driver_host_side(nb):
+    """The C++ host interface (include/nbvh.hpp) driving the library: builders, .nbvh round trip,
+    and the no-device error path on a machine without a GPU."""
+    exe = os.path.join(ROOT, "nbvh_amd", "nbvh_driver")
+    assert os.path.exists(exe), "build with `python -m nbvh_amd.build`"
+    r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "host ok: 1280 triangles" in r.stdout
+    assert "N-BVH nodes" in r.stdout and "device:" in r.stdout
