@@ -47,6 +47,201 @@ def measured_traffic(precision, width, height):
     return None
 
 
+_EXTRA_SCENES = {}
+
+
+def sphere_field_scene(nb, ctx):
+    """64 displaced geodesic spheres (2.17M triangles, seeded placement in the unit cube),
+    N-BVH 4096 leaves, FP16-ready network of the headline shape (seed 2). Built once."""
+    if "field" not in _EXTRA_SCENES:
+        t0 = time.perf_counter()
+        sc = nb.Scene(ctx, *nb.sphere_field(64, seed=5))
+        sc.build_nbvh(4096)
+        sc.init_network(nb.network_config(), seed=2, table_range=1e-4)
+        _EXTRA_SCENES["field"] = sc
+        log(f"sphere-field scene built in {time.perf_counter() - t0:.1f} s")
+    return _EXTRA_SCENES["field"]
+
+
+def wavefront_rays(nb, verts, tris, n_closest, n_shadow, seed=21):
+    """n_closest incoherent rays (origins uniform in the unit cube, uniform directions) and
+    n_shadow shadow rays from uniformly chosen triangles (uniform barycentric points, offset 1e-4
+    along the face normal toward the light) to one of 8 seeded point lights, tmax = distance."""
+    rng = np.random.default_rng(seed)
+    rc = np.zeros(n_closest, nb.RAY_DTYPE)
+    d = rng.normal(size=(n_closest, 3))
+    rc["o"] = rng.uniform(0, 1, (n_closest, 3)).astype(np.float32)
+    rc["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rc["tmin"], rc["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
+    lights = rng.uniform(-0.25, 1.25, (8, 3))
+    tri = tris[rng.integers(0, len(tris), n_shadow)]
+    a, b, c = verts[tri[:, 0]].astype(np.float64), verts[tri[:, 1]].astype(np.float64), verts[tri[:, 2]].astype(np.float64)
+    u, v = rng.uniform(0, 1, n_shadow), rng.uniform(0, 1, n_shadow)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    p = a + (b - a) * u[:, None] + (c - a) * v[:, None]
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    L = lights[rng.integers(0, 8, n_shadow)]
+    side = np.sign(np.einsum("ij,ij->i", L - p, nrm))[:, None]
+    o = p + 1e-4 * side * nrm
+    to = L - o
+    dist = np.linalg.norm(to, axis=1)
+    rs = np.zeros(n_shadow, nb.RAY_DTYPE)
+    rs["o"] = o.astype(np.float32)
+    rs["d"] = (to / dist[:, None]).astype(np.float32)
+    rs["tmin"], rs["tmax"] = 1e-4, dist.astype(np.float32)
+    return rc, rs
+
+
+def wavefront_bench(nb, hip, ctx, steps):
+    """Batched wavefront: the sphere-field scene, 4M rays in ONE persistent launch of two parts
+    (2M incoherent closest-hit, 2M shadow any-hit to 8 lights), FP16 and FP32, eager and replayed
+    from a captured HIP graph; the triangle-BVH fallback on the same rays beside it."""
+    sc = sphere_field_scene(nb, ctx)
+    cfg_v, cfg_t = nb.sphere_field(64, seed=5)
+    n = 1 << 21
+    rc, rs = wavefront_rays(nb, cfg_v, cfg_t, n, n)
+    dc, ds = hip.Buffer.from_array(rc), hip.Buffer.from_array(rs)
+    hc, hs = hip.Buffer(n * 48), hip.Buffer(n * 48)
+    mod = nb.NeuralBVHModule(ctx, sc)
+    parts = [(dc.ptr, n, hc.ptr, False), (ds.ptr, n, hs.ptr, True)]
+    s = hip.Stream()
+
+    def time_it(fn, k):
+        fn()
+        s.synchronize()
+        ev = [hip.Event() for _ in range(k + 1)]
+        ev[0].record(s)
+        for i in range(k):
+            fn()
+            ev[i + 1].record(s)
+        s.synchronize()
+        return float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(k)]))
+
+    out = {"workload": "64 displaced geodesic spheres, 2.17M triangles, N-BVH 4096 leaves; 4,194,304 rays in one "
+                       "launch: 2M incoherent closest-hit + 2M shadow any-hit to 8 point lights",
+           "rays": 2 * n}
+    for name, prec in (("fp16", nb.PREC_FP16), ("fp32", nb.PREC_FP32)):
+        p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
+        ms = time_it(lambda: mod.query_parts_device(p, parts, stream=s.handle), steps if name == "fp16" else 3)
+        out[name] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
+        if name == "fp16":
+            h1, h2 = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
+            out["segments_per_ray"] = round(float((nb.hit_segments(h1).astype(np.int64).sum()
+                                                   + nb.hit_segments(h2).astype(np.int64).sum()) / (2 * n)), 3)
+            out["closest_hit_fraction"] = round(float((nb.hit_kind(h1) != 0).mean()), 4)
+            out["shadow_occluded_fraction"] = round(float((nb.hit_kind(h2) != 0).mean()), 4)
+            with hip.Graph.capture(s) as g:
+                mod.query_parts_device(p, parts, stream=s.handle)
+            gms = time_it(lambda: g.launch(s), steps)
+            g.destroy()
+            out["fp16_graph"] = {"ms": round(gms, 4), "mrays_per_s": round(2 * n / (gms * 1e-3) / 1e6, 1)}
+    pt = mod.params(kind=nb.QUERY_TRIANGLE_CLOSEST)
+    ms = time_it(lambda: mod.query_parts_device(pt, parts, stream=s.handle), 3)
+    out["triangle_fallback"] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
+    s.destroy()
+    return out
+
+
+def _sig(z):
+    return 1.0 / (1.0 + np.exp(-np.asarray(z, np.float64)))
+
+
+def output_errors(z, z64):
+    """Per-output error of network logits z against FP64 logits z64, in what the query uses:
+    visibility decision flips, |depth - depth64| (sigmoid output, LocalT fraction of the segment),
+    normal angle (degrees) and |albedo - albedo64| (sigmoid outputs)."""
+    s, s64 = _sig(z), _sig(z64)
+    n = 2 * s[:, 2:5] - 1
+    n64 = 2 * s64[:, 2:5] - 1
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    n64 /= np.maximum(np.linalg.norm(n64, axis=1, keepdims=True), 1e-30)
+    ang = np.degrees(np.arccos(np.clip(np.einsum("ij,ij->i", n, n64), -1, 1)))
+    dd = np.abs(s[:, 1] - s64[:, 1])
+    da = np.abs(s[:, 5:8] - s64[:, 5:8]).max(axis=1)
+    return {"vis_flip_rate": float(np.mean((z[:, 0] > 0) != (z64[:, 0] > 0))),
+            "depth_abs_p99": float(np.percentile(dd, 99)), "depth_abs_max": float(dd.max()),
+            "normal_deg_p99": float(np.percentile(ang, 99)), "albedo_abs_p99": float(np.percentile(da, 99))}
+
+
+def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, every=64):
+    """Precision workload: 1920x1080 primary rays (1 spp) and 4 diffuse bounces. Paths continue
+    from the FP16 query's hits through nbvh_scatter_diffuse (cosine-sampled directions from the
+    reference's LCG); rays that missed end. Per bounce: FP16 and FP32 query times over that
+    bounce's rays, and on every 64th ray the FP64 oracle's segments evaluated by the FP16 / BF16 /
+    FP8 networks -> per-output errors vs FP64; the FP16 query's hit/miss decisions vs the FP64
+    oracle query on the same rays."""
+    mod = nb.NeuralBVHModule(ctx, scene)
+    cfg, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = oracle.Net(cfg, tables, mlp, bmin, bmax)
+    nodes = scene.nbvh()
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    n = W * H
+    d_rays, d_hits, d_next = hip.Buffer(n * 32), hip.Buffer(n * 48), hip.Buffer(n * 32)
+    s = hip.Stream()
+    ctx.raygen(cam, W, H, d_rays.ptr, stream=s.handle)
+    s.synchronize()
+
+    def timed(prec, k, cur):
+        p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
+        mod.query_device(p, d_rays.ptr, cur, d_hits.ptr, stream=s.handle)
+        e = [hip.Event() for _ in range(k + 1)]
+        e[0].record(s)
+        for i in range(k):
+            mod.query_device(p, d_rays.ptr, cur, d_hits.ptr, stream=s.handle)
+            e[i + 1].record(s)
+        s.synchronize()
+        return float(np.mean([e[i].elapsed_ms(e[i + 1]) for i in range(k)]))
+
+    rows, worst = [], {}
+    cur = n
+    for b in range(bounces + 1):
+        ms32 = timed(nb.PREC_FP32, 2, cur)
+        ms16 = timed(nb.PREC_FP16, 5, cur)  # leaves the FP16 hits in d_hits
+        rays = d_rays.download(nb.RAY_DTYPE, cur)
+        hits = d_hits.download(nb.HIT_DTYPE, cur)
+        samp = rays[::every]
+        e64, log = net.neural_query(nodes, samp, mode=oracle.FP64, log_segments=True)
+        row = {"bounce": b, "rays": cur, "fp16_mrays_per_s": round(cur / (ms16 * 1e-3) / 1e6, 1),
+               "fp32_mrays_per_s": round(cur / (ms32 * 1e-3) / 1e6, 1),
+               "fp16_query_kind_mismatch": round(float(np.mean(nb.hit_kind(hits[::every]) != nb.hit_kind(e64))), 5),
+               "sample_segments": int(len(log))}
+        if len(log):
+            x = net.encode(samp[log["ray"]], log["mint"], log["maxt"], mode=oracle.FP32)
+            z64 = log["logit"].astype(np.float64)
+            for name, prec in (("fp16", nb.PREC_FP16), ("bf16", nb.PREC_BF16), ("fp8", nb.PREC_FP8)):
+                _, z = mod.network_inference(x, precision=prec)
+                err = output_errors(z, z64)
+                row[name] = {k: round(v, 5) for k, v in err.items()}
+                w = worst.setdefault(name, {k: 0.0 for k in err})
+                for k, v in err.items():
+                    w[k] = round(max(w[k], v), 5)
+        rows.append(row)
+        log_msg = f"path-traced bounce {b}: {cur} rays, fp16 {row['fp16_mrays_per_s']} / fp32 " \
+                  f"{row['fp32_mrays_per_s']} Mrays/s"
+        if "fp16" in row:
+            log_msg += f", flips fp16 {row['fp16']['vis_flip_rate']} bf16 {row['bf16']['vis_flip_rate']} " \
+                       f"fp8 {row['fp8']['vis_flip_rate']}"
+        log(log_msg)
+        if b == bounces:
+            break
+        mod.scatter_diffuse_device(d_rays.ptr, d_hits.ptr, cur, d_next.ptr, sample=0, bounce=b, stream=s.handle)
+        s.synchronize()
+        nxt = d_next.download(nb.RAY_DTYPE, cur)
+        nxt = nxt[nxt["tmax"] > 0]
+        cur = len(nxt)
+        if cur == 0:
+            break
+        d_rays.upload(nxt)
+    s.destroy()
+    return {"workload": f"{W}x{H} primary rays, 1 spp, {bounces} diffuse bounces from the FP16 query's hits "
+                        "(misses end the path), icosphere 81,920 tris, N-BVH 256, headline network; errors on "
+                        f"every {every}th ray's segments vs the FP64 oracle",
+            "per_bounce": rows, "worst_over_bounces": worst}
+
+
 def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     """Continuous-batching serving run over three resident scenes.
 
@@ -61,15 +256,12 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     server's own steady-clock timestamps."""
     import threading
     rng = np.random.default_rng(seed)
-    log("serving: building the 2.17M- and 8.4M-triangle scenes")
+    log("serving: building the 8.4M-triangle scene")
     t0 = time.perf_counter()
-    scenes = [scene_a]
-    for verts_tris, leaves, net_seed in ((nb.sphere_field(64, seed=5), 4096, 2),
-                                         (nb.geosphere(648, seed=9, amplitude=0.1), 16384, 3)):
-        sc = nb.Scene(ctx, *verts_tris)
-        sc.build_nbvh(leaves)
-        sc.init_network(nb.network_config(), seed=net_seed, table_range=1e-4)
-        scenes.append(sc)
+    big = nb.Scene(ctx, *nb.geosphere(648, seed=9, amplitude=0.1))
+    big.build_nbvh(16384)
+    big.init_network(nb.network_config(), seed=3, table_range=1e-4)
+    scenes = [scene_a, sphere_field_scene(nb, ctx), big]
     log(f"serving: scenes ready in {time.perf_counter() - t0:.1f} s")
     W, H = 2048, 1024
     pool = []
@@ -177,8 +369,7 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     t_last = max(i["complete_ns"] for i, _ in infos)
     rays = sum(n for _, n in infos)
     mrays = rays / ((t_last - t_first) * 1e-9) / 1e6
-    for sc in scenes[1:]:
-        sc.close()
+    big.close()
     return {
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "mean_ms": round(float(lat.mean()), 3), "mrays_per_s": round(mrays, 1), "requests": len(infos),
@@ -202,6 +393,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (continuous batching) run")
+    ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M mixed-ray wavefront run")
+    ap.add_argument("--no-path-traced", action="store_true", help="skip the 1920x1080 4-bounce precision run")
     ap.add_argument("--serving-seconds", type=float, default=4.0)
     ap.add_argument("--serving-load", type=float, default=0.7, help="offered load / capacity of the mix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -312,6 +505,10 @@ def main():
         result["vis_flip_sample_segments"] = int(len(x))
         log(f"visibility flips vs FP64 oracle: {flips:.5f} over {len(x)} segments")
 
+        if not args.no_path_traced and prec != nb.PREC_FP32:
+            pt = path_traced_bench(nb, hip, ctx, scene, oracle)
+            result["path_traced"] = pt
+
         if not args.no_cpu_baseline:
             # CPU baseline: the oracle's FP32 path (same traversal + hash grid + FP32 MLP), all
             # host threads OpenMP-parallel, on the full frame; repeated to >= 10 s of CPU time.
@@ -332,6 +529,12 @@ def main():
     except ImportError as e:
         log(f"oracle unavailable ({e}); no flip rate / cpu baseline")
 
+    if not args.no_wavefront and prec != nb.PREC_FP32:
+        wf = wavefront_bench(nb, hip, ctx, max(5, args.steps // 2))
+        result["wavefront"] = wf
+        log(f"wavefront: fp16 {wf['fp16']['mrays_per_s']} Mrays/s (graph {wf['fp16_graph']['mrays_per_s']}), "
+            f"fp32 {wf['fp32']['mrays_per_s']}, triangles {wf['triangle_fallback']['mrays_per_s']}")
+
     if not args.no_serving and prec != nb.PREC_FP32:
         sv = serving_bench(nb, hip, ctx, scene, args.serving_seconds, args.serving_load)
         result["serving"] = sv
@@ -350,6 +553,8 @@ def main():
         log(f"calibration copy: {nbytes} bytes device to device")
 
     print(json.dumps(result), flush=True)
+    for sc in _EXTRA_SCENES.values():
+        sc.close()
     scene.close()
     ctx.close()
 

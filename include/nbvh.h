@@ -272,9 +272,33 @@ int32_t nbvh_scene_get_network(const nbvh_scene* scene, nbvh_network_config* cfg
 int32_t nbvh_raygen(nbvh_ctx* ctx, const nbvh_camera* camera, uint32_t width, uint32_t height, uint32_t sample,
                     uint32_t frame, nbvh_ray* d_rays, void* stream);
 
+/* One diffuse bounce of a path: for every hit, the next ray from the hit point (offset 1e-4 along
+ * the shading normal) in a cosine-distributed direction about it (the reference's Lambertian
+ * sampler, material.cuh:101-120,193-201, path_inference_kernels.cu:380-393). Shading normal:
+ * the predicted normal of a neural hit, the geometric normal of a triangle hit, turned toward
+ * the incoming ray. Random numbers: the raygen LCG with frame = bounce + 1. Rays without a hit
+ * come out inactive with tmax = -1 (compact them before the next query). Device pointers. */
+int32_t nbvh_scatter_diffuse(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_ray* d_rays, const nbvh_hit* d_hits,
+                             uint32_t n, uint32_t sample, uint32_t bounce, nbvh_ray* d_next, void* stream);
+
 /* One query over n rays already in device memory; hit records to d_hits. */
 int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
                    uint32_t n_rays, nbvh_hit* d_hits, void* stream);
+
+/* One launch over several device ray arrays (parts), each closest-hit or any-hit: e.g. the
+ * closest-hit and shadow rays of a wavefront bounce in one persistent kernel. params->kind picks
+ * the family (neural or triangle) and its precision / lod / limits; each part's any_hit picks
+ * closest (0) or any (1) hit. Stream-ordered like nbvh_query; the part table is staged in a ring
+ * of 2048 per-context slots, so at most 2048 such calls may be in flight per context. */
+typedef struct nbvh_query_part {
+    const nbvh_ray* rays;    /* device pointer */
+    nbvh_hit*       hits;    /* device pointer */
+    uint32_t        n_rays;
+    uint32_t        any_hit; /* 0: closest hit, 1: any hit (shadow ray) */
+} nbvh_query_part;
+
+int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
+                         const nbvh_query_part* parts, uint32_t n_parts, void* stream);
 
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
@@ -297,9 +321,9 @@ int32_t nbvh_network_inference_host(nbvh_ctx* ctx, const nbvh_scene* scene, uint
  * src/cuda/neural/path/path_module.cu:54-137, a single host thread with blocking syncs) for
  * callers that issue many independent ray batches. Any thread may submit; a scheduler thread
  * keeps up to n_streams launches in flight. Each launch takes the oldest queued request and
- * every later queued request with the same scene and query parameters (kind, precision, lod,
- * max_traversal_steps, visibility_threshold), up to 64 requests and max_batch_rays rays, and
- * runs them as one persistent kernel launch. A request's hits do not depend on what it was
+ * every later queued request with the same scene, kind family (neural or triangle; closest and
+ * any hit mix) and query parameters (precision, lod, max_traversal_steps, visibility_threshold),
+ * up to 64 requests and max_batch_rays rays, and runs them as one persistent kernel launch. A request's hits do not depend on what it was
  * batched with. Requests with host buffers are staged through a device arena of staging_bytes
  * (80 B per ray) and are admitted when their share fits; device-buffer requests are zero-copy.
  * The scene must not be modified while requests on it are queued or running.

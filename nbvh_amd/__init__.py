@@ -304,6 +304,21 @@ class NeuralBVHModule:
                                          ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
                                          ctypes.c_void_p(stream or 0)))
 
+    def scatter_diffuse_device(self, d_rays_ptr, d_hits_ptr, n, d_next_ptr, sample=0, bounce=0, stream=None):
+        """Next rays of a diffuse bounce (nbvh_scatter_diffuse); rays without a hit get tmax = -1."""
+        self.ctx.check(_lib().nbvh_scatter_diffuse(self.ctx.handle, self.scene.handle, ctypes.c_void_p(d_rays_ptr),
+                                                   ctypes.c_void_p(d_hits_ptr), n, sample, bounce,
+                                                   ctypes.c_void_p(d_next_ptr), ctypes.c_void_p(stream or 0)))
+
+    def query_parts_device(self, params, parts, stream=None):
+        """One launch over several device ray arrays: parts = [(d_rays_ptr, n, d_hits_ptr, any_hit)].
+        params.kind selects the family (neural or triangle); any_hit is per part."""
+        arr = (_ffi.QueryPart * len(parts))()
+        for i, (r, n, h, any_hit) in enumerate(parts):
+            arr[i] = _ffi.QueryPart(r, h, n, 1 if any_hit else 0)
+        self.ctx.check(_lib().nbvh_query_parts(self.ctx.handle, self.scene.handle, ctypes.byref(params), arr,
+                                               len(parts), ctypes.c_void_p(stream or 0)))
+
     def network_inference(self, inputs, precision=PREC_FP32):
         """inputs float32 [n, 3P] -> (outputs [n, 8] post-sigmoid, logits [n, 8])."""
         x = np.ascontiguousarray(inputs, np.float32)

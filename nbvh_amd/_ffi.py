@@ -67,6 +67,11 @@ class CreateInfo(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("reserved0", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64)]
 
 
+class QueryPart(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p), ("n_rays", ctypes.c_uint32),
+                ("any_hit", ctypes.c_uint32)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("n_streams", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
                 ("staging_bytes", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 4)]
@@ -136,7 +141,9 @@ SIGNATURES = {
     "nbvh_scene_get_network": [P, ctypes.POINTER(NetworkConfig), P, U64, P, U64],
     "nbvh_raygen": [P, ctypes.POINTER(Camera), U32, U32, U32, U32, P, P],
     "nbvh_query": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
+    "nbvh_scatter_diffuse": [P, P, P, P, U32, U32, U32, P, P],
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
+    "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
     "nbvh_network_inference": [P, P, U32, P, U32, P, P, P],
     "nbvh_network_inference_host": [P, P, U32, P, U32, P, P],
     "nbvh_server_create": [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)],

@@ -460,5 +460,13 @@ __device__ __forceinline__ nbvh_hit* batch_hits(const RayBatch& b, uint32_t part
     return b.hits;
 }
 
+// Any-hit flag of a ray: per part in a multi-part launch, else the launch's.
+template <bool PARTS>
+__device__ __forceinline__ bool batch_any_hit(const RayBatch& b, uint32_t part, uint32_t launch_any_hit)
+{
+    if constexpr (PARTS) return b.parts[part].any_hit != 0u;
+    return launch_any_hit != 0u;
+}
+
 }  // namespace dev
 }  // namespace nbvh

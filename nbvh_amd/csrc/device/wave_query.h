@@ -47,6 +47,7 @@ struct Slot {
     uint32_t state;
     uint32_t part;    // part of a multi-part launch (0 for a single array)
     uint32_t ray_id;  // index within the part
+    bool any_hit;     // stop at the first accepted segment (shadow ray)
     Ray ray;
     float best_t;
     float seg_mint, seg_maxt;
@@ -60,7 +61,7 @@ template <bool PARTS>
 __device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, const RayBatch& b)
 {
     nbvh_hit h;
-    h.t       = (s.kind != 0u && !a.any_hit) ? s.best_t : FLT_MAX;
+    h.t       = (s.kind != 0u && !s.any_hit) ? s.best_t : FLT_MAX;
     h.prim_id = 0xFFFFFFFFu;
     h.node_id = s.kind ? s.hit_node : 0xFFFFFFFFu;
     h.info    = s.kind | (s.flags << 8) | ((s.segments < 65535u ? s.segments : 65535u) << 16);
@@ -160,6 +161,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
         if (s.state == SLOT_NEW) {
             s.state    = SLOT_RAY;
             s.ray      = load_ray(batch_rays<PARTS>(b, s.part), s.ray_id);
+            s.any_hit  = batch_any_hit<PARTS>(b, s.part, a.any_hit);
             s.best_t   = s.ray.tmax;
             s.segments = 0;
             s.flags    = 0;
@@ -202,7 +204,7 @@ __device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const
     s.state = SLOT_RAY;
     if (!(z[0] > a.thr_logit)) return;
     const bool inside = s.seg_mint < 0.f;
-    if (a.any_hit) {
+    if (s.any_hit) {
         bool h = true;
         if (inside) h = decode_depth(sigmoidf_cw(z[1]), s.seg_mint, s.seg_maxt) != FLT_MAX;
         if (h) {

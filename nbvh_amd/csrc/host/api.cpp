@@ -317,6 +317,8 @@ NBVH_API int32_t nbvh_destroy(nbvh_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_parts) (void)hipFree(ctx->d_parts);
+    if (ctx->h_parts) (void)hipHostFree(ctx->h_parts);
     delete ctx;
     return NBVH_OK;
 }
@@ -676,6 +678,22 @@ NBVH_API int32_t nbvh_raygen(nbvh_ctx* ctx, const nbvh_camera* cam, uint32_t w, 
     });
 }
 
+NBVH_API int32_t nbvh_scatter_diffuse(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_ray* d_rays,
+                                      const nbvh_hit* d_hits, uint32_t n, uint32_t sample, uint32_t bounce,
+                                      nbvh_ray* d_next, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        if (n == 0) return;
+        require(d_rays && d_hits && d_next, "NULL buffer");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        TriArgs t{s->d_verts.as<float>(), s->d_tris.as<uint32_t>(), s->d_bvh.as<nbvh_node>(), s->d_prims.as<uint32_t>(),
+                  0u};
+        hip_check(scatter_diffuse(t, d_rays, d_hits, n, sample, bounce, d_next, (hipStream_t)stream), "scatter launch");
+    });
+}
+
 NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays,
                             uint32_t n, nbvh_hit* d_hits, void* stream)
 {
@@ -683,6 +701,44 @@ NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query
         require(ctx && s, "NULL argument");
         require(s->ctx == ctx, "scene belongs to another context");
         run_query(ctx, s, p, RayBatch{d_rays, d_hits, nullptr, 0, n}, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p,
+                                  const nbvh_query_part* parts, uint32_t n_parts, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s && p && parts, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        require(n_parts >= 1 && n_parts <= kMaxBatchParts, "n_parts must be in 1..64");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            if (!ctx->d_parts) {
+                const size_t bytes = (size_t)nbvh_ctx::kPartSlots * kMaxBatchParts * sizeof(BatchPart);
+                hip_check(hipMalloc((void**)&ctx->d_parts, bytes), "hipMalloc part tables");
+                if (hipHostMalloc((void**)&ctx->h_parts, bytes, hipHostMallocDefault) != hipSuccess) {
+                    (void)hipFree(ctx->d_parts);
+                    ctx->d_parts = nullptr;
+                    fail(NBVH_ERR_OUT_OF_MEMORY, "hipHostMalloc part tables failed");
+                }
+            }
+        }
+        const uint32_t k = ctx->parts_next.fetch_add(1) % nbvh_ctx::kPartSlots;
+        BatchPart* h = ctx->h_parts + (size_t)k * kMaxBatchParts;
+        BatchPart* d = ctx->d_parts + (size_t)k * kMaxBatchParts;
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n_parts; i++) {
+            require(parts[i].n_rays == 0 || (parts[i].rays && parts[i].hits), "NULL ray or hit buffer in a part");
+            require(parts[i].any_hit <= 1u, "any_hit must be 0 or 1");
+            h[i] = BatchPart{parts[i].rays, parts[i].hits, (uint32_t)total, parts[i].n_rays, parts[i].any_hit, 0u};
+            total += parts[i].n_rays;
+        }
+        require(total < (1ull << 32), "more than 2^32 - 1 rays in one query");
+        if (total == 0) return;
+        hip_check(hipMemcpyAsync(d, h, n_parts * sizeof(BatchPart), hipMemcpyHostToDevice, (hipStream_t)stream),
+                  "part table H2D");
+        run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, (hipStream_t)stream);
     });
 }
 

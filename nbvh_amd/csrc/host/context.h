@@ -25,6 +25,12 @@ struct nbvh_ctx {
     std::atomic<uint32_t> counter_next{0};
     static constexpr uint32_t kCounterWords = 32;            // per query: 8 band heads, 16 B apart
     static constexpr uint32_t kCounters     = 2048 * kCounterWords;
+    // Part tables of multi-part queries (nbvh_query_parts): a ring of kPartSlots tables of
+    // kMaxBatchParts entries, device copy + pinned host staging, allocated on first use.
+    static constexpr uint32_t kPartSlots = 2048;
+    nbvh::BatchPart* d_parts = nullptr;
+    nbvh::BatchPart* h_parts = nullptr;
+    std::atomic<uint32_t> parts_next{0};
 };
 
 namespace nbvh {

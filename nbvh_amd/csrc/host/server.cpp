@@ -6,8 +6,9 @@
 // array, a scene, query parameters) and wait on tickets. One scheduler thread keeps up to
 // n_streams launches in flight on their own streams, so transfers and kernels of different
 // launches overlap. Whenever a stream frees up it takes the oldest queued request and every
-// later queued request with the same scene and parameters (up to kMaxBatchParts requests and
-// max_batch_rays rays) and runs them as ONE persistent launch over a part table: requests that
+// later queued request with the same scene and parameters (closest- and any-hit requests of one
+// family mix: the any-hit flag is per part; up to kMaxBatchParts requests and max_batch_rays
+// rays) and runs them as ONE persistent launch over a part table: requests that
 // arrive while others are on the GPU join the next launch. Requests whose rays are in host
 // memory are staged through a fixed device arena; a request waits until its 80 B per ray fit
 // (admission by HBM budget), and requests larger than the arena are refused at submit.
@@ -126,10 +127,16 @@ struct Launch {
     uint64_t done_ns = 0;
 };
 
+bool is_triangle(uint32_t kind) { return kind == NBVH_QUERY_TRIANGLE_CLOSEST || kind == NBVH_QUERY_TRIANGLE_ANY; }
+bool is_any_hit(uint32_t kind) { return kind == NBVH_QUERY_NEURAL_ANY || kind == NBVH_QUERY_TRIANGLE_ANY; }
+
+// Requests that can share a launch: same scene and kind family (closest and any hit mix, per
+// part), same precision, lod, step limit and threshold.
 bool same_key(const Request& a, const Request& b)
 {
-    return a.scene == b.scene && a.params.kind == b.params.kind && a.params.precision == b.params.precision &&
-           a.params.lod == b.params.lod && a.params.max_traversal_steps == b.params.max_traversal_steps &&
+    return a.scene == b.scene && is_triangle(a.params.kind) == is_triangle(b.params.kind) &&
+           a.params.precision == b.params.precision && a.params.lod == b.params.lod &&
+           a.params.max_traversal_steps == b.params.max_traversal_steps &&
            a.params.visibility_threshold == b.params.visibility_threshold;
 }
 
@@ -258,8 +265,10 @@ void nbvh_server::launch(Launch& L)
             part.rays = dr;
             part.hits = dh;
         }
-        part.begin = begin;
-        part.n     = r.n;
+        part.begin    = begin;
+        part.n        = r.n;
+        part.any_hit  = is_any_hit(r.params.kind) ? 1u : 0u;
+        part.reserved = 0u;
         begin += r.n;
         r.info.launch_ns   = now_ns();
         r.info.n_launches  = 1;

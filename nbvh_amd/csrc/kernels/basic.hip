@@ -51,6 +51,98 @@ __global__ void raygen_kernel(nbvh_camera cam, uint32_t w, uint32_t h, uint32_t 
     p[1] = make_float4(dx * inv, dy * inv, dz * inv, FLT_MAX);
 }
 
+// One diffuse bounce (path_inference_kernels.cu:380-393): Lambertian direction about the shading
+// normal (sample_lambertian_dir + cos_sample_hemisphere, material.cuh:101-120,193-201, frame from
+// ortho_basis, utils/common.cuh:21-35), origin offset by normal_epsilon_offset = 1e-4 along it
+// (path_module.h:220). The shading normal is the hit's predicted normal (neural) or the
+// triangle's geometric normal, turned toward the incoming ray. Random numbers: the reference's
+// LCG seeded like raygen with frame = bounce + 1. Rays without a hit come out inactive (tmax = -1).
+__device__ __forceinline__ void ortho_basis(const float n[3], float t[3], float b[3])
+{
+    float bt[3] = {0.f, 0.f, 0.f};
+    if (n[0] < 0.6f && n[0] > -0.6f)
+        bt[0] = 1.f;
+    else if (n[1] < 0.6f && n[1] > -0.6f)
+        bt[1] = 1.f;
+    else if (n[2] < 0.6f && n[2] > -0.6f)
+        bt[2] = 1.f;
+    else
+        bt[0] = 1.f;
+    t[0] = bt[1] * n[2] - bt[2] * n[1];
+    t[1] = bt[2] * n[0] - bt[0] * n[2];
+    t[2] = bt[0] * n[1] - bt[1] * n[0];
+    float inv = 1.f / sqrtf((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
+    for (int c = 0; c < 3; c++) t[c] *= inv;
+    b[0] = n[1] * t[2] - n[2] * t[1];
+    b[1] = n[2] * t[0] - n[0] * t[2];
+    b[2] = n[0] * t[1] - n[1] * t[0];
+    inv = 1.f / sqrtf((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
+    for (int c = 0; c < 3; c++) b[c] *= inv;
+}
+
+__global__ void scatter_diffuse_kernel(TriArgs tri, const nbvh_ray* __restrict__ rays, const nbvh_hit* __restrict__ hits,
+                                       uint32_t n, uint32_t sample, uint32_t bounce, nbvh_ray* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Ray r = load_ray(rays, i);
+    const nbvh_hit h = hits[i];
+    const uint32_t kind = h.info & 0xFFu;
+    float4* o = reinterpret_cast<float4*>(out + i);
+    float nrm[3];
+    if (kind == NBVH_HIT_NEURAL) {
+        nrm[0] = h.normal_or_uv[0];
+        nrm[1] = h.normal_or_uv[1];
+        nrm[2] = h.normal_or_uv[2];
+    } else if (kind == NBVH_HIT_TRIANGLE && tri.verts != nullptr) {
+        const float* a = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id];
+        const float* b = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id + 1];
+        const float* c = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id + 2];
+        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    } else {
+        o[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+        o[1] = make_float4(0.f, 0.f, 1.f, -1.f);
+        return;
+    }
+    const float len = sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+    const float sgn = ((nrm[0] * r.dx + nrm[1] * r.dy) + nrm[2] * r.dz) > 0.f ? -1.f : 1.f;
+    const float inv = len > 0.f ? sgn / len : 0.f;
+    for (int c = 0; c < 3; c++) nrm[c] *= inv;
+    if (!(len > 0.f)) nrm[2] = 1.f;
+    uint32_t st = murmur_fmix(murmur_mix(murmur_mix(0u, i), 512u * (bounce + 1u) + sample));
+    const float u0 = lcg_float(st), u1 = lcg_float(st);
+    // cos_sample_hemisphere: concentric disk mapping, z = sqrt(1 - r^2).
+    const float sx = 2.f * u0 - 1.f, sy = 2.f * u1 - 1.f;
+    float dx = 0.f, dy = 0.f;
+    if (sx != 0.f || sy != 0.f) {
+        float rad, th;
+        if (fabsf(sx) > fabsf(sy)) {
+            rad = sx;
+            th  = 0.785398163397448310f * (sy / sx);
+        } else {
+            rad = sy;
+            th  = 1.570796326794896619f - 0.785398163397448310f * (sx / sy);
+        }
+        dx = rad * cosf(th);
+        dy = rad * sinf(th);
+    }
+    float hz = sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy));
+    const float hinv = 1.f / sqrtf((dx * dx + dy * dy) + hz * hz);
+    const float hx = dx * hinv, hy = dy * hinv;
+    hz *= hinv;
+    float t[3], b[3];
+    ortho_basis(nrm, t, b);
+    float wi[3];
+    for (int c = 0; c < 3; c++) wi[c] = (hx * t[c] + hy * b[c]) + hz * nrm[c];
+    float p[3] = {r.ox + h.t * r.dx, r.oy + h.t * r.dy, r.oz + h.t * r.dz};
+    for (int c = 0; c < 3; c++) p[c] = p[c] + nrm[c] * 1e-4f;
+    o[0] = make_float4(p[0], p[1], p[2], 1e-4f);
+    o[1] = make_float4(wi[0], wi[1], wi[2], FLT_MAX);
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,6 +159,7 @@ __global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, RayBatch b)
     uint32_t part, i;
     batch_locate<PARTS>(b, g, part, i);
     const Ray ray = load_ray(batch_rays<PARTS>(b, part), i);
+    const bool any_hit = batch_any_hit<PARTS>(b, part, a.any_hit);
     const TriSetup s = tri_setup(ray);
     float tmax = ray.tmax;
     uint32_t stack[kTriStackSize];
@@ -89,7 +182,7 @@ __global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, RayBatch b)
                     prim   = tri;
                     bu     = u;
                     bv     = v;
-                    if (a.any_hit) goto done;
+                    if (any_hit) goto done;
                 }
             }
             if (ss == 0) break;
@@ -151,6 +244,14 @@ hipError_t raygen(const nbvh_camera& cam, uint32_t w, uint32_t h, uint32_t sampl
 hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream)
 {
     hipLaunchKernelGGL(dev::zero_u32_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t scatter_diffuse(const TriArgs& tri, const nbvh_ray* rays, const nbvh_hit* hits, uint32_t n, uint32_t sample,
+                           uint32_t bounce, nbvh_ray* out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::scatter_diffuse_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, tri, rays, hits, n,
+                       sample, bounce, out);
     return hipGetLastError();
 }
 

@@ -14,6 +14,10 @@ hipError_t raygen(const nbvh_camera& cam, uint32_t w, uint32_t h, uint32_t sampl
 // Zero a small device counter array (a kernel, not hipMemsetAsync, so the call can be captured).
 hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 
+// One diffuse bounce: next rays from rays + hits (tri.verts may be null when no triangle hits occur).
+hipError_t scatter_diffuse(const TriArgs& tri, const nbvh_ray* rays, const nbvh_hit* hits, uint32_t n, uint32_t sample,
+                           uint32_t bounce, nbvh_ray* out, hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 

@@ -1,4 +1,4 @@
-// neural_mfma.hip -- FP16 / BF16 neural query and network inference on CDNA4 matrix cores.
+// neural_mfma.hip -- FP16 / BF16 / FP8 neural query and network inference on CDNA4 matrix cores.
 //
 // One wave evaluates 64 segments per iteration (the slots of the persistent wave scheduler):
 //   * FP16 hash tables (scaled by a power of two), trilinear interpolation in FP32;
@@ -198,6 +198,23 @@ __device__ __forceinline__ void layer1_operands(const NetArgs& a, const float* p
     for (int s = 0; s < K0 / 16; s++) swap_halves(lo[s], hi[s]);
 }
 
+// The 8 output rows of this lane's own segment (column lane%32 of column block g), times inv.
+// Rows 0..3 of a column block sit in lane group 0, rows 4..7 in lane group 1 (registers 0..3).
+__device__ __forceinline__ void gather_logits(const floatx16 out[2], uint32_t g, float inv, float z[8])
+{
+    float send[4], recv[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        send[r] = g ? out[0][r] : out[1][r];
+        recv[r] = __shfl_xor(send[r], 32);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        z[r]     = (g ? recv[r] : out[0][r]) * inv;
+        z[4 + r] = (g ? out[1][r] : recv[r]) * inv;
+    }
+}
+
 // ReLU + conversion of the accumulators acc[rb][cb] into the B operands of the next layer.
 template <bool BF16>
 __device__ __forceinline__ void acc_to_b(const floatx16 acc[2][2], typename Frag<BF16>::vec bh[2][4])
@@ -279,18 +296,7 @@ __device__ __forceinline__ void mfma_network(const NetArgs& a, const uint4* __re
             out[1] = FR::mfma(w, bh[1][s], out[1]);
         }
     }
-    // Rows 0..3 of a column block sit in lane group 0, rows 4..7 in lane group 1 (registers 0..3).
-    float send[4], recv[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        send[r] = g ? out[0][r] : out[1][r];
-        recv[r] = __shfl_xor(send[r], 32);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        z[r]     = (g ? recv[r] : out[0][r]) * a.inv_table_scale;
-        z[4 + r] = (g ? out[1][r] : recv[r]) * a.inv_table_scale;
-    }
+    gather_logits(out, g, a.inv_table_scale, z);
 }
 
 template <int P, int L, int F, bool BF16>
@@ -300,6 +306,167 @@ __device__ __forceinline__ void stage_weights(const NetArgs& a, uint4* wfrag)
     const uint4* src = reinterpret_cast<const uint4*>(a.mlp_frag) + (BF16 ? (size_t)NF * 64 : 0);
     for (uint32_t i = threadIdx.x; i < (uint32_t)NF * 64; i += blockDim.x) wfrag[i] = src[i];
     __syncthreads();
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP8 variant: FP16 first and output layers, e4m3 hidden layers (v_mfma_f32_32x32x16_fp8_fp8).
+// Weights of hidden layer l carry a per-layer power-of-two scale (host, amax -> [224, 448)).
+// Activations entering an e4m3 layer are scaled per segment (per MFMA column) by a power of two
+// that brings the segment's largest activation into [224, 448): the scale depends on the segment
+// alone, so results do not depend on what else the wave evaluates. The bias-free ReLU MLP is
+// positively homogeneous, so every scale is undone exactly on the logits.
+// ---------------------------------------------------------------------------------------------
+typedef long fp8x8_t;  // one K = 16 operand per lane: 8 e4m3 values, element e in byte e
+
+struct Fp8Weights {
+    static constexpr int kHidden = 3;       // hidden 64x64 layers in e4m3
+    static constexpr int kFragsPerLayer = 8;
+};
+
+// 2^k with amax * 2^k in [224, 448); 1 for zero, subnormal or non-finite amax.
+__device__ __forceinline__ float fp8_column_scale(float amax)
+{
+    const uint32_t b = __float_as_uint(amax);
+    const int e = (int)(b >> 23) - 127;  // amax >= 0 (post-ReLU maximum)
+    if (!(amax > 0.f) || e == -127 || e == 128) return 1.f;
+    int k = ((b & 0x7FFFFFu) < 0x600000u ? 8 : 7) - e;  // mantissa < 1.75: 2^8 side, else 2^7
+    k = k < -126 ? -126 : (k > 127 ? 127 : k);
+    return __uint_as_float((uint32_t)(k + 127) << 23);
+}
+
+// Per column block: the largest (post-ReLU) activation of this lane's column, over both lane halves.
+__device__ __forceinline__ void column_scales(const floatx16 acc[2][2], float sx[2])
+{
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        float m = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) m = fmaxf(m, acc[rb][cb][r]);
+        m = fmaxf(m, __shfl_xor(m, 32));
+        sx[cb] = fp8_column_scale(m);
+    }
+}
+
+__device__ __forceinline__ fp8x8_t pack_fp8(const float v[8])
+{
+    int lo = 0, hi = 0;
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], lo, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], hi, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+    return (fp8x8_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// LDS images of the FP8-variant weights: FP16 layer-1 and output fragments, e4m3 hidden ones.
+template <int K0>
+struct Fp8Lds {
+    uint4 l1[MlpShape<K0>::F_L1 * 64];
+    uint4 out[4 * 64];
+    uint2 hid[Fp8Weights::kHidden * Fp8Weights::kFragsPerLayer * 64];
+};
+
+template <int K0>
+__device__ __forceinline__ void stage_weights_fp8(const NetArgs& a, Fp8Lds<K0>& w)
+{
+    typedef MlpShape<K0> S;
+    const uint4* f16 = reinterpret_cast<const uint4*>(a.mlp_frag);
+    const uint2* f8  = reinterpret_cast<const uint2*>(a.mlp_frag_fp8);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)S::F_L1 * 64; i += blockDim.x) w.l1[i] = f16[i];
+    for (uint32_t i = threadIdx.x; i < 4u * 64; i += blockDim.x) w.out[i] = f16[(size_t)(S::F_L1 + 3 * S::F_HID) * 64 + i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)Fp8Weights::kHidden * Fp8Weights::kFragsPerLayer * 64; i += blockDim.x)
+        w.hid[i] = f8[i];
+    __syncthreads();
+}
+
+template <int P, int L, int F>
+__device__ __forceinline__ void mfma_network_fp8(const NetArgs& a, const Fp8Lds<P * L * F>& w, const float* own,
+                                                 float z[8])
+{
+    constexpr int K0 = P * L * F;
+    typedef MlpShape<K0> S;
+    typedef Frag<false> FR;
+    typedef typename FR::vec V;
+    typedef typename FR::elem E;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t g    = lane >> 5;
+    V b1[2][S::KS1];
+    layer1_operands<P, L, F, false>(a, own, b1[0], b1[1]);
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int s = 0; s < S::KS1; s++) {
+            const V wv = __builtin_bit_cast(V, w.l1[(rb * S::KS1 + s) * 64 + lane]);
+            acc[rb][0] = FR::mfma(wv, b1[0][s], acc[rb][0]);
+            acc[rb][1] = FR::mfma(wv, b1[1][s], acc[rb][1]);
+        }
+    float cum[2] = {a.table_scale, a.table_scale};  // product of every scale applied, per column block
+#pragma unroll
+    for (int layer = 0; layer < Fp8Weights::kHidden; layer++) {
+        float sx[2];
+        column_scales(acc, sx);
+        fp8x8_t bh[2][4];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                float v[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[e] = fmaxf(acc[s / 2][cb][8 * (s % 2) + e], 0.f) * sx[cb];
+                bh[cb][s] = pack_fp8(v);
+            }
+            cum[cb] *= sx[cb] * a.fp8_wscale[layer];
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const fp8x8_t wv = __builtin_bit_cast(
+                    fp8x8_t, w.hid[(layer * Fp8Weights::kFragsPerLayer + rb * 4 + s) * 64 + lane]);
+                acc[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(wv, bh[0][s], acc[rb][0], 0, 0, 0);
+                acc[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(wv, bh[1][s], acc[rb][1], 0, 0, 0);
+            }
+    }
+    floatx16 out[2];
+    {
+        float sx[2];
+        column_scales(acc, sx);
+        V bh[2][4];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) bh[cb][s][e] = (E)(fmaxf(acc[s / 2][cb][8 * (s % 2) + e], 0.f) * sx[cb]);
+            cum[cb] *= sx[cb];
+        }
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) out[cb][r] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const V wv = __builtin_bit_cast(V, w.out[s * 64 + lane]);
+            out[0] = FR::mfma(wv, bh[0][s], out[0]);
+            out[1] = FR::mfma(wv, bh[1][s], out[1]);
+        }
+    }
+    // cum is a product of powers of two: its reciprocal is exact.
+    gather_logits(out, g, 1.f / cum[g], z);
 }
 
 // Waves per workgroup of the query kernel; the workgroup shares one LDS copy of the weights.
@@ -364,6 +531,56 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetAr
     }
 }
 
+template <int P, int L, int F, bool PARTS>
+__global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void neural_query_fp8_kernel(NetArgs a,
+                                                                               RayBatch b,
+                                                                               uint32_t* __restrict__ queue)
+{
+    __shared__ Fp8Lds<P * L * F> w;
+    __shared__ uint32_t stack_lds[kWavesPerBlock * kNeuralStackSize * 64];
+    stage_weights_fp8<P * L * F>(a, w);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    LdsStack st{stack_lds + wave * kNeuralStackSize * 64 + lane, 0};
+    Slot s;
+    s.state = SLOT_EMPTY;
+    RayQueue q;
+    q.init(queue, b.n, a.image_width);
+    while (fill_slots<PARTS>(a, s, st, b, q)) {
+        float own[3 * P];
+        if (s.state == SLOT_SEG) {
+            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;
+        }
+        float z[8];
+        mfma_network_fp8<P, L, F>(a, w, own, z);
+        if (s.state == SLOT_SEG) consume_segment<PARTS>(a, s, z, b);
+    }
+}
+
+template <int P, int L, int F>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void network_fp8_kernel(NetArgs a, const float* __restrict__ in,
+                                                                          uint32_t n, float* __restrict__ out,
+                                                                          float* __restrict__ logits)
+{
+    __shared__ Fp8Lds<P * L * F> w;
+    stage_weights_fp8<P * L * F>(a, w);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float own[3 * P];
+#pragma unroll
+    for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
+    float z[8];
+    mfma_network_fp8<P, L, F>(a, w, own, z);
+    if (i < n) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
+            if (logits) logits[(size_t)i * 8 + j] = z[j];
+        }
+    }
+}
+
 }  // namespace dev
 
 namespace {
@@ -408,6 +625,36 @@ hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* 
     return hipGetLastError();
 }
 
+template <int P, int L, int F, bool PARTS>
+hipError_t launch_query_fp8_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                              hipStream_t stream)
+{
+    auto k = dev::neural_query_fp8_kernel<P, L, F, PARTS>;
+    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
+    const uint32_t per_block = 64 * dev::kWavesPerBlock;
+    uint32_t blocks = (rb.n + per_block - 1) / per_block;
+    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rb, queue);
+    return hipGetLastError();
+}
+
+template <int P, int L, int F>
+hipError_t launch_query_fp8(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
+{
+    return rb.parts ? launch_query_fp8_k<P, L, F, true>(a, rb, queue, n_cus, stream)
+                    : launch_query_fp8_k<P, L, F, false>(a, rb, queue, n_cus, stream);
+}
+
+template <int P, int L, int F>
+hipError_t launch_network_fp8(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits,
+                              hipStream_t stream)
+{
+    const uint32_t per_block = 64 * dev::kWavesPerBlock;
+    hipLaunchKernelGGL((dev::network_fp8_kernel<P, L, F>), dim3((n + per_block - 1) / per_block), dim3(per_block),
+                       0, stream, a, in, n, out, logits);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 #define NBVH_MFMA_SHAPES(X) X(3, 16, 2) X(3, 8, 4)
@@ -423,6 +670,8 @@ hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, 
             return launch_query<P_, L_, F_, false>(a, rb, queue, n_cus, stream);          \
         if (precision == NBVH_PREC_BF16)                                             \
             return launch_query<P_, L_, F_, true>(a, rb, queue, n_cus, stream);           \
+        if (precision == NBVH_PREC_FP8 && a.mlp_frag_fp8 != nullptr)                 \
+            return launch_query_fp8<P_, L_, F_>(a, rb, queue, n_cus, stream);             \
     }
         NBVH_MFMA_SHAPES(X)
 #undef X
@@ -442,6 +691,8 @@ hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, u
             return launch_network<P_, L_, F_, false>(a, in, n, out, logits, stream);  \
         if (precision == NBVH_PREC_BF16)                                             \
             return launch_network<P_, L_, F_, true>(a, in, n, out, logits, stream);   \
+        if (precision == NBVH_PREC_FP8 && a.mlp_frag_fp8 != nullptr)                 \
+            return launch_network_fp8<P_, L_, F_>(a, in, n, out, logits, stream);     \
     }
         NBVH_MFMA_SHAPES(X)
 #undef X

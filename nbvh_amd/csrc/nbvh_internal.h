@@ -61,13 +61,15 @@ struct TriArgs {
     uint32_t         any_hit;
 };
 
-// One part of a multi-request launch: n rays read from `rays`, hit records written to `hits`,
-// occupying claim indices [begin, begin + n) of the launch.
+// One part of a multi-part launch: n rays read from `rays`, hit records written to `hits`,
+// occupying claim indices [begin, begin + n) of the launch; closest or any hit per part.
 struct BatchPart {
     const nbvh_ray* rays;
     nbvh_hit*       hits;
     uint32_t        begin;
     uint32_t        n;
+    uint32_t        any_hit;  // the part's rays stop at the first hit (shadow rays)
+    uint32_t        reserved;
 };
 constexpr uint32_t kMaxBatchParts = 64;
 

@@ -1,4 +1,4 @@
-"""Minimal HIP runtime binding (device buffers, streams, events) for callers of the C ABI.
+"""Minimal HIP runtime binding (device buffers, streams, events, graphs) for callers of the C ABI.
 
 It binds the libamdhip64 that libnbvh_hip.so itself links (the system ROCm runtime), so device
 pointers and streams are shared with the library. torch is deliberately not used: its wheel
@@ -49,6 +49,12 @@ def _runtime():
             "hipEventElapsedTime": [ctypes.POINTER(ctypes.c_float), vp, vp],
             "hipGetErrorString": [i32],
             "hipDeviceGetName": [ctypes.c_char_p, i32, i32],
+            "hipStreamBeginCapture": [vp, i32],
+            "hipStreamEndCapture": [vp, ctypes.POINTER(vp)],
+            "hipGraphInstantiate": [ctypes.POINTER(vp), vp, vp, vp, sz],
+            "hipGraphLaunch": [vp, vp],
+            "hipGraphExecDestroy": [vp],
+            "hipGraphDestroy": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -158,3 +164,47 @@ class Event:
         ms = ctypes.c_float()
         check(_runtime().hipEventElapsedTime(ctypes.byref(ms), self.handle, end.handle), "hipEventElapsedTime")
         return ms.value
+
+
+class Graph:
+    """Stream capture into a HIP graph: `with Graph.capture(stream) as g: <enqueue work>`, then
+    g.launch(stream) replays the captured work."""
+
+    CAPTURE_GLOBAL = 0
+
+    def __init__(self):
+        self.graph = None
+        self.exec = None
+
+    @classmethod
+    def capture(cls, stream):
+        g = cls()
+        g._stream = stream
+        return g
+
+    def __enter__(self):
+        check(_runtime().hipStreamBeginCapture(self._stream.handle, self.CAPTURE_GLOBAL), "hipStreamBeginCapture")
+        return self
+
+    def __exit__(self, exc_type, *rest):
+        gr = ctypes.c_void_p()
+        err = _runtime().hipStreamEndCapture(self._stream.handle, ctypes.byref(gr))
+        if exc_type is not None:
+            return False
+        check(err, "hipStreamEndCapture")
+        self.graph = gr.value
+        ex = ctypes.c_void_p()
+        check(_runtime().hipGraphInstantiate(ctypes.byref(ex), self.graph, None, None, 0), "hipGraphInstantiate")
+        self.exec = ex.value
+        return False
+
+    def launch(self, stream):
+        check(_runtime().hipGraphLaunch(self.exec, stream.handle), "hipGraphLaunch")
+
+    def destroy(self):
+        if self.exec:
+            _runtime().hipGraphExecDestroy(self.exec)
+            self.exec = None
+        if self.graph:
+            _runtime().hipGraphDestroy(self.graph)
+            self.graph = None

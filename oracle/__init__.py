@@ -55,6 +55,7 @@ def lib():
             "oracle_lcg_float": ([vp], f32), "oracle_expf": ([f32], f32), "oracle_sigmoidf": ([f32], f32),
             "oracle_num_threads": ([], ctypes.c_int),
             "oracle_raygen": ([vp, u32, u32, u32, u32, vp], None),
+            "oracle_scatter_diffuse": ([vp, vp, vp, vp, u32, u32, u32, vp], None),
             "oracle_slab": ([vp, vp, vp, vp, vp], ctypes.c_int),
             "oracle_dilate": ([vp, f32, vp, vp], None),
             "oracle_nbvh_leaf_sequence": ([vp, u32, vp, f32, vp, vp, vp, u32], ctypes.c_int),
@@ -148,6 +149,17 @@ def raygen(cam12, w, h, sample=0, frame=0):
     c = np.ascontiguousarray(cam12, np.float32)
     lib().oracle_raygen(_p(c), w, h, sample, frame, _p(rays))
     return rays
+
+
+def scatter_diffuse(rays, hits, sample=0, bounce=0, verts=None, tris=None):
+    """Next rays of one diffuse bounce (the restatement of the bounce kernel)."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    out = np.zeros(len(rays), RAY_DTYPE)
+    v = None if verts is None else np.ascontiguousarray(verts, np.float32)
+    t = None if tris is None else np.ascontiguousarray(tris, np.uint32)
+    lib().oracle_scatter_diffuse(_p(v), _p(t), _p(rays), _p(hits), len(rays), sample, bounce, _p(out))
+    return out
 
 
 def leaf_sequence(nodes, ray, lod=0, current_maxt=np.float32(3.4028234663852886e38), cap=4096):

@@ -694,6 +694,93 @@ ORACLE_API void oracle_raygen(const float *cam, uint32_t w, uint32_t h, uint32_t
     }
 }
 
+// One diffuse bounce (path_inference_kernels.cu:380-393): the next ray leaves p = o + t*d + 1e-4*n
+// (normal_epsilon_offset, path_module.h:220) in direction sample_lambertian_dir(n, ortho_basis(n),
+// u) (material.cuh:101-120,193-201; utils/common.cuh:21-35), n the shading normal (predicted for a
+// neural hit, geometric for a triangle hit) turned toward the incoming ray, u two LCG floats
+// seeded with (sample, bounce + 1, ray). Hit-less rays come out with tmax = -1.
+ORACLE_API void oracle_scatter_diffuse(const float *verts, const uint32_t *tris, const void *rays_in,
+                                       const void *hits_in, uint32_t n, uint32_t sample, uint32_t bounce,
+                                       void *rays_out)
+{
+    const Ray *rays = (const Ray *)rays_in;
+    const Hit *hits = (const Hit *)hits_in;
+    Ray *out        = (Ray *)rays_out;
+    for (uint32_t i = 0; i < n; i++) {
+        const Ray &r   = rays[i];
+        const Hit &h   = hits[i];
+        const uint32_t kind = h.info & 0xFFu;
+        Ray &q = out[i];
+        float nrm[3];
+        if (kind == 1u) {
+            for (int c = 0; c < 3; c++) nrm[c] = h.normal_or_uv[c];
+        } else if (kind == 2u && verts) {
+            const float *a = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id];
+            const float *b = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 1];
+            const float *c = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 2];
+            const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        } else {
+            q = Ray{{0.f, 0.f, 0.f}, 0.f, {0.f, 0.f, 1.f}, -1.f};
+            continue;
+        }
+        const float len = std::sqrt((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+        const float sgn = ((nrm[0] * r.d[0] + nrm[1] * r.d[1]) + nrm[2] * r.d[2]) > 0.f ? -1.f : 1.f;
+        const float inv = len > 0.f ? sgn / len : 0.f;
+        for (int c = 0; c < 3; c++) nrm[c] *= inv;
+        if (!(len > 0.f)) nrm[2] = 1.f;
+        uint32_t st    = lcg_seed(sample, bounce + 1u, i);
+        const float u0 = lcg_float(st), u1 = lcg_float(st);
+        // cos_sample_hemisphere: concentric disk mapping.
+        const float sx = 2.f * u0 - 1.f, sy = 2.f * u1 - 1.f;
+        float dx = 0.f, dy = 0.f;
+        if (sx != 0.f || sy != 0.f) {
+            float rad, th;
+            if (std::fabs(sx) > std::fabs(sy)) {
+                rad = sx;
+                th  = 0.785398163397448310f * (sy / sx);
+            } else {
+                rad = sy;
+                th  = 1.570796326794896619f - 0.785398163397448310f * (sx / sy);
+            }
+            dx = rad * std::cos(th);
+            dy = rad * std::sin(th);
+        }
+        float hz         = std::sqrt(std::max(0.f, 1.f - dx * dx - dy * dy));
+        const float hinv = 1.f / std::sqrt((dx * dx + dy * dy) + hz * hz);
+        const float hx = dx * hinv, hy = dy * hinv;
+        hz *= hinv;
+        // ortho_basis
+        float bt[3] = {0.f, 0.f, 0.f}, t[3], b[3];
+        if (nrm[0] < 0.6f && nrm[0] > -0.6f)
+            bt[0] = 1.f;
+        else if (nrm[1] < 0.6f && nrm[1] > -0.6f)
+            bt[1] = 1.f;
+        else if (nrm[2] < 0.6f && nrm[2] > -0.6f)
+            bt[2] = 1.f;
+        else
+            bt[0] = 1.f;
+        t[0] = bt[1] * nrm[2] - bt[2] * nrm[1];
+        t[1] = bt[2] * nrm[0] - bt[0] * nrm[2];
+        t[2] = bt[0] * nrm[1] - bt[1] * nrm[0];
+        float ti = 1.f / std::sqrt((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
+        for (int c = 0; c < 3; c++) t[c] *= ti;
+        b[0] = nrm[1] * t[2] - nrm[2] * t[1];
+        b[1] = nrm[2] * t[0] - nrm[0] * t[2];
+        b[2] = nrm[0] * t[1] - nrm[1] * t[0];
+        ti = 1.f / std::sqrt((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
+        for (int c = 0; c < 3; c++) b[c] *= ti;
+        for (int c = 0; c < 3; c++) {
+            q.d[c] = (hx * t[c] + hy * b[c]) + hz * nrm[c];
+            q.o[c] = (r.o[c] + h.t * r.d[c]) + nrm[c] * 1e-4f;
+        }
+        q.tmin = 1e-4f;
+        q.tmax = FLT_MAX;
+    }
+}
+
 // Slab test of one box (check_ray_bounds false): returns hit, writes tmin/tmax.
 ORACLE_API int oracle_slab(const float *bmin, const float *bmax, const void *ray, float *tn, float *tf)
 {

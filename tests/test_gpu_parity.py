@@ -237,3 +237,76 @@ def test_cpp_driver_matches_python_path(nb, orc, bench, tmp_path, precision):
     exp = bench["module"].neural_bvh_traversal(rays, precision=prec)
     assert hits_equal(got, exp)
     assert (nb.hit_kind(got) == nb.HIT_NEURAL).mean() > 0.2
+
+
+@pytest.mark.parametrize("family,precision", [("neural", "fp16"), ("neural", "fp32"), ("triangle", "fp32")])
+def test_query_parts_mixed_kinds(nb, orc, bench, family, precision):
+    """One launch over three ray arrays (closest, any, closest, with an empty part between):
+    each part's hits equal a direct query of its own kind."""
+    from nbvh_amd import hip
+    prec = {"fp16": nb.PREC_FP16, "fp32": nb.PREC_FP32}[precision]
+    closest = nb.QUERY_NEURAL_CLOSEST if family == "neural" else nb.QUERY_TRIANGLE_CLOSEST
+    any_k = nb.QUERY_NEURAL_ANY if family == "neural" else nb.QUERY_TRIANGLE_ANY
+    arrays = [random_rays(3000, 90, inside_frac=0.3), random_rays(2500, 91, inside_frac=0.5),
+              random_rays(0, 92), random_rays(1700, 93)]
+    kinds = [False, True, True, False]
+    bufs = [(hip.Buffer.from_array(r) if len(r) else None, hip.Buffer(max(1, len(r)) * 48)) for r in arrays]
+    parts = [((b.ptr if b else 0), len(r), h.ptr, k) for (b, h), r, k in zip(bufs, arrays, kinds)]
+    mod = bench["module"]
+    mod.query_parts_device(mod.params(kind=closest, precision=prec), parts)
+    hip.synchronize()
+    for (b, h), r, k in zip(bufs, arrays, kinds):
+        if not len(r):
+            continue
+        got = h.download(nb.HIT_DTYPE)[:len(r)]
+        exp = mod.neural_bvh_traversal(r, kind=any_k if k else closest, precision=prec)
+        assert hits_equal(got, exp)
+
+
+def test_graph_replays_match_eager(nb, orc, bench):
+    """A multi-part query captured into a HIP graph (counter reset kernel + part table copy +
+    query kernel) gives the eager result on every replay, with eager work in between."""
+    from nbvh_amd import hip
+    mod = bench["module"]
+    _, cam = camera_rays(nb, orc, 128, 96)
+    shadow = random_rays(4000, 95, inside_frac=0.4)
+    dc, ds = hip.Buffer.from_array(cam), hip.Buffer.from_array(shadow)
+    hc, hs = hip.Buffer(len(cam) * 48), hip.Buffer(len(shadow) * 48)
+    p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=nb.PREC_FP16)
+    parts = [(dc.ptr, len(cam), hc.ptr, False), (ds.ptr, len(shadow), hs.ptr, True)]
+    s = hip.Stream()
+    mod.query_parts_device(p, parts, stream=s.handle)  # eager (also allocates the part tables)
+    s.synchronize()
+    ref_c, ref_s = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
+    with hip.Graph.capture(s) as g:
+        mod.query_parts_device(p, parts, stream=s.handle)
+    for rep in range(4):
+        hc.zero()
+        hs.zero()
+        g.launch(s)
+        s.synchronize()
+        assert hits_equal(hc.download(nb.HIT_DTYPE), ref_c), rep
+        assert hits_equal(hs.download(nb.HIT_DTYPE), ref_s), rep
+        mod.neural_bvh_traversal(shadow[:500], precision=nb.PREC_FP32)  # eager work between replays
+    g.destroy()
+    s.destroy()
+
+
+@pytest.mark.parametrize("kind", ["neural", "triangle"])
+def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
+    """The bounce kernel against its restatement: same inactive rays, origins and directions
+    within 2e-6 (the device and host sin/cos differ by a few ulp)."""
+    from nbvh_amd import hip
+    _, rays = camera_rays(nb, orc, 160, 120)
+    k = nb.QUERY_NEURAL_CLOSEST if kind == "neural" else nb.QUERY_TRIANGLE_CLOSEST
+    hits = bench["module"].neural_bvh_traversal(rays, kind=k, precision=nb.PREC_FP32)
+    dr, dh, dn = hip.Buffer.from_array(rays), hip.Buffer.from_array(hits), hip.Buffer(len(rays) * 32)
+    bench["module"].scatter_diffuse_device(dr.ptr, dh.ptr, len(rays), dn.ptr, sample=2, bounce=1)
+    hip.synchronize()
+    got = dn.download(nb.RAY_DTYPE)
+    exp = orc.scatter_diffuse(rays, hits, sample=2, bounce=1, verts=bench["v"], tris=bench["t"])
+    assert np.array_equal(got["tmax"], exp["tmax"])
+    live = exp["tmax"] > 0
+    assert live.mean() > 0.2
+    assert np.abs(got["d"][live] - exp["d"][live]).max() < 2e-6
+    assert np.abs(got["o"][live] - exp["o"][live]).max() < 2e-6

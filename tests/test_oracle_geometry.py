@@ -185,3 +185,58 @@ def test_watertight_shared_edges(nb, orc):
     rays["tmin"], rays["tmax"] = 1e-4, FLT_MAX
     h = orc.tri_query(v, t, nodes, prims, rays)
     assert np.all(h["info"] & 0xFF == 2)
+
+
+def test_scatter_diffuse_is_cosine_distributed(orc):
+    """The bounce restatement: unit directions in the hemisphere of the normal turned toward the
+    incoming ray, cosine-weighted (E[cos] = 2/3, E[cos^2] = 1/2 for p(w) = cos/pi; a uniform
+    hemisphere would give 1/2 and 1/3), origin = hit point + 1e-4 * normal, hit-less rays inactive."""
+    n = 200000
+    rng = np.random.default_rng(4)
+    rays = np.zeros(n, orc.RAY_DTYPE)
+    rays["o"] = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    hits = np.zeros(n, orc.HIT_DTYPE)
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    hits["normal_or_uv"] = nrm.astype(np.float32)
+    hits["t"] = rng.uniform(0.1, 2.0, n).astype(np.float32)
+    hits["info"] = 1
+    hits["info"][:1000] = 0  # misses
+    out = orc.scatter_diffuse(rays, hits, sample=3, bounce=1)
+    assert np.all(out["tmax"][:1000] == -1) and np.all(out["tmax"][1000:] > 1e38)
+    live = slice(1000, None)
+    wi = out["d"][live].astype(np.float64)
+    assert np.allclose(np.linalg.norm(wi, axis=1), 1.0, atol=1e-5)
+    nf = np.asarray(hits["normal_or_uv"][live], np.float64)
+    facing = np.einsum("ij,ij->i", nf, rays["d"][live].astype(np.float64)) > 0
+    nf[facing] *= -1  # turned toward the incoming ray
+    cos = np.einsum("ij,ij->i", wi, nf)
+    assert cos.min() > -1e-5
+    assert abs(cos.mean() - 2 / 3) < 0.004 and abs((cos ** 2).mean() - 0.5) < 0.004
+    p = rays["o"][live].astype(np.float64) + hits["t"][live, None] * rays["d"][live].astype(np.float64)
+    assert np.allclose(out["o"][live], p + 1e-4 * nf, atol=2e-6)
+    # Deterministic in (sample, bounce, ray); a different bounce draws different directions.
+    again = orc.scatter_diffuse(rays, hits, sample=3, bounce=1)
+    other = orc.scatter_diffuse(rays, hits, sample=3, bounce=2)
+    assert np.array_equal(again.view(np.uint32), out.view(np.uint32))
+    assert np.mean(np.all(other["d"][live] == out["d"][live], axis=1)) < 1e-3
+
+
+def test_scatter_diffuse_triangle_normal(nb, orc):
+    v, t = nb.icosphere(2, amplitude=0.0)
+    rays = np.zeros(len(t), orc.RAY_DTYPE)
+    cen = v[t].mean(axis=1)
+    rays["o"] = (cen * 3).astype(np.float32)
+    rays["d"] = (-cen / np.linalg.norm(cen, axis=1, keepdims=True)).astype(np.float32)
+    hits = np.zeros(len(t), orc.HIT_DTYPE)
+    hits["info"] = 2
+    hits["prim_id"] = np.arange(len(t))
+    hits["t"] = np.linalg.norm(cen * 2, axis=1).astype(np.float32)
+    out = orc.scatter_diffuse(rays, hits, verts=v, tris=t)
+    # Outward geometric normal (counter-clockwise faces), facing the incoming ray from outside.
+    g = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    assert np.all(np.einsum("ij,ij->i", out["d"], g) > -1e-5)
+    assert np.allclose(out["o"], cen + 1e-4 * g, atol=1e-5)
