@@ -300,6 +300,24 @@ typedef struct nbvh_query_part {
 int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                          const nbvh_query_part* parts, uint32_t n_parts, void* stream);
 
+/* Launch shape of the persistent query kernels: how many workgroups per CU stay resident and
+ * into how many bands the ray queue is split (each XCD starts on its own band, so neighbouring
+ * rays share an L2). Results never depend on it. Kept per scene and precision. */
+typedef struct nbvh_launch_shape {
+    uint32_t blocks_per_cu; /* 0: as many as fit */
+    uint32_t queue_bands;   /* 1..8; 0: 8 */
+} nbvh_launch_shape;
+
+int32_t nbvh_scene_set_launch_shape(nbvh_scene* scene, uint32_t precision, const nbvh_launch_shape* shape);
+int32_t nbvh_scene_get_launch_shape(const nbvh_scene* scene, uint32_t precision, nbvh_launch_shape* shape);
+
+/* Autotuner: times the candidate launch shapes (workgroups per CU in {max, 1, 2, 3} x bands in
+ * {1, 2, 4, 8}) of `params` on the given device rays and keeps the fastest for the scene and
+ * params->precision. Synchronous on `stream`; d_hits receives the last run's (shape-independent)
+ * hits. best / best_ms may be NULL. */
+int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
+                      uint32_t n_rays, nbvh_hit* d_hits, void* stream, nbvh_launch_shape* best, float* best_ms);
+
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                         const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);

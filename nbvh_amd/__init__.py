@@ -304,6 +304,24 @@ class NeuralBVHModule:
                                          ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
                                          ctypes.c_void_p(stream or 0)))
 
+    def autotune(self, params, d_rays_ptr, n, d_hits_ptr, stream=None):
+        """Time the candidate launch shapes on these device rays; keep the fastest for the scene and
+        precision. Returns ((blocks_per_cu, queue_bands), ms)."""
+        best, ms = _ffi.LaunchShape(), ctypes.c_float()
+        self.ctx.check(_lib().nbvh_autotune(self.ctx.handle, self.scene.handle, ctypes.byref(params),
+                                            ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
+                                            ctypes.c_void_p(stream or 0), ctypes.byref(best), ctypes.byref(ms)))
+        return (best.blocks_per_cu, best.queue_bands), ms.value
+
+    def set_launch_shape(self, precision, blocks_per_cu=0, queue_bands=0):
+        shape = _ffi.LaunchShape(blocks_per_cu, queue_bands)
+        self.ctx.check(_lib().nbvh_scene_set_launch_shape(self.scene.handle, precision, ctypes.byref(shape)))
+
+    def launch_shape(self, precision):
+        shape = _ffi.LaunchShape()
+        self.ctx.check(_lib().nbvh_scene_get_launch_shape(self.scene.handle, precision, ctypes.byref(shape)))
+        return shape.blocks_per_cu, shape.queue_bands
+
     def scatter_diffuse_device(self, d_rays_ptr, d_hits_ptr, n, d_next_ptr, sample=0, bounce=0, stream=None):
         """Next rays of a diffuse bounce (nbvh_scatter_diffuse); rays without a hit get tmax = -1."""
         self.ctx.check(_lib().nbvh_scatter_diffuse(self.ctx.handle, self.scene.handle, ctypes.c_void_p(d_rays_ptr),

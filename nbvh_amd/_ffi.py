@@ -72,6 +72,10 @@ class QueryPart(ctypes.Structure):
                 ("any_hit", ctypes.c_uint32)]
 
 
+class LaunchShape(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("queue_bands", ctypes.c_uint32)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("n_streams", ctypes.c_uint32), ("max_batch_rays", ctypes.c_uint32),
                 ("staging_bytes", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 4)]
@@ -144,6 +148,10 @@ SIGNATURES = {
     "nbvh_scatter_diffuse": [P, P, P, P, U32, U32, U32, P, P],
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
+    "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
+    "nbvh_scene_get_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
+    "nbvh_autotune": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P, ctypes.POINTER(LaunchShape),
+                      ctypes.POINTER(ctypes.c_float)],
     "nbvh_network_inference": [P, P, U32, P, U32, P, P, P],
     "nbvh_network_inference_host": [P, P, U32, P, U32, P, P],
     "nbvh_server_create": [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)],

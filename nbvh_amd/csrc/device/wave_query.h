@@ -83,7 +83,8 @@ __device__ __forceinline__ uint32_t xcc_id()
     return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;
 }
 
-// Ray queue split into kQueueBands contiguous bands of the ray array, one counter per band.
+// Ray queue split into `bands` (at most kQueueBands) contiguous bands of the ray array, one
+// counter per band.
 // A wave starts on the band of its XCD, so rays that are neighbours in the array (neighbouring
 // pixels for image-ordered rays) are traced on the same XCD and share its L2; a wave moves on
 // to the next band only when its band is drained (work stealing for the tail).
@@ -100,17 +101,19 @@ struct RayQueue {
     uint32_t image_width;  // 0: claim order = array order; else 8x8-tile order of a scanline image
     uint32_t band;         // wave-uniform
     uint32_t tried;        // wave-uniform: bands found drained
-    __device__ __forceinline__ void init(uint32_t* h, uint32_t n, uint32_t width)
+    uint32_t bands;        // 1..kQueueBands (launch-uniform)
+    __device__ __forceinline__ void init(uint32_t* h, uint32_t n, uint32_t width, uint32_t n_bands)
     {
         heads = h;
         n_rays = n;
         image_width = width;
-        band = xcc_id() % kQueueBands;
+        bands = (n_bands >= 1u && n_bands <= kQueueBands) ? n_bands : kQueueBands;
+        band = xcc_id() % bands;
         tried = 0;
     }
     __device__ __forceinline__ uint32_t band_lo(uint32_t b) const
     {
-        return (uint32_t)(((uint64_t)b * n_rays) / kQueueBands);
+        return (uint32_t)(((uint64_t)b * n_rays) / bands);
     }
     // Claim index -> ray index. With an image width W (W % 8 == 0, n % (8W) == 0, checked on the
     // host) consecutive claims walk 8x8 pixel tiles, tile rows of 8 scanlines in order, so the
@@ -134,7 +137,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
         // Claim rays for the empty slots (convergent: every lane takes part in the ballot).
         const uint64_t m = __ballot(s.state == SLOT_EMPTY);
         if (m != 0ull) {
-            if (q.tried >= kQueueBands) {
+            if (q.tried >= q.bands) {
                 if (s.state == SLOT_EMPTY) s.state = SLOT_DONE;
             } else {
                 const uint32_t leader = __ffsll((unsigned long long)m) - 1u;
@@ -153,7 +156,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
                     }
                 }
                 if (__any(failed)) {
-                    q.band = (q.band + 1u) % kQueueBands;
+                    q.band = (q.band + 1u) % q.bands;
                     q.tried++;
                 }
             }

@@ -216,6 +216,8 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
     const uint64_t w = p->image_width;
     a.image_width = (batch.parts == nullptr && w != 0 && w % 8 == 0 && (uint64_t)n % (8 * w) == 0) ? (uint32_t)w : 0u;
+    a.blocks_per_cu = s->shape[p->precision].blocks_per_cu;
+    a.queue_bands   = s->shape[p->precision].queue_bands;
     uint32_t* counter = ctx->d_counters + counter_slot(ctx);
     hip_check(zero_u32(counter, nbvh_ctx::kCounterWords, stream), "counter reset");
     bool supported = false;
@@ -739,6 +741,80 @@ NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh
         hip_check(hipMemcpyAsync(d, h, n_parts * sizeof(BatchPart), hipMemcpyHostToDevice, (hipStream_t)stream),
                   "part table H2D");
         run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_scene_set_launch_shape(nbvh_scene* s, uint32_t precision, const nbvh_launch_shape* shape)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && shape, "NULL argument");
+        require(precision <= NBVH_PREC_FP8, "unknown precision");
+        require(shape->queue_bands <= 8, "queue_bands must be 0..8");
+        s->shape[precision] = *shape;
+    });
+}
+
+NBVH_API int32_t nbvh_scene_get_launch_shape(const nbvh_scene* s, uint32_t precision, nbvh_launch_shape* shape)
+{
+    return guarded(s ? s->ctx : nullptr, [&] {
+        require(s && shape, "NULL argument");
+        require(precision <= NBVH_PREC_FP8, "unknown precision");
+        *shape = s->shape[precision];
+    });
+}
+
+// Launch-shape autotuner: persistent workgroups per CU x ray-queue bands, each candidate timed
+// with events over `reps` launches on the caller's rays; the fastest is kept for the scene and
+// precision. The shape only changes which wave traces which ray, never a result.
+NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays,
+                               uint32_t n, nbvh_hit* d_hits, void* stream, nbvh_launch_shape* best, float* best_ms)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s && p, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        require(p->precision <= NBVH_PREC_FP8, "unknown precision");
+        require(n > 0 && d_rays && d_hits, "empty ray batch");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t st = (hipStream_t)stream;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            fail(NBVH_ERR_HIP, "hipEventCreate failed");
+        }
+        const nbvh_launch_shape saved = s->shape[p->precision];
+        nbvh_launch_shape win = saved;
+        float win_ms = FLT_MAX;
+        constexpr int kReps = 3;
+        try {
+            for (uint32_t bpc : {0u, 1u, 2u, 3u})
+                for (uint32_t bands : {1u, 2u, 4u, 8u}) {
+                    s->shape[p->precision] = nbvh_launch_shape{bpc, bands};
+                    const RayBatch batch{d_rays, d_hits, nullptr, 0, n};
+                    run_query(ctx, s, p, batch, st);  // warm-up
+                    hip_check(hipEventRecord(e0, st), "hipEventRecord");
+                    for (int r = 0; r < kReps; r++) run_query(ctx, s, p, batch, st);
+                    hip_check(hipEventRecord(e1, st), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    ms /= kReps;
+                    if (ms < win_ms) {
+                        win_ms = ms;
+                        win    = s->shape[p->precision];
+                    }
+                }
+        } catch (...) {
+            s->shape[p->precision] = saved;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        s->shape[p->precision] = win;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (best) *best = win;
+        if (best_ms) *best_ms = win_ms;
     });
 }
 

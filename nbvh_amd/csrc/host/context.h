@@ -101,6 +101,7 @@ struct nbvh_scene {
     nbvh::NetArgs args{};
     float table_scale = 1.f;
     float fp8_wscale[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    nbvh_launch_shape shape[4] = {};  // per precision (nbvh_autotune / nbvh_scene_set_launch_shape)
 };
 
 namespace nbvh {

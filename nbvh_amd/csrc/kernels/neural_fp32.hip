@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width);
+    q.init(queue, b.n, a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, s, st, b, q)) {
         float x[K0];
         if (s.state == SLOT_SEG) {
@@ -157,7 +157,8 @@ static hipError_t launch_fp32_query_k(const NetArgs& a, const RayBatch& rb, uint
         return (uint32_t)b;
     }();
     uint32_t blocks = (rb.n + 63) / 64;
-    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    const uint32_t cap = a.blocks_per_cu != 0u && a.blocks_per_cu < bpc ? a.blocks_per_cu : bpc;
+    blocks = blocks < n_cus * cap ? blocks : n_cus * cap;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(64), 0, stream, a, rb, queue);
     return hipGetLastError();
 }

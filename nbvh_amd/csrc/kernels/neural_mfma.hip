@@ -493,7 +493,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width);
+    q.init(queue, b.n, a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width);
+    q.init(queue, b.n, a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
@@ -603,7 +603,8 @@ hipError_t launch_query_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue,
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
     uint32_t blocks = (rb.n + per_block - 1) / per_block;
-    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    const uint32_t cap = a.blocks_per_cu != 0u && a.blocks_per_cu < bpc ? a.blocks_per_cu : bpc;
+    blocks = blocks < n_cus * cap ? blocks : n_cus * cap;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rb, queue);
     return hipGetLastError();
 }
@@ -633,7 +634,8 @@ hipError_t launch_query_fp8_k(const NetArgs& a, const RayBatch& rb, uint32_t* qu
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
     uint32_t blocks = (rb.n + per_block - 1) / per_block;
-    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    const uint32_t cap = a.blocks_per_cu != 0u && a.blocks_per_cu < bpc ? a.blocks_per_cu : bpc;
+    blocks = blocks < n_cus * cap ? blocks : n_cus * cap;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, rb, queue);
     return hipGetLastError();
 }

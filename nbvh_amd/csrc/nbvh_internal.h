@@ -38,6 +38,8 @@ struct NetArgs {
     float    thr_logit;                // visibility threshold on the logit (0 for 0.5)
     uint32_t max_steps;                // max traversal steps (segments <= max_steps + 1)
     uint32_t image_width;              // tiled claim order of a scanline image (0: array order)
+    uint32_t queue_bands;              // ray-queue bands (1..8; 0: 8), launch shape
+    uint32_t blocks_per_cu;            // persistent workgroups per CU (0: as many as fit), launch shape
     const nbvh_node* nodes;            // N-BVH nodes with DILATED boxes (1 + 1e-5 about the centre)
     uint32_t n_nodes;
     uint32_t any_hit;

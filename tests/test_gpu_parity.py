@@ -310,3 +310,23 @@ def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
     assert live.mean() > 0.2
     assert np.abs(got["d"][live] - exp["d"][live]).max() < 2e-6
     assert np.abs(got["o"][live] - exp["o"][live]).max() < 2e-6
+
+
+def test_launch_shapes_and_autotune_keep_results(nb, orc, bench):
+    """Every launch shape (workgroups per CU x queue bands) gives identical hits; the autotuner
+    keeps one of the candidates."""
+    from nbvh_amd import hip
+    mod = bench["module"]
+    _, rays = camera_rays(nb, orc, 128, 128)
+    dr, dh = hip.Buffer.from_array(rays), hip.Buffer(len(rays) * 48)
+    ref = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    saved = mod.launch_shape(nb.PREC_FP16)
+    for bpc, bands in [(1, 1), (2, 8), (1, 4), (3, 2)]:
+        mod.set_launch_shape(nb.PREC_FP16, bpc, bands)
+        assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands)
+        assert hits_equal(mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16), ref)
+    (bpc, bands), ms = mod.autotune(mod.params(precision=nb.PREC_FP16), dr.ptr, len(rays), dh.ptr)
+    assert bpc in (0, 1, 2, 3) and bands in (1, 2, 4, 8) and ms > 0
+    assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands)
+    assert hits_equal(dh.download(nb.HIT_DTYPE), ref)
+    mod.set_launch_shape(nb.PREC_FP16, *saved)
