@@ -281,6 +281,16 @@ int32_t nbvh_raygen(nbvh_ctx* ctx, const nbvh_camera* camera, uint32_t width, ui
 int32_t nbvh_scatter_diffuse(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_ray* d_rays, const nbvh_hit* d_hits,
                              uint32_t n, uint32_t sample, uint32_t bounce, nbvh_ray* d_next, void* stream);
 
+/* Ground-truth network targets of n segments (ray i clipped to [d_mint[i], d_maxt[i])) from the
+ * scene's triangles, as the reference labels training data
+ * (generate_single_bvh_network_output_reference, neural_bvh_module_common.cuh:783-861, LocalT
+ * depth :586-629): d_targets[i*8 + j] = visibility (1/0), depth (t - mint) / ((maxt - mint) +
+ * 1e-3), normal (n + 1) / 2 (geometric normal), albedo (default base colour 0.9); zeros when the
+ * segment holds no triangle. Compare with the network's sigmoid outputs for label-vs-neural
+ * diagnostics. Device pointers, stream-ordered. */
+int32_t nbvh_segment_labels(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_ray* d_rays, const float* d_mint,
+                            const float* d_maxt, uint32_t n, float* d_targets, void* stream);
+
 /* One query over n rays already in device memory; hit records to d_hits. */
 int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
                    uint32_t n_rays, nbvh_hit* d_hits, void* stream);

@@ -322,6 +322,23 @@ class NeuralBVHModule:
         self.ctx.check(_lib().nbvh_scene_get_launch_shape(self.scene.handle, precision, ctypes.byref(shape)))
         return shape.blocks_per_cu, shape.queue_bands
 
+    def segment_labels(self, rays, mint, maxt):
+        """Ground-truth targets [n, 8] of host segments (rays clipped to [mint, maxt)) from the
+        scene's triangles (vis, LocalT depth, (n+1)/2 normal, albedo); synchronous."""
+        from . import hip
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = len(rays)
+        if n == 0:
+            return np.zeros((0, 8), np.float32)
+        dr, d0, d1 = hip.Buffer.from_array(rays), hip.Buffer.from_array(np.asarray(mint, np.float32)), \
+            hip.Buffer.from_array(np.asarray(maxt, np.float32))
+        dt = hip.Buffer(n * 32)
+        self.ctx.check(_lib().nbvh_segment_labels(self.ctx.handle, self.scene.handle, ctypes.c_void_p(dr.ptr),
+                                                  ctypes.c_void_p(d0.ptr), ctypes.c_void_p(d1.ptr), n,
+                                                  ctypes.c_void_p(dt.ptr), None))
+        hip.synchronize()
+        return dt.download(np.float32).reshape(n, 8)
+
     def scatter_diffuse_device(self, d_rays_ptr, d_hits_ptr, n, d_next_ptr, sample=0, bounce=0, stream=None):
         """Next rays of a diffuse bounce (nbvh_scatter_diffuse); rays without a hit get tmax = -1."""
         self.ctx.check(_lib().nbvh_scatter_diffuse(self.ctx.handle, self.scene.handle, ctypes.c_void_p(d_rays_ptr),

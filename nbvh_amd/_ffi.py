@@ -146,6 +146,7 @@ SIGNATURES = {
     "nbvh_raygen": [P, ctypes.POINTER(Camera), U32, U32, U32, U32, P, P],
     "nbvh_query": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
     "nbvh_scatter_diffuse": [P, P, P, P, U32, U32, U32, P, P],
+    "nbvh_segment_labels": [P, P, P, P, P, U32, P, P],
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],

@@ -696,6 +696,22 @@ NBVH_API int32_t nbvh_scatter_diffuse(nbvh_ctx* ctx, const nbvh_scene* s, const 
     });
 }
 
+NBVH_API int32_t nbvh_segment_labels(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_ray* d_rays, const float* d_mint,
+                                     const float* d_maxt, uint32_t n, float* d_targets, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        if (n == 0) return;
+        require(d_rays && d_mint && d_maxt && d_targets, "NULL buffer");
+        if (s->bvh.empty()) fail(NBVH_ERR_STATE, "segment labels need the scene's triangles");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        TriArgs t{s->d_verts.as<float>(), s->d_tris.as<uint32_t>(), s->d_bvh.as<nbvh_node>(), s->d_prims.as<uint32_t>(),
+                  0u};
+        hip_check(segment_labels(t, d_rays, d_mint, d_maxt, n, d_targets, (hipStream_t)stream), "labels launch");
+    });
+}
+
 NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays,
                             uint32_t n, nbvh_hit* d_hits, void* stream)
 {

@@ -18,6 +18,10 @@ hipError_t zero_u32(uint32_t* p, uint32_t n, hipStream_t stream);
 hipError_t scatter_diffuse(const TriArgs& tri, const nbvh_ray* rays, const nbvh_hit* hits, uint32_t n, uint32_t sample,
                            uint32_t bounce, nbvh_ray* out, hipStream_t stream);
 
+// Network targets (labels) of segments from the triangle BVH: targets[n x 8].
+hipError_t segment_labels(const TriArgs& a, const nbvh_ray* rays, const float* mint, const float* maxt, uint32_t n,
+                          float* targets, hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 

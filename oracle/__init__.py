@@ -201,6 +201,33 @@ def tri_query(verts, tris, nodes, prims, rays, any_hit=False):
     return hits
 
 
+def segment_labels(verts, tris, nodes, prims, rays, mint, maxt):
+    """Network targets of segments from the triangles (generate_single_bvh_network_output_reference,
+    include/cuda/neural/neural_bvh_module_common.cuh:783-861; LocalT depth :586-629): the closest
+    triangle hit of each ray clipped to [mint, maxt) -> [vis, depth, normal (n+1)/2, albedo 0.9],
+    zeros without a hit. Depth divides in double (the reference's 1e-3 is a double literal)."""
+    r = np.array(rays, RAY_DTYPE, copy=True)
+    mint = np.asarray(mint, np.float32)
+    maxt = np.asarray(maxt, np.float32)
+    r["tmin"], r["tmax"] = mint, maxt
+    h = tri_query(verts, tris, nodes, prims, r)
+    hit = (h["info"] & 0xFF) == 2
+    out = np.zeros((len(r), 8), np.float32)
+    out[hit, 0] = 1.0
+    seg = (maxt - mint).astype(np.float32)
+    num = (h["t"] - mint).astype(np.float32)
+    out[hit, 1] = (num[hit].astype(np.float64) / (seg[hit].astype(np.float64) + 1e-3)).astype(np.float32)
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    t = np.asarray(tris, np.uint32).reshape(-1, 3)[h["prim_id"][hit]]
+    e1, e2 = v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]]
+    n = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1], e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                  e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], axis=1).astype(np.float32)
+    inv = np.float32(1) / np.sqrt((n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1]) + n[:, 2] * n[:, 2])
+    out[hit, 2:5] = (n * inv[:, None] + np.float32(1)) * np.float32(0.5)
+    out[hit, 5:8] = np.float32(0.9)
+    return out
+
+
 def tri_brute(verts, tris, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     tris = np.ascontiguousarray(tris, np.uint32)

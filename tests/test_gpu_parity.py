@@ -330,3 +330,17 @@ def test_launch_shapes_and_autotune_keep_results(nb, orc, bench):
     assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands)
     assert hits_equal(dh.download(nb.HIT_DTYPE), ref)
     mod.set_launch_shape(nb.PREC_FP16, *saved)
+
+
+def test_segment_labels_match_oracle(nb, orc, bench):
+    """GPU label kernel vs the restatement on real segments: visibility and depth bit-exact,
+    normals within 1e-7."""
+    rays = np.concatenate([camera_rays(nb, orc, 96, 96)[1], random_rays(3000, 97, inside_frac=0.3)])
+    _, log = bench["net"].neural_query(bench["nbvh"], rays, threshold=0.999999, mode=orc.FP32, log_segments=True)
+    r = rays[log["ray"]]
+    mint = np.maximum(log["mint"], 0).astype(np.float32)
+    got = bench["module"].segment_labels(r, mint, log["maxt"])
+    exp = orc.segment_labels(bench["v"], bench["t"], bench["nodes"], bench["prims"], r, mint, log["maxt"])
+    assert (exp[:, 0] == 1).mean() > 0.1 and len(r) > 5000
+    assert np.array_equal(got[:, :2], exp[:, :2])
+    assert np.abs(got[:, 2:] - exp[:, 2:]).max() <= 1e-7

@@ -240,3 +240,36 @@ def test_scatter_diffuse_triangle_normal(nb, orc):
     g /= np.linalg.norm(g, axis=1, keepdims=True)
     assert np.all(np.einsum("ij,ij->i", out["d"], g) > -1e-5)
     assert np.allclose(out["o"], cen + 1e-4 * g, atol=1e-5)
+
+
+def test_segment_labels_restatement(nb, orc, small_scene):
+    """Labels of real segments (the oracle's own segment log): visibility agrees with brute-force
+    triangle intersection inside [mint, maxt), depth in [0, 1) and equal to the closed form, normal
+    (n+1)/2 of the hit triangle's face normal, albedo the default 0.9, zeros without a hit."""
+    v, t, nodes, prims, nbn = small_scene
+    cfg = nb.network_config(n_levels=2, log2_hashmap_size=10)
+    tables, mlp = nb.network_init_params(cfg, 1, 1e-4)
+    lo, hi = nodes[0]["aabb_min"], nodes[0]["aabb_max"]
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    rays = random_rays(400, 17, inside_frac=0.3)
+    _, log = net.neural_query(nbn, rays, threshold=0.999999, mode=orc.FP32, log_segments=True)  # every leaf
+    r = rays[log["ray"]]
+    mint = np.maximum(log["mint"], 0).astype(np.float32)
+    maxt = log["maxt"]
+    lab = orc.segment_labels(v, t, nodes, prims, r, mint, maxt)
+    rr = np.array(r, copy=True)
+    rr["tmin"], rr["tmax"] = mint, maxt
+    brute = orc.tri_brute(v, t, rr)
+    hit = (brute["info"] & 0xFF) == 2
+    assert hit.mean() > 0.1 and (~hit).mean() > 0.02
+    assert np.array_equal(lab[:, 0] == 1, hit)
+    assert np.all(lab[~hit] == 0)
+    d = lab[hit, 1]
+    assert np.all((d >= 0) & (d < 1))
+    exp_d = (brute["t"][hit] - mint[hit]) / ((maxt[hit] - mint[hit]).astype(np.float64) + 1e-3)
+    assert np.allclose(d, exp_d, rtol=1e-6, atol=1e-7)
+    tri = t[brute["prim_id"][hit]].astype(np.int64)
+    n = np.cross(v[tri[:, 1]] - v[tri[:, 0]], v[tri[:, 2]] - v[tri[:, 0]]).astype(np.float64)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    assert np.allclose(lab[hit, 2:5], (n + 1) / 2, atol=1e-6)
+    assert np.all(lab[hit, 5:8] == np.float32(0.9))
