@@ -259,8 +259,8 @@ def test_segment_labels_restatement(nb, orc, small_scene):
     lab = orc.segment_labels(v, t, nodes, prims, r, mint, maxt)
     rr = np.array(r, copy=True)
     rr["tmin"], rr["tmax"] = mint, maxt
-    brute = orc.tri_brute(v, t, rr)
-    hit = (brute["info"] & 0xFF) == 2
+    brute = orc.tri_brute(v, t, rr)  # closest hit at t >= tmin, any distance
+    hit = ((brute["info"] & 0xFF) == 2) & (brute["t"] < maxt)
     assert hit.mean() > 0.1 and (~hit).mean() > 0.02
     assert np.array_equal(lab[:, 0] == 1, hit)
     assert np.all(lab[~hit] == 0)
