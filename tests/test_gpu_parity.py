@@ -166,9 +166,9 @@ def test_chess_nbvh_fp32_bit_exact(nb, orc, ctx, lod):
     scene.close()
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
 def test_network_mfma_vs_fp64(nb, orc, bench, precision):
-    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16}[precision]
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp8": nb.PREC_FP8}[precision]
     # Segment inputs of real queries (the oracle's segment log), encoded in FP32.
     _, rays = camera_rays(nb, orc, 96, 96)
     _, log = bench["net"].neural_query(bench["nbvh"], rays, mode=orc.FP64, log_segments=True)
@@ -177,11 +177,12 @@ def test_network_mfma_vs_fp64(nb, orc, bench, precision):
     o64, z64 = bench["net"].network(x, mode=orc.FP64)
     scale = np.abs(z64).max(axis=0)
     err = np.abs(logits - z64) / scale
-    # Relative to each output's range: FP16 keeps ~11 bits, BF16 ~8 bits through 5 layers.
-    tol = {"fp16": 2e-2, "bf16": 8e-2}[precision]
+    # Relative to each output's range: FP16 keeps ~11 bits, BF16 ~8 bits through 5 layers, FP8
+    # e4m3 ~4 bits through its 3 hidden layers (measured max 0.14, flips 2.2-2.4 %).
+    tol = {"fp16": 2e-2, "bf16": 8e-2, "fp8": 0.25}[precision]
     assert err.max() < tol, err.max(axis=0)
     flips = np.mean(np.sign(logits[:, 0]) != np.sign(z64[:, 0]))
-    assert flips <= 0.005, flips
+    assert flips <= {"fp16": 0.005, "bf16": 0.005, "fp8": 0.05}[precision], flips
     assert len(x) > 5000
 
 
@@ -294,8 +295,9 @@ def test_graph_replays_match_eager(nb, orc, bench):
 
 @pytest.mark.parametrize("kind", ["neural", "triangle"])
 def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
-    """The bounce kernel against its restatement: same inactive rays, origins and directions
-    within 2e-6 (the device and host sin/cos differ by a few ulp)."""
+    """The bounce kernel against its restatement: same inactive rays, origins within 2e-6 and
+    directions within 1e-5. Device and host sin/cos differ by a few ulp, and near the horizon the
+    hemisphere's z = sqrt(1 - x^2 - y^2) amplifies that (1 - r^2 cancels): 1e-5 is 6e-4 degrees."""
     from nbvh_amd import hip
     _, rays = camera_rays(nb, orc, 160, 120)
     k = nb.QUERY_NEURAL_CLOSEST if kind == "neural" else nb.QUERY_TRIANGLE_CLOSEST
@@ -308,7 +310,7 @@ def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
     assert np.array_equal(got["tmax"], exp["tmax"])
     live = exp["tmax"] > 0
     assert live.mean() > 0.2
-    assert np.abs(got["d"][live] - exp["d"][live]).max() < 2e-6
+    assert np.abs(got["d"][live] - exp["d"][live]).max() < 1e-5
     assert np.abs(got["o"][live] - exp["o"][live]).max() < 2e-6
 
 
