@@ -203,14 +203,14 @@ def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, ev
         rays = d_rays.download(nb.RAY_DTYPE, cur)
         hits = d_hits.download(nb.HIT_DTYPE, cur)
         samp = rays[::every]
-        e64, log = net.neural_query(nodes, samp, mode=oracle.FP64, log_segments=True)
+        e64, seglog = net.neural_query(nodes, samp, mode=oracle.FP64, log_segments=True)
         row = {"bounce": b, "rays": cur, "fp16_mrays_per_s": round(cur / (ms16 * 1e-3) / 1e6, 1),
                "fp32_mrays_per_s": round(cur / (ms32 * 1e-3) / 1e6, 1),
                "fp16_query_kind_mismatch": round(float(np.mean(nb.hit_kind(hits[::every]) != nb.hit_kind(e64))), 5),
-               "sample_segments": int(len(log))}
-        if len(log):
-            x = net.encode(samp[log["ray"]], log["mint"], log["maxt"], mode=oracle.FP32)
-            z64 = log["logit"].astype(np.float64)
+               "sample_segments": int(len(seglog))}
+        if len(seglog):
+            x = net.encode(samp[seglog["ray"]], seglog["mint"], seglog["maxt"], mode=oracle.FP32)
+            z64 = seglog["logit"].astype(np.float64)
             for name, prec in (("fp16", nb.PREC_FP16), ("bf16", nb.PREC_BF16), ("fp8", nb.PREC_FP8)):
                 _, z = mod.network_inference(x, precision=prec)
                 err = output_errors(z, z64)
