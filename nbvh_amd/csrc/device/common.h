@@ -41,6 +41,28 @@ __device__ __forceinline__ Box load_node(const nbvh_node* nodes, uint32_t i)
     return Box{a.x, a.y, a.z, b.x, b.y, b.z, __float_as_uint(a.w), __float_as_uint(b.w)};
 }
 
+// N-BVH node sources. GlobalNodes reads HBM/L2; CachedNodes serves the first n_cached nodes
+// from an LDS copy (the top of a breadth-first numbered tree: every traversal starts there).
+struct GlobalNodes {
+    const nbvh_node* nodes;
+    __device__ __forceinline__ Box operator()(uint32_t i) const { return load_node(nodes, i); }
+};
+
+struct CachedNodes {
+    const nbvh_node* nodes;
+    const uint4* lds;  // 2 x 16 B per node
+    uint32_t n_cached;
+    __device__ __forceinline__ Box operator()(uint32_t i) const
+    {
+        if (i < n_cached) {
+            const uint4 a = lds[2 * i], b = lds[2 * i + 1];
+            return Box{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z), __uint_as_float(b.x),
+                       __uint_as_float(b.y), __uint_as_float(b.z), a.w, b.w};
+        }
+        return load_node(nodes, i);
+    }
+};
+
 // Slab test, Majercik et al. 2018 (include/cuda/base/ray.cuh:116-135); tn/tf unclamped.
 __device__ __forceinline__ void slab(const Box& b, const Ray& r, float& tn, float& tf)
 {
@@ -59,15 +81,21 @@ __device__ __forceinline__ void slab(const Box& b, const Ray& r, float& tn, floa
 // NeuralBVH2::intersect (include/cuda/neural/accel/neural_bvh2.cuh:170-292) with the inference
 // epsilons, on boxes that are already dilated by 1 + fixed_node_dilation about their centre.
 // Stack accessor S provides push/pop on whatever storage the kernel uses.
-// Returns 1 = leaf segment found, 0 = stack exhausted, -1 = stack overflow.
-template <typename Stack>
-__device__ __forceinline__ int nbvh_next_leaf(const nbvh_node* __restrict__ nodes, uint32_t lod, const Ray& ray,
-                                              Stack& st, float current_maxt, float& seg_mint, float& seg_maxt,
-                                              uint32_t& seg_node)
+// Returns 1 = leaf segment found, 0 = stack exhausted, -1 = stack overflow, 2 = suspended after
+// `budget` node visits with the next node pushed back (the next call resumes exactly there, so
+// suspension never changes which segments are found or their order).
+template <typename Nodes, typename Stack>
+__device__ __forceinline__ int nbvh_next_leaf(const Nodes& nodes, uint32_t lod, const Ray& ray, Stack& st,
+                                              float current_maxt, float& seg_mint, float& seg_maxt,
+                                              uint32_t& seg_node, uint32_t budget = 0xFFFFFFFFu)
 {
     uint32_t idx = st.pop();
     while (true) {
-        const Box node = load_node(nodes, idx);
+        if (budget-- == 0u) {
+            st.push(idx);  // at most the size before the pop that produced idx: never overflows
+            return 2;
+        }
+        const Box node = nodes(idx);
         const bool leaf = (node.prim_count > 0u && node.prim_count <= 2u + lod) || node.left == NBVH_NO_NEURAL_CHILD;
         if (leaf) {
             float mint, maxt;
@@ -93,10 +121,10 @@ __device__ __forceinline__ int nbvh_next_leaf(const nbvh_node* __restrict__ node
         float m1, m2;
         {
             float tn, tf;
-            slab(load_node(nodes, c1), ray, tn, tf);
+            slab(nodes(c1), ray, tn, tf);
             float mint = gmax(-FLT_MAX, tn), maxt = gmin(current_maxt, tf);
             m1 = (mint <= maxt) ? mint : FLT_MAX;
-            slab(load_node(nodes, c2), ray, tn, tf);
+            slab(nodes(c2), ray, tn, tf);
             mint = gmax(-FLT_MAX, tn);
             maxt = gmin(current_maxt, tf);
             m2   = (mint <= maxt) ? mint : FLT_MAX;

@@ -129,8 +129,24 @@ struct RayQueue {
 };
 
 // Phase A. Returns false when the wave has no segment left to evaluate (all slots DONE).
-template <bool PARTS>
-__device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& st, const RayBatch& b, RayQueue& q)
+// Traversal rounds: every slot still looking for a leaf advances at most kTraversalBudget nodes
+// per round; the wave goes on to evaluate its segments once at least kReadyNum / kReadyDen of its
+// live slots hold one, so one long traversal does not hold up the other 63 slots' network
+// evaluation (the suspended slots resume in the next iteration; results are unchanged).
+#ifndef NBVH_TRAVERSAL_BUDGET
+#define NBVH_TRAVERSAL_BUDGET 16
+#endif
+#ifndef NBVH_READY_NUM
+#define NBVH_READY_NUM 1
+#endif
+#ifndef NBVH_READY_DEN
+#define NBVH_READY_DEN 2
+#endif
+constexpr uint32_t kTraversalBudget = NBVH_TRAVERSAL_BUDGET;
+
+template <bool PARTS, typename Nodes>
+__device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes, Slot& s, LdsStack& st,
+                                           const RayBatch& b, RayQueue& q)
 {
     const uint32_t lane = threadIdx.x & 63u;
     while (true) {
@@ -180,8 +196,11 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
             if (st.empty()) {
                 finish_ray<PARTS>(a, s, b);
             } else {
-                const int r = nbvh_next_leaf(a.nodes, a.lod, s.ray, st, s.best_t, s.seg_mint, s.seg_maxt, s.seg_node);
-                if (r == 1) {
+                const int r = nbvh_next_leaf(nodes, a.lod, s.ray, st, s.best_t, s.seg_mint, s.seg_maxt, s.seg_node,
+                                             kTraversalBudget);
+                if (r == 2) {
+                    // suspended: stays SLOT_RAY
+                } else if (r == 1) {
                     if (s.segments == a.max_steps + 1u) {
                         s.flags |= NBVH_FLAG_STEP_CAP;
                         finish_ray<PARTS>(a, s, b);
@@ -195,6 +214,9 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, Slot& s, LdsStack& 
             }
         }
         if (__all(s.state == SLOT_SEG || s.state == SLOT_DONE)) break;
+        const uint32_t ready = (uint32_t)__popcll(__ballot(s.state == SLOT_SEG));
+        const uint32_t live  = (uint32_t)__popcll(__ballot(s.state != SLOT_DONE));
+        if (ready * NBVH_READY_DEN >= live * NBVH_READY_NUM && ready > 0u) break;
     }
     return !__all(s.state == SLOT_DONE);
 }

@@ -107,6 +107,27 @@ __device__ __forceinline__ void interp_level_f16(const GridLevel& L, __amdgpu_bu
     }
 }
 
+// Cache policy (buffer-load aux bits: 1 sc0, 2 nt) of the corner gathers of levels
+// >= NBVH_GATHER_POLICY_FROM; the coarser levels use the default policy.
+#ifndef NBVH_GATHER_POLICY_FROM
+#define NBVH_GATHER_POLICY_FROM 64
+#endif
+#ifndef NBVH_GATHER_POLICY
+#define NBVH_GATHER_POLICY 0
+#endif
+
+template <int F, int AUX>
+__device__ __forceinline__ void load_corner(__amdgpu_buffer_rsrc_t tab, uint32_t off, uint32_t q[F / 2])
+{
+    if constexpr (F == 2) {
+        q[0] = __builtin_amdgcn_raw_buffer_load_b32(tab, off, 0, AUX);
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(tab, off, 0, AUX);
+        q[0] = v[0];
+        q[1] = v[1];
+    }
+}
+
 // Levels whose gathers are issued together (G x 8 loads in flight per lane before the first use).
 #ifndef NBVH_GATHER_GROUP
 #define NBVH_GATHER_GROUP 4
@@ -147,13 +168,10 @@ __device__ __forceinline__ void gather_operands(const NetArgs& a, __amdgpu_buffe
             for (int i = 0; i < G; i++)
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
-                    if constexpr (F == 2) {
-                        q[i][c][0] = __builtin_amdgcn_raw_buffer_load_b32(tab, off[i][c], 0, 0);
-                    } else {
-                        const auto v = __builtin_amdgcn_raw_buffer_load_b64(tab, off[i][c], 0, 0);
-                        q[i][c][0] = v[0];
-                        q[i][c][1] = v[1];
-                    }
+                    if (l0 + i >= NBVH_GATHER_POLICY_FROM)
+                        load_corner<F, NBVH_GATHER_POLICY>(tab, off[i][c], q[i][c]);
+                    else
+                        load_corner<F, 0>(tab, off[i][c], q[i][c]);
                 }
 #pragma unroll
             for (int i = 0; i < G; i++) {
@@ -469,6 +487,22 @@ __device__ __forceinline__ void mfma_network_fp8(const NetArgs& a, const Fp8Lds<
     gather_logits(out, g, 1.f / cum[g], z);
 }
 
+// N-BVH nodes copied into LDS per workgroup (the first ones: the top of the breadth-first tree,
+// all of it for up to this many nodes); 16 KB for 512 nodes keeps two workgroups per CU.
+#ifndef NBVH_NODE_CACHE
+#define NBVH_NODE_CACHE 512
+#endif
+constexpr uint32_t kNodeCache = NBVH_NODE_CACHE;
+
+// Copy the first min(n_nodes, kNodeCache) nodes into LDS (the caller synchronises).
+__device__ __forceinline__ uint32_t stage_nodes(const NetArgs& a, uint4* lds)
+{
+    const uint32_t n = a.n_nodes < kNodeCache ? a.n_nodes : kNodeCache;
+    const uint4* src = reinterpret_cast<const uint4*>(a.nodes);
+    for (uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) lds[i] = src[i];
+    return n;
+}
+
 // Waves per workgroup of the query kernel; the workgroup shares one LDS copy of the weights.
 #ifndef NBVH_MFMA_BLOCK_WAVES
 #define NBVH_MFMA_BLOCK_WAVES 4
@@ -487,14 +521,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     constexpr int NF = MlpShape<P * L * F>::N_FRAG;
     __shared__ uint4 wfrag[NF * 64];
     __shared__ uint32_t stack_lds[kWavesPerBlock * kNeuralStackSize * 64];
-    stage_weights<P, L, F, BF16>(a, wfrag);
+    __shared__ uint4 node_lds[2 * (kNodeCache > 0 ? kNodeCache : 1)];
+    const uint32_t n_cached = stage_nodes(a, node_lds);
+    stage_weights<P, L, F, BF16>(a, wfrag);  // ends with the barrier
+    const CachedNodes nodes{a.nodes, node_lds, n_cached};
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     LdsStack st{stack_lds + wave * kNeuralStackSize * 64 + lane, 0};
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
     q.init(queue, b.n, a.image_width, a.queue_bands);
-    while (fill_slots<PARTS>(a, s, st, b, q)) {
+    while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
             encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
@@ -538,14 +575,17 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
 {
     __shared__ Fp8Lds<P * L * F> w;
     __shared__ uint32_t stack_lds[kWavesPerBlock * kNeuralStackSize * 64];
-    stage_weights_fp8<P * L * F>(a, w);
+    __shared__ uint4 node_lds[2 * (kNodeCache > 0 ? kNodeCache : 1)];
+    const uint32_t n_cached = stage_nodes(a, node_lds);
+    stage_weights_fp8<P * L * F>(a, w);  // ends with the barrier
+    const CachedNodes nodes{a.nodes, node_lds, n_cached};
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     LdsStack st{stack_lds + wave * kNeuralStackSize * 64 + lane, 0};
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
     q.init(queue, b.n, a.image_width, a.queue_bands);
-    while (fill_slots<PARTS>(a, s, st, b, q)) {
+    while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
             encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
