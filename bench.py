@@ -126,6 +126,9 @@ def wavefront_bench(nb, hip, ctx, steps):
         p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
         ms = time_it(lambda: mod.query_parts_device(p, parts, stream=s.handle), steps if name == "fp16" else 3)
         out[name] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
+        ps = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, sort_rays=True)
+        ms = time_it(lambda: mod.query_parts_device(ps, parts, stream=s.handle), steps if name == "fp16" else 3)
+        out[name + "_sorted"] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
         if name == "fp16":
             h1, h2 = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
             out["segments_per_ray"] = round(float((nb.hit_segments(h1).astype(np.int64).sum()
@@ -184,8 +187,8 @@ def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, ev
     ctx.raygen(cam, W, H, d_rays.ptr, stream=s.handle)
     s.synchronize()
 
-    def timed(prec, k, cur):
-        p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
+    def timed(prec, k, cur, sort_rays=False):
+        p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, sort_rays=sort_rays)
         mod.query_device(p, d_rays.ptr, cur, d_hits.ptr, stream=s.handle)
         e = [hip.Event() for _ in range(k + 1)]
         e[0].record(s)
@@ -198,13 +201,15 @@ def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, ev
     rows, worst = [], {}
     cur = n
     for b in range(bounces + 1):
-        ms32 = timed(nb.PREC_FP32, 2, cur)
-        ms16 = timed(nb.PREC_FP16, 5, cur)  # leaves the FP16 hits in d_hits
+        # Bounces are incoherent: they are traced in the sorted coherence order (same hits).
+        srt = b > 0
+        ms32 = timed(nb.PREC_FP32, 2, cur, srt)
+        ms16 = timed(nb.PREC_FP16, 5, cur, srt)  # leaves the FP16 hits in d_hits
         rays = d_rays.download(nb.RAY_DTYPE, cur)
         hits = d_hits.download(nb.HIT_DTYPE, cur)
         samp = rays[::every]
         e64, seglog = net.neural_query(nodes, samp, mode=oracle.FP64, log_segments=True)
-        row = {"bounce": b, "rays": cur, "fp16_mrays_per_s": round(cur / (ms16 * 1e-3) / 1e6, 1),
+        row = {"bounce": b, "rays": cur, "sorted": srt, "fp16_mrays_per_s": round(cur / (ms16 * 1e-3) / 1e6, 1),
                "fp32_mrays_per_s": round(cur / (ms32 * 1e-3) / 1e6, 1),
                "fp16_query_kind_mismatch": round(float(np.mean(nb.hit_kind(hits[::every]) != nb.hit_kind(e64))), 5),
                "sample_segments": int(len(seglog))}
@@ -532,8 +537,9 @@ def main():
     if not args.no_wavefront and prec != nb.PREC_FP32:
         wf = wavefront_bench(nb, hip, ctx, max(5, args.steps // 2))
         result["wavefront"] = wf
-        log(f"wavefront: fp16 {wf['fp16']['mrays_per_s']} Mrays/s (graph {wf['fp16_graph']['mrays_per_s']}), "
-            f"fp32 {wf['fp32']['mrays_per_s']}, triangles {wf['triangle_fallback']['mrays_per_s']}")
+        log(f"wavefront: fp16 {wf['fp16']['mrays_per_s']} Mrays/s (graph {wf['fp16_graph']['mrays_per_s']}, "
+            f"sorted {wf['fp16_sorted']['mrays_per_s']}), fp32 {wf['fp32']['mrays_per_s']} (sorted "
+            f"{wf['fp32_sorted']['mrays_per_s']}), triangles {wf['triangle_fallback']['mrays_per_s']}")
 
     if not args.no_serving and prec != nb.PREC_FP32:
         sv = serving_bench(nb, hip, ctx, scene, args.serving_seconds, args.serving_load)

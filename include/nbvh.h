@@ -154,8 +154,17 @@ typedef struct nbvh_query_params {
                                      order (width % 8 == 0, n_rays % (8*width) == 0); they are then
                                      traced in 8x8 pixel tiles. 0 => array order. Results do not
                                      depend on it. */
-    uint32_t reserved[2];
+    uint32_t flags;               /* nbvh_query_flags */
+    uint32_t reserved;
 } nbvh_query_params;
+
+enum nbvh_query_flags {
+    /* Trace the rays in a coherence order: sorted on the device by direction octant and the Morton
+     * code of the origin (per part of a multi-part launch) before the launch. Hits stay at each
+     * ray's own index and are identical; worthwhile for incoherent batches (bounces, shadow rays).
+     * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray. */
+    NBVH_QUERY_SORT_RAYS = 1u
+};
 
 /* Pinhole camera in the reference's CameraParams form (origin, du, dv, dir_top_left). */
 typedef struct nbvh_camera {

@@ -181,9 +181,11 @@ struct QueryOptions {
     uint32_t max_traversal_steps = 0;     // 0: 1000 (m_max_traversal_steps)
     float visibility_threshold   = 0.f;   // 0: 0.5
     uint32_t image_width         = 0;     // scheduling hint for image-ordered rays
+    uint32_t flags               = 0;     // nbvh_query_flags (NBVH_QUERY_SORT_RAYS)
     nbvh_query_params params() const
     {
         nbvh_query_params p{};
+        p.flags                = flags;
         p.kind                 = kind;
         p.precision            = precision;
         p.lod                  = lod;

@@ -25,6 +25,7 @@ ERR_CAPACITY = 9
 
 # ---- enums ----
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
+QUERY_SORT_RAYS = 1
 QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
@@ -55,7 +56,7 @@ class NetworkConfig(ctypes.Structure):
 class QueryParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("precision", ctypes.c_uint32), ("lod", ctypes.c_uint32),
                 ("max_traversal_steps", ctypes.c_uint32), ("visibility_threshold", ctypes.c_float),
-                ("image_width", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("image_width", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Camera(ctypes.Structure):

@@ -460,6 +460,12 @@ __device__ __forceinline__ uint32_t batch_part(const RayBatch& b, uint32_t i)
     return lo;
 }
 
+// Claim index -> ray index of the launch (the coherence order, when the batch carries one).
+__device__ __forceinline__ uint32_t batch_ray(const RayBatch& b, uint32_t claim)
+{
+    return b.order ? b.order[claim] : claim;
+}
+
 // Claim index -> (part, index within the part). PARTS: the launch has a part table (a
 // compile-time switch, so single-array launches carry no part bookkeeping).
 template <bool PARTS>

@@ -166,7 +166,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes,
                     const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                     if (r < hi - lo) {
                         s.state = SLOT_NEW;
-                        batch_locate<PARTS>(b, q.ray_of(lo + r), s.part, s.ray_id);
+                        batch_locate<PARTS>(b, batch_ray(b, q.ray_of(lo + r)), s.part, s.ray_id);
                     } else {
                         failed = true;
                     }

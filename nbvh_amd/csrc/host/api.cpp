@@ -184,8 +184,41 @@ uint32_t counter_slot(nbvh_ctx* ctx)
 
 }  // namespace
 
+void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+                       hipStream_t stream);
+
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                hipStream_t stream)
+{
+    require(p != nullptr, "query params are NULL");
+    require((p->flags & ~(uint32_t)NBVH_QUERY_SORT_RAYS) == 0u, "unknown query flags");
+    if ((p->flags & NBVH_QUERY_SORT_RAYS) == 0u || batch.n < 2 || batch.order != nullptr) {
+        run_query_ordered(ctx, s, p, batch, stream);
+        return;
+    }
+    // Coherence order: keys + radix sort into transient stream-ordered memory, freed after the launch.
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t scratch = ray_order_scratch_bytes(batch.n);
+    void* mem = nullptr;
+    hip_check(hipMallocAsync(&mem, scratch + (size_t)batch.n * sizeof(uint32_t), stream), "hipMallocAsync");
+    uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + scratch);
+    const float* bmin = s->bbox_min;
+    float ext[3];
+    for (int c = 0; c < 3; c++) ext[c] = s->bbox_max[c] - s->bbox_min[c];
+    try {
+        hip_check(ray_order(batch, bmin, ext, order, mem, scratch, stream), "ray order");
+        RayBatch sorted = batch;
+        sorted.order    = order;
+        run_query_ordered(ctx, s, p, sorted, stream);
+    } catch (...) {
+        (void)hipFreeAsync(mem, stream);
+        throw;
+    }
+    hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
+}
+
+void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+                       hipStream_t stream)
 {
     require(p != nullptr, "query params are NULL");
     const uint32_t n = batch.n;
@@ -215,7 +248,10 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
     // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
     const uint64_t w = p->image_width;
-    a.image_width = (batch.parts == nullptr && w != 0 && w % 8 == 0 && (uint64_t)n % (8 * w) == 0) ? (uint32_t)w : 0u;
+    a.image_width = (batch.parts == nullptr && batch.order == nullptr && w != 0 && w % 8 == 0 &&
+                     (uint64_t)n % (8 * w) == 0)
+                        ? (uint32_t)w
+                        : 0u;
     a.blocks_per_cu = s->shape[p->precision].blocks_per_cu;
     a.queue_bands   = s->shape[p->precision].queue_bands;
     uint32_t* counter = ctx->d_counters + counter_slot(ctx);

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, RayBatch b)
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= b.n) return;
     uint32_t part, i;
-    batch_locate<PARTS>(b, g, part, i);
+    batch_locate<PARTS>(b, batch_ray(b, g), part, i);
     const Ray ray = load_ray(batch_rays<PARTS>(b, part), i);
     const TriResult r = tri_trace(a, ray, batch_any_hit<PARTS>(b, part, a.any_hit));
     nbvh_hit h;

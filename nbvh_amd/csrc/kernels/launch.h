@@ -22,6 +22,12 @@ hipError_t scatter_diffuse(const TriArgs& tri, const nbvh_ray* rays, const nbvh_
 hipError_t segment_labels(const TriArgs& a, const nbvh_ray* rays, const float* mint, const float* maxt, uint32_t n,
                           float* targets, hipStream_t stream);
 
+// Coherence order of a ray batch (sorted by part, direction octant, origin Morton code over the
+// box enc_min + [0, enc_ext]): order[claim] = ray index. Scratch: ray_order_scratch_bytes(n).
+size_t ray_order_scratch_bytes(uint32_t n);
+hipError_t ray_order(const RayBatch& rb, const float enc_min[3], const float enc_ext[3], uint32_t* order,
+                     void* scratch, size_t scratch_bytes, hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 

@@ -84,6 +84,7 @@ struct RayBatch {
     const BatchPart* parts;
     uint32_t         n_parts;
     uint32_t         n;
+    const uint32_t*  order;  // optional claim order: claim i traces ray order[i] (a permutation of [0, n))
 };
 
 }  // namespace nbvh

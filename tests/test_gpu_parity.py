@@ -346,3 +346,26 @@ def test_segment_labels_match_oracle(nb, orc, bench):
     assert (exp[:, 0] == 1).mean() > 0.1 and len(r) > 5000
     assert np.array_equal(got[:, :2], exp[:, :2])
     assert np.abs(got[:, 2:] - exp[:, 2:]).max() <= 1e-7
+
+
+@pytest.mark.parametrize("kind,precision", [("neural", "fp16"), ("neural", "fp32"), ("triangle", "fp32")])
+def test_sorted_ray_order_same_results(nb, orc, bench, kind, precision):
+    """NBVH_QUERY_SORT_RAYS only changes the tracing order: hits are identical, for one array and
+    for a multi-part launch (each part sorted within itself)."""
+    from nbvh_amd import hip
+    mod = bench["module"]
+    prec = {"fp16": nb.PREC_FP16, "fp32": nb.PREC_FP32}[precision]
+    k = nb.QUERY_NEURAL_CLOSEST if kind == "neural" else nb.QUERY_TRIANGLE_CLOSEST
+    rays = random_rays(20000, 98, inside_frac=0.3)
+    a = mod.neural_bvh_traversal(rays, kind=k, precision=prec)
+    b = mod.neural_bvh_traversal(rays, kind=k, precision=prec, sort_rays=True)
+    assert hits_equal(a, b)
+    r2 = random_rays(7000, 99, inside_frac=0.5)
+    d1, d2 = hip.Buffer.from_array(rays), hip.Buffer.from_array(r2)
+    h1, h2 = hip.Buffer(len(rays) * 48), hip.Buffer(len(r2) * 48)
+    mod.query_parts_device(mod.params(kind=k, precision=prec, sort_rays=True),
+                           [(d1.ptr, len(rays), h1.ptr, False), (d2.ptr, len(r2), h2.ptr, True)])
+    hip.synchronize()
+    assert hits_equal(h1.download(nb.HIT_DTYPE), a)
+    any_k = nb.QUERY_NEURAL_ANY if kind == "neural" else nb.QUERY_TRIANGLE_ANY
+    assert hits_equal(h2.download(nb.HIT_DTYPE), mod.neural_bvh_traversal(r2, kind=any_k, precision=prec))
