@@ -66,47 +66,6 @@ __device__ __forceinline__ void swap_halves(V& x, V& y)
     y = __builtin_bit_cast(V, b);
 }
 
-// Layer-1 B operands. Every lane computes all K0 features of its own segment (wave-uniform
-// gather code); lo[s] holds features 8s..8s+7, hi[s] features K0/2+8s..K0/2+8s+7. Column block 0
-// (segments 0..31) needs lo from lanes 0..31 and hi of the same segments in lanes 32..63;
-// column block 1 (segments 32..63) needs lo of those segments in lanes 0..31 and hi in lanes
-// 32..63. Swapping lanes 32..63 of lo with lanes 0..31 of hi produces exactly these operands
-// (lo becomes column block 0, hi column block 1), matching the weight order
-// K(s, g, e) = g*K0/2 + 8s + e of host/mfma_layout.cpp.
-// Trilinear interpolation of one level from the FP16 tables through a buffer resource: 32-bit
-// byte offsets (one VGPR per gather instead of a 64-bit address), and the hardware range check
-// returns 0 for any offset past the table instead of faulting.
-template <int F, bool EXACT>
-__device__ __forceinline__ void interp_level_f16(const GridLevel& L, __amdgpu_buffer_rsrc_t tab, float x, float y,
-                                                 float z, float acc[F])
-{
-    uint32_t g[3];
-    float f[3];
-    grid_cell(L, x, y, z, g, f);
-    uint32_t idx[8];
-    grid_corners<EXACT>(L, g, idx);
-    const uint32_t base = L.offset * (uint32_t)(2 * F);
-#pragma unroll
-    for (int k = 0; k < F; k++) acc[k] = 0.f;
-#pragma unroll
-    for (uint32_t c = 0; c < 8; c++) {
-        const float w = corner_weight(c, f);
-        const uint32_t off = base + idx[c] * (uint32_t)(2 * F);
-        if constexpr (F == 2) {
-            const uint32_t q = __builtin_amdgcn_raw_buffer_load_b32(tab, off, 0, 0);
-            acc[0] = fmaf(w, half_bits_to_float(q & 0xFFFFu), acc[0]);
-            acc[1] = fmaf(w, half_bits_to_float(q >> 16), acc[1]);
-        } else {
-            static_assert(F == 4, "FP16 gather implemented for F = 2 and F = 4");
-            const auto q = __builtin_amdgcn_raw_buffer_load_b64(tab, off, 0, 0);
-            acc[0] = fmaf(w, half_bits_to_float(q[0] & 0xFFFFu), acc[0]);
-            acc[1] = fmaf(w, half_bits_to_float(q[0] >> 16), acc[1]);
-            acc[2] = fmaf(w, half_bits_to_float(q[1] & 0xFFFFu), acc[2]);
-            acc[3] = fmaf(w, half_bits_to_float(q[1] >> 16), acc[3]);
-        }
-    }
-}
-
 // Cache policy (buffer-load aux bits: 1 sc0, 2 nt) of the corner gathers of levels
 // >= NBVH_GATHER_POLICY_FROM; the coarser levels use the default policy.
 #ifndef NBVH_GATHER_POLICY_FROM
@@ -133,10 +92,19 @@ __device__ __forceinline__ void load_corner(__amdgpu_buffer_rsrc_t tab, uint32_t
 #define NBVH_GATHER_GROUP 4
 #endif
 
+// Layer-1 B operands. Every lane computes all K0 features of its own segment (wave-uniform
+// gather code); lo[s] holds features 8s..8s+7, hi[s] features K0/2+8s..K0/2+8s+7. Column block 0
+// (segments 0..31) needs lo from lanes 0..31 and hi of the same segments in lanes 32..63;
+// column block 1 (segments 32..63) needs lo of those segments in lanes 0..31 and hi in lanes
+// 32..63. Swapping lanes 32..63 of lo with lanes 0..31 of hi (layer1_operands) produces exactly
+// these operands, matching the weight order K(s, g, e) = g*K0/2 + 8s + e of host/mfma_layout.cpp.
+//
 // Features of all P points of this lane's segment, converted to 16 bits as each level completes:
-// feature k = p*L*F + l*F + f goes to lo[k/8][k%8] (k < K0/2) or hi[(k-K0/2)/8][..].
-// Levels are processed in groups of G: all corner offsets of the group, then all its loads,
-// then the weights and the accumulation (same per-level arithmetic as interp_level_f16).
+// feature k = p*L*F + l*F + f goes to lo[k/8][k%8] (k < K0/2) or hi[(k-K0/2)/8][..]. The FP16
+// tables are read through a buffer resource: 32-bit byte offsets (one VGPR per gather), and the
+// hardware range check returns 0 for any offset past the table instead of faulting. Levels are
+// processed in groups of G: all corner offsets of the group, then all its loads, then the
+// corner weights and the FP32 accumulation over corners 0..7 (tcnn's trilinear weights).
 template <int P, int L, int F, bool BF16, bool EXACT>
 __device__ __forceinline__ void gather_operands(const NetArgs& a, __amdgpu_buffer_rsrc_t tab, const float* pts,
                                                 typename Frag<BF16>::vec* lo, typename Frag<BF16>::vec* hi)
