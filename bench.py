@@ -147,6 +147,46 @@ def wavefront_bench(nb, hip, ctx, steps):
     return out
 
 
+def plain_triangles(nb, hip, ctx, scene, oracle, W=512, H=512):
+    """The plain non-neural workload: 512x512 jittered primary rays, closest hit on the triangle BVH
+    (binned SAH, leaf <= 2) of the 81,920-triangle icosphere. GPU kernel vs the oracle's CPU
+    traversal (OpenMP, all host threads); triangle ids and t must be bit-identical."""
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    rays = oracle.raygen(nb.camera_array(cam), W, H)
+    v, t = nb.icosphere(6, seed=7, amplitude=0.1)
+    nodes, prims = scene.bvh()
+    mod = nb.NeuralBVHModule(ctx, scene)
+    dr, dh = hip.Buffer.from_array(rays), hip.Buffer(len(rays) * 48)
+    p = mod.params(kind=nb.QUERY_TRIANGLE_CLOSEST)
+    s = hip.Stream()
+    mod.query_device(p, dr.ptr, len(rays), dh.ptr, stream=s.handle)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for _ in range(10):
+        mod.query_device(p, dr.ptr, len(rays), dh.ptr, stream=s.handle)
+    e1.record(s)
+    s.synchronize()
+    gpu_ms = e0.elapsed_ms(e1) / 10
+    got = dh.download(nb.HIT_DTYPE)
+    s.destroy()
+    oracle.tri_query(v, t, nodes, prims, rays[:1024])
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        exp = oracle.tri_query(v, t, nodes, prims, rays)
+        reps += 1
+        el = time.perf_counter() - t0
+        if reps >= 5 or el > 10.0:
+            break
+    same = bool(np.array_equal(got["prim_id"], exp["prim_id"]) and
+                np.array_equal(got["t"].view(np.uint32), exp["t"].view(np.uint32)))
+    out = {"rays": len(rays), "gpu_mrays_per_s": round(len(rays) / (gpu_ms * 1e-3) / 1e6, 1),
+           "cpu_mrays_per_s": round(reps * len(rays) / el / 1e6, 2), "cpu_threads": oracle.num_threads(),
+           "prim_id_and_t_bit_exact": same, "hit_fraction": round(float((nb.hit_kind(got) != 0).mean()), 4)}
+    log(f"plain triangles 512x512: GPU {out['gpu_mrays_per_s']} Mrays/s, CPU {out['cpu_mrays_per_s']} Mrays/s "
+        f"({out['cpu_threads']} threads), bit-exact {same}")
+    return out
+
+
 def _sig(z):
     return 1.0 / (1.0 + np.exp(-np.asarray(z, np.float64)))
 
@@ -510,6 +550,9 @@ def main():
         result["vis_flip_sample_segments"] = int(len(x))
         log(f"visibility flips vs FP64 oracle: {flips:.5f} over {len(x)} segments")
 
+        if not args.no_cpu_baseline:
+            result["plain_triangles_512"] = plain_triangles(nb, hip, ctx, scene, oracle)
+
         if not args.no_path_traced and prec != nb.PREC_FP32:
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)
             result["path_traced"] = pt
@@ -524,7 +567,7 @@ def main():
                 net.neural_query(nodes, rays, mode=oracle.FP32)
                 reps += 1
                 el = time.perf_counter() - t0
-                if el * oracle.num_threads() >= 10.0 or el > 30.0:
+                if (reps >= 3 and el * oracle.num_threads() >= 10.0) or el > 30.0:
                     break
             cpu = reps * n_rays / el / 1e6
             result["cpu_baseline"] = {"value": round(cpu, 4), "unit": "Mrays/s", "cores": oracle.num_threads(),
