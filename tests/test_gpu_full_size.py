@@ -1,0 +1,81 @@
+"""GPU parity at the benchmark sizes: the whole batch is traced on the GPU, and a deterministic
+sample of it is traced again by the oracle (the oracle finishes a sample in seconds).
+
+  - the 1024x1024 headline frame: FP32 query bit-exact on every 61st ray, FP16 hit/miss and depth
+    against FP64 on the same sample;
+  - the 64-sphere scene (2.17M triangles, N-BVH 4096): triangle closest and any hit bit-exact and
+    the FP32 neural query bit-exact on samples of 1M incoherent rays;
+  - size-independent properties of the full batches: every ray gets a record, kinds are valid,
+    any-hit occlusion implies a closest hit before tmax.
+"""
+import numpy as np
+import pytest
+
+from conftest import bench_scene, camera_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def hits_equal(a, b):
+    return np.array_equal(a.view(np.uint32).reshape(len(a), -1), b.view(np.uint32).reshape(len(b), -1))
+
+
+@pytest.fixture(scope="module")
+def ctx(nb):
+    c = nb.Context()
+    yield c
+    c.close()
+
+
+def test_headline_frame(nb, orc, ctx):
+    v, t, nodes, prims, _ = bench_scene(nb)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    scene.init_network(nb.network_config(), seed=1, table_range=1e-4)
+    cfg, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, bmin, bmax)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    _, rays = camera_rays(nb, orc, 1024, 1024)
+    g32 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP32, image_width=1024)
+    g16 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16, image_width=1024)
+    assert len(g32) == len(rays) and set(np.unique(nb.hit_kind(g16))) <= {0, 1}
+    sample = np.arange(0, len(rays), 61)
+    e32 = net.neural_query(scene.nbvh(), rays[sample], mode=orc.FP32)
+    assert hits_equal(g32[sample], e32)
+    e64 = net.neural_query(scene.nbvh(), rays[sample], mode=orc.FP64)
+    kg, ke = nb.hit_kind(g16[sample]), nb.hit_kind(e64)
+    assert (kg != ke).mean() <= 0.005
+    both = (kg == 1) & (ke == 1) & (g16[sample]["node_id"] == e64["node_id"])
+    rel = np.abs(g16[sample]["t"][both] - e64["t"][both]) / np.maximum(e64["t"][both], 1e-3)
+    assert np.percentile(rel, 99) < 1e-2
+    scene.close()
+
+
+def test_sphere_field_full_batches(nb, orc, ctx):
+    import bench
+    v, t = nb.sphere_field(64, seed=5)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(4096)
+    scene.init_network(nb.network_config(), seed=2, table_range=1e-4)
+    assert len(t) > 2_000_000 and len(scene.nbvh()) == 2 * 4096 - 1
+    nodes, prims = scene.bvh()
+    mod = nb.NeuralBVHModule(ctx, scene)
+    rc, rs = bench.wavefront_rays(nb, v, t, 1 << 20, 1 << 20)
+    tc = mod.neural_bvh_traversal(rc, kind=nb.QUERY_TRIANGLE_CLOSEST)
+    ts = mod.neural_bvh_traversal(rs, kind=nb.QUERY_TRIANGLE_ANY)
+    tsc = mod.neural_bvh_traversal(rs, kind=nb.QUERY_TRIANGLE_CLOSEST)
+    # Shadow rays: occluded (any hit) exactly when the closest hit lies before the light.
+    assert np.array_equal(nb.hit_kind(ts) != 0, nb.hit_kind(tsc) != 0)
+    assert 0.05 < (nb.hit_kind(ts) != 0).mean() < 0.95
+    sample = np.arange(0, 1 << 20, 97)
+    assert hits_equal(tc[sample], orc.tri_query(v, t, nodes, prims, rc[sample]))
+    e_any = orc.tri_query(v, t, nodes, prims, rs[sample], any_hit=True)
+    assert np.array_equal(ts[sample]["info"], e_any["info"])
+    cfg, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, bmin, bmax)
+    small = np.arange(0, 1 << 20, 499)
+    n32 = mod.neural_bvh_traversal(rc, precision=nb.PREC_FP32, sort_rays=True)
+    assert hits_equal(n32[small], net.neural_query(scene.nbvh(), rc[small], mode=orc.FP32))
+    scene.close()
