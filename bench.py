@@ -287,6 +287,75 @@ def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, ev
             "per_bounce": rows, "worst_over_bounces": worst}
 
 
+def hybrid_environment(nb):
+    """Triangle meshes around the neural icosphere: a 6x6 floor quad at y = -1.3 and four
+    displaced geodesic spheres (frequency 12, radius 0.45)."""
+    vs = [np.array([[-3, -1.3, -3], [3, -1.3, -3], [3, -1.3, 3], [-3, -1.3, 3]], np.float32)]
+    ts = [np.array([[0, 2, 1], [0, 3, 2]], np.uint32)]
+    off = 4
+    for i, c in enumerate([(1.6, -0.8, 0.2), (-1.5, -0.7, -0.4), (0.3, 1.5, -0.9), (-0.2, -0.9, 1.6)]):
+        v, t = nb.geosphere(12, seed=40 + i, amplitude=0.05, center=c, radius=0.45)
+        vs.append(v)
+        ts.append(t + np.uint32(off))
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
+
+
+def hybrid_path_traced_bench(nb, hip, ctx, scene, W=1920, H=1080, bounces=4):
+    """Hybrid paths: the neural icosphere inside a triangle environment (hybrid_environment),
+    1920x1080 primary rays and 4 diffuse bounces, every bounce a hybrid closest-hit query (triangles
+    first, the neural object in front of them); FP16 and FP32 timed per bounce."""
+    env = nb.Scene(ctx, *hybrid_environment(nb))
+    mod = nb.NeuralBVHModule(ctx, scene)
+    cam = nb.camera_look_at((0.5, 0.6, 3.2), (0, -0.2, 0), (0, 1, 0), 45.0, W, H)
+    n = W * H
+    d_rays, d_hits, d_next = hip.Buffer(n * 32), hip.Buffer(n * 48), hip.Buffer(n * 32)
+    s = hip.Stream()
+    ctx.raygen(cam, W, H, d_rays.ptr, stream=s.handle)
+    s.synchronize()
+    env_mod = nb.NeuralBVHModule(ctx, env)
+
+    def timed(prec, k, cur, srt):
+        p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, sort_rays=srt)
+        mod.query_hybrid_device(env, p, d_rays.ptr, cur, d_hits.ptr, stream=s.handle)
+        e = [hip.Event() for _ in range(k + 1)]
+        e[0].record(s)
+        for i in range(k):
+            mod.query_hybrid_device(env, p, d_rays.ptr, cur, d_hits.ptr, stream=s.handle)
+            e[i + 1].record(s)
+        s.synchronize()
+        return float(np.mean([e[i].elapsed_ms(e[i + 1]) for i in range(k)]))
+
+    rows, cur = [], n
+    for b in range(bounces + 1):
+        srt = b > 0
+        ms32 = timed(nb.PREC_FP32, 2, cur, srt)
+        ms16 = timed(nb.PREC_FP16, 5, cur, srt)
+        hits = d_hits.download(nb.HIT_DTYPE, cur)
+        k = nb.hit_kind(hits)
+        rows.append({"bounce": b, "rays": cur, "sorted": srt, "fp16_mrays_per_s": round(cur / (ms16 * 1e-3) / 1e6, 1),
+                     "fp32_mrays_per_s": round(cur / (ms32 * 1e-3) / 1e6, 1),
+                     "neural_hits": round(float((k == nb.HIT_NEURAL).mean()), 4),
+                     "triangle_hits": round(float((k == nb.HIT_TRIANGLE).mean()), 4)})
+        if b == bounces:
+            break
+        env_mod.scatter_diffuse_device(d_rays.ptr, d_hits.ptr, cur, d_next.ptr, sample=0, bounce=b, stream=s.handle)
+        s.synchronize()
+        nxt = d_next.download(nb.RAY_DTYPE, cur)
+        nxt = nxt[nxt["tmax"] > 0]
+        cur = len(nxt)
+        if cur == 0:
+            break
+        d_rays.upload(nxt)
+    s.destroy()
+    env.close()
+    log("hybrid path-traced: " + ", ".join(f"b{r['bounce']} {r['fp16_mrays_per_s']}/{r['fp32_mrays_per_s']}"
+                                           for r in rows) + " Mrays/s (fp16/fp32)")
+    return {"workload": f"{W}x{H} primary rays + {bounces} diffuse bounces, hybrid closest hit: neural icosphere "
+                        "(N-BVH 256, headline network) inside a floor quad and 4 geodesic spheres (triangles)",
+            "per_bounce": rows}
+
+
 def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     """Continuous-batching serving run over three resident scenes.
 
@@ -556,6 +625,7 @@ def main():
         if not args.no_path_traced and prec != nb.PREC_FP32:
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)
             result["path_traced"] = pt
+            result["hybrid_path_traced"] = hybrid_path_traced_bench(nb, hip, ctx, scene)
 
         if not args.no_cpu_baseline:
             # CPU baseline: the oracle's FP32 path (same traversal + hash grid + FP32 MLP), all

@@ -337,6 +337,17 @@ int32_t nbvh_scene_get_launch_shape(const nbvh_scene* scene, uint32_t precision,
 int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
                       uint32_t n_rays, nbvh_hit* d_hits, void* stream, nbvh_launch_shape* best, float* best_ms);
 
+/* Hybrid query over one neural object and a scene of ordinary triangle meshes (the reference's
+ * hybrid TLAS of the neural BLAS and mesh BLASes, tlas_traversal_and_fill_next_hybrid_traversal,
+ * include/cuda/neural/neural_bvh_module_common.cuh:291-420): closest hit over both, or for
+ * NBVH_QUERY_NEURAL_ANY occlusion by either. The triangles are traced first; the neural query then
+ * looks only in front of each ray's triangle hit, and a neural hit replaces the triangle record.
+ * params->kind is NBVH_QUERY_NEURAL_CLOSEST or _ANY. Stream-ordered; uses transient
+ * stream-ordered memory (hipMallocAsync) of 80 B per ray. */
+int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles,
+                          const nbvh_query_params* params, const nbvh_ray* d_rays, uint32_t n_rays, nbvh_hit* d_hits,
+                          void* stream);
+
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                         const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);

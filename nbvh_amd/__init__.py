@@ -346,6 +346,23 @@ class NeuralBVHModule:
                                                    ctypes.c_void_p(d_hits_ptr), n, sample, bounce,
                                                    ctypes.c_void_p(d_next_ptr), ctypes.c_void_p(stream or 0)))
 
+    def query_hybrid_device(self, triangles, params, d_rays_ptr, n, d_hits_ptr, stream=None):
+        """Closest (or any) hit over this module's neural scene and the triangle scene `triangles`."""
+        self.ctx.check(_lib().nbvh_query_hybrid(self.ctx.handle, self.scene.handle, triangles.handle,
+                                                ctypes.byref(params), ctypes.c_void_p(d_rays_ptr), n,
+                                                ctypes.c_void_p(d_hits_ptr), ctypes.c_void_p(stream or 0)))
+
+    def query_hybrid(self, triangles, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, **kw):
+        """Host rays -> host hits of a hybrid query (synchronous)."""
+        from . import hip
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        if len(rays) == 0:
+            return np.zeros(0, HIT_DTYPE)
+        dr, dh = hip.Buffer.from_array(rays), hip.Buffer(len(rays) * 48)
+        self.query_hybrid_device(triangles, self.params(kind, precision, **kw), dr.ptr, len(rays), dh.ptr)
+        hip.synchronize()
+        return dh.download(HIT_DTYPE)
+
     def query_parts_device(self, params, parts, stream=None):
         """One launch over several device ray arrays: parts = [(d_rays_ptr, n, d_hits_ptr, any_hit)].
         params.kind selects the family (neural or triangle); any_hit is per part."""

@@ -150,6 +150,7 @@ SIGNATURES = {
     "nbvh_segment_labels": [P, P, P, P, P, U32, P, P],
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
+    "nbvh_query_hybrid": [P, P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_scene_get_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_autotune": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P, ctypes.POINTER(LaunchShape),

@@ -870,6 +870,40 @@ NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_pa
     });
 }
 
+NBVH_API int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles,
+                                   const nbvh_query_params* p, const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits,
+                                   void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && neural && triangles && p, "NULL argument");
+        require(neural->ctx == ctx && triangles->ctx == ctx, "scene belongs to another context");
+        require(p->kind == NBVH_QUERY_NEURAL_CLOSEST || p->kind == NBVH_QUERY_NEURAL_ANY,
+                "hybrid queries take a neural kind (closest or any)");
+        if (n == 0) return;
+        require(d_rays && d_hits, "NULL buffer");
+        if (triangles->bvh.empty()) fail(NBVH_ERR_STATE, "the triangle scene has no triangles");
+        hipStream_t st = (hipStream_t)stream;
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        const bool any = p->kind == NBVH_QUERY_NEURAL_ANY;
+        nbvh_query_params tp = *p;
+        tp.kind = any ? NBVH_QUERY_TRIANGLE_ANY : NBVH_QUERY_TRIANGLE_CLOSEST;
+        run_query(ctx, triangles, &tp, RayBatch{d_rays, d_hits, nullptr, 0, n}, st);
+        void* mem = nullptr;
+        hip_check(hipMallocAsync(&mem, (size_t)n * (sizeof(nbvh_ray) + sizeof(nbvh_hit)), st), "hipMallocAsync");
+        nbvh_ray* clipped = static_cast<nbvh_ray*>(mem);
+        nbvh_hit* nh      = reinterpret_cast<nbvh_hit*>(clipped + n);
+        try {
+            hip_check(hybrid_clip(d_rays, d_hits, n, clipped, st), "hybrid clip");
+            run_query(ctx, neural, p, RayBatch{clipped, nh, nullptr, 0, n}, st);
+            hip_check(hybrid_merge(d_hits, nh, n, any, st), "hybrid merge");
+        } catch (...) {
+            (void)hipFreeAsync(mem, st);
+            throw;
+        }
+        hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
+    });
+}
+
 NBVH_API int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* rays,
                                  uint32_t n, nbvh_hit* hits)
 {

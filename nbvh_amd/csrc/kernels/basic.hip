@@ -143,6 +143,32 @@ __global__ void scatter_diffuse_kernel(TriArgs tri, const nbvh_ray* __restrict__
     o[1] = make_float4(wi[0], wi[1], wi[2], FLT_MAX);
 }
 
+// Hybrid query helpers. clip: the neural pass only looks in front of the triangle hit
+// (tmax = min(tmax, t_tri)); merge: the neural hit replaces the triangle record when there is one
+// (it is then closer, or for any-hit queries the ray was not occluded by triangles).
+__global__ void hybrid_clip_kernel(const nbvh_ray* __restrict__ rays, const nbvh_hit* __restrict__ tri, uint32_t n,
+                                   nbvh_ray* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ray r = load_ray(rays, i);
+    const nbvh_hit h = tri[i];
+    if ((h.info & 0xFFu) != 0u && h.t < r.tmax) r.tmax = h.t;
+    float4* o = reinterpret_cast<float4*>(out + i);
+    o[0] = make_float4(r.ox, r.oy, r.oz, r.tmin);
+    o[1] = make_float4(r.dx, r.dy, r.dz, r.tmax);
+}
+
+__global__ void hybrid_merge_kernel(nbvh_hit* __restrict__ hits, const nbvh_hit* __restrict__ neural, uint32_t n,
+                                    uint32_t any_hit)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const nbvh_hit nh = neural[i];
+    const uint32_t tri_kind = hits[i].info & 0xFFu;
+    if ((nh.info & 0xFFu) != 0u && !(any_hit && tri_kind != 0u)) store_hit(hits, i, nh);
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -304,6 +330,19 @@ hipError_t segment_labels(const TriArgs& a, const nbvh_ray* rays, const float* m
 {
     hipLaunchKernelGGL(dev::segment_labels_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, rays, mint, maxt, n,
                        targets);
+    return hipGetLastError();
+}
+
+hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::hybrid_clip_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rays, tri, n, out);
+    return hipGetLastError();
+}
+
+hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dev::hybrid_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, hits, neural, n,
+                       any_hit ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -28,6 +28,10 @@ size_t ray_order_scratch_bytes(uint32_t n);
 hipError_t ray_order(const RayBatch& rb, const float enc_min[3], const float enc_ext[3], uint32_t* order,
                      void* scratch, size_t scratch_bytes, hipStream_t stream);
 
+// Hybrid query steps: rays clipped to their triangle hits; neural records merged over triangle ones.
+hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream);
+hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 

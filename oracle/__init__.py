@@ -228,6 +228,23 @@ def segment_labels(verts, tris, nodes, prims, rays, mint, maxt):
     return out
 
 
+def hybrid_query(net, nbvh_nodes, verts, tris, nodes, prims, rays, any_hit=False, mode=FP32):
+    """Closest (any) hit over a neural object and a triangle scene, composed from the two
+    restatements like the reference's hybrid traversal resolves a ray: the triangle hit bounds the
+    neural search (tmax), and a neural hit found in front of it wins."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    th = tri_query(verts, tris, nodes, prims, rays, any_hit=any_hit)
+    clipped = rays.copy()
+    hit = (th["info"] & 0xFF) != 0
+    closer = hit & (th["t"] < clipped["tmax"])
+    clipped["tmax"][closer] = th["t"][closer]
+    nh = net.neural_query(nbvh_nodes, clipped, any_hit=any_hit, mode=mode)
+    take = ((nh["info"] & 0xFF) != 0) & ~(any_hit & hit)
+    out = th.copy()
+    out[take] = nh[take]
+    return out
+
+
 def tri_brute(verts, tris, rays):
     verts = np.ascontiguousarray(verts, np.float32)
     tris = np.ascontiguousarray(tris, np.uint32)

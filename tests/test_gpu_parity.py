@@ -369,3 +369,44 @@ def test_sorted_ray_order_same_results(nb, orc, bench, kind, precision):
     assert hits_equal(h1.download(nb.HIT_DTYPE), a)
     any_k = nb.QUERY_NEURAL_ANY if kind == "neural" else nb.QUERY_TRIANGLE_ANY
     assert hits_equal(h2.download(nb.HIT_DTYPE), mod.neural_bvh_traversal(r2, kind=any_k, precision=prec))
+
+
+def hybrid_environment(nb):
+    """Triangle meshes around the neural icosphere: a floor quad and four geodesic spheres."""
+    vs, ts, off = [], [], 0
+    quad_v = np.array([[-3, -1.3, -3], [3, -1.3, -3], [3, -1.3, 3], [-3, -1.3, 3]], np.float32)
+    quad_t = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
+    vs.append(quad_v)
+    ts.append(quad_t)
+    off = 4
+    for i, c in enumerate([(1.6, -0.8, 0.2), (-1.5, -0.7, -0.4), (0.3, 1.5, -0.9), (-0.2, -0.9, 1.6)]):
+        v, t = nb.geosphere(12, seed=40 + i, amplitude=0.05, center=c, radius=0.45)
+        vs.append(v)
+        ts.append(t + np.uint32(off))
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_hybrid_query_matches_oracle(nb, orc, ctx, bench, any_hit):
+    """Hybrid closest/any hit over the neural icosphere and a triangle environment: FP32 bit-exact
+    with the oracle's composition; the hybrid t never exceeds the triangle t; both kinds occur."""
+    ev, et = hybrid_environment(nb)
+    env = nb.Scene(ctx, ev, et)
+    enodes, eprims = env.bvh()
+    _, cam = camera_rays(nb, orc, 128, 128, eye=(0.5, 0.6, 3.2))
+    rays = np.concatenate([cam, random_rays(6000, 101, origin_box=3.0, inside_frac=0.2)])
+    kind = nb.QUERY_NEURAL_ANY if any_hit else nb.QUERY_NEURAL_CLOSEST
+    got = bench["module"].query_hybrid(env, rays, kind=kind, precision=nb.PREC_FP32)
+    exp = orc.hybrid_query(bench["net"], bench["nbvh"], ev, et, enodes, eprims, rays, any_hit=any_hit)
+    assert hits_equal(got, exp)
+    kinds = nb.hit_kind(got)
+    assert (kinds == nb.HIT_NEURAL).mean() > 0.05 and (kinds == nb.HIT_TRIANGLE).mean() > 0.05
+    if not any_hit:
+        tenv = orc.tri_query(ev, et, enodes, eprims, rays)
+        assert np.all(got["t"] <= tenv["t"])
+    g16 = bench["module"].query_hybrid(env, rays, kind=kind, precision=nb.PREC_FP16)
+    e64 = orc.hybrid_query(bench["net"], bench["nbvh"], ev, et, enodes, eprims, rays, any_hit=any_hit,
+                           mode=orc.FP64)
+    assert (nb.hit_kind(g16) != nb.hit_kind(e64)).mean() <= 0.005
+    env.close()
