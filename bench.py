@@ -644,6 +644,18 @@ def main():
                                       "kind": "port",
                                       "sample": f"{reps} x full {W}x{H} frame, oracle FP32 path, {el:.1f} s wall"}
             log(f"cpu baseline: {cpu:.3f} Mrays/s on {oracle.num_threads()} threads")
+            # One thread, on every 8th row of the frame (about 10 s of CPU work).
+            threads = oracle.num_threads()
+            oracle.set_num_threads(1)
+            sub = rays.reshape(H, W)[::8].reshape(-1)
+            t0 = time.perf_counter()
+            net.neural_query(nodes, sub, mode=oracle.FP32)
+            el1 = time.perf_counter() - t0
+            oracle.set_num_threads(threads)
+            result["cpu_baseline_1_thread"] = {"value": round(len(sub) / el1 / 1e6, 4), "unit": "Mrays/s",
+                                               "sample": f"every 8th row of the {W}x{H} frame ({len(sub)} rays), "
+                                                         f"{el1:.1f} s"}
+            log(f"cpu baseline, 1 thread: {len(sub) / el1 / 1e6:.4f} Mrays/s")
     except ImportError as e:
         log(f"oracle unavailable ({e}); no flip rate / cpu baseline")
 

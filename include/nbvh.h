@@ -162,7 +162,8 @@ enum nbvh_query_flags {
     /* Trace the rays in a coherence order: sorted on the device by direction octant and the Morton
      * code of the origin (per part of a multi-part launch) before the launch. Hits stay at each
      * ray's own index and are identical; worthwhile for incoherent batches (bounces, shadow rays).
-     * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray. */
+     * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray, so a
+     * sorted query is not meant for HIP graph capture. */
     NBVH_QUERY_SORT_RAYS = 1u
 };
 
@@ -343,7 +344,7 @@ int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_query_params*
  * NBVH_QUERY_NEURAL_ANY occlusion by either. The triangles are traced first; the neural query then
  * looks only in front of each ray's triangle hit, and a neural hit replaces the triangle record.
  * params->kind is NBVH_QUERY_NEURAL_CLOSEST or _ANY. Stream-ordered; uses transient
- * stream-ordered memory (hipMallocAsync) of 80 B per ray. */
+ * stream-ordered memory (hipMallocAsync) of 80 B per ray (not meant for HIP graph capture). */
 int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles,
                           const nbvh_query_params* params, const nbvh_ray* d_rays, uint32_t n_rays, nbvh_hit* d_hits,
                           void* stream);

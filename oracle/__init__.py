@@ -52,7 +52,7 @@ def lib():
         sig = {
             "oracle_murmur_mix": ([u32, u32], u32), "oracle_murmur_fmix": ([u32], u32),
             "oracle_lcg_seed": ([u32, u32, u32], u32), "oracle_lcg_next": ([vp], u32),
-            "oracle_lcg_float": ([vp], f32), "oracle_expf": ([f32], f32), "oracle_sigmoidf": ([f32], f32),
+            "oracle_lcg_float": ([vp], f32), "oracle_set_num_threads": ([ctypes.c_int], None), "oracle_expf": ([f32], f32), "oracle_sigmoidf": ([f32], f32),
             "oracle_num_threads": ([], ctypes.c_int),
             "oracle_raygen": ([vp, u32, u32, u32, u32, vp], None),
             "oracle_scatter_diffuse": ([vp, vp, vp, vp, u32, u32, u32, vp], None),
@@ -142,6 +142,11 @@ class Net:
         lib().oracle_neural_query(ctypes.byref(self.s), _p(nodes), _p(rays), len(rays), int(any_hit), mode, lod,
                                   max_steps, threshold, _p(hits), _p(log), len(log), ctypes.byref(n_seg))
         return hits, log
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's parallel loops (the CPU baseline's 1-thread figure)."""
+    lib().oracle_set_num_threads(int(n))
 
 
 def raygen(cam12, w, h, sample=0, frame=0):

@@ -663,6 +663,15 @@ ORACLE_API float oracle_lcg_float(uint32_t *state) { return lcg_float(*state); }
 ORACLE_API float oracle_expf(float x) { return expf_cw(x); }
 ORACLE_API float oracle_sigmoidf(float x) { return sigmoid_f(x); }
 
+ORACLE_API void oracle_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 ORACLE_API int oracle_num_threads(void)
 {
 #ifdef _OPENMP
