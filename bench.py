@@ -644,7 +644,7 @@ def main():
                                       "kind": "port",
                                       "sample": f"{reps} x full {W}x{H} frame, oracle FP32 path, {el:.1f} s wall"}
             log(f"cpu baseline: {cpu:.3f} Mrays/s on {oracle.num_threads()} threads")
-            # One thread, on every 8th row of the frame (about 10 s of CPU work).
+            # One thread, on every 8th row of the frame (a few seconds of CPU work).
             threads = oracle.num_threads()
             oracle.set_num_threads(1)
             sub = rays.reshape(H, W)[::8].reshape(-1)
