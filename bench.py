@@ -124,6 +124,11 @@ def wavefront_bench(nb, hip, ctx, steps):
            "rays": 2 * n}
     for name, prec in (("fp16", nb.PREC_FP16), ("fp32", nb.PREC_FP32)):
         p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
+        # Launch shape chosen by the autotuner on the closest-hit half of the batch (kept for the
+        # scene and precision, so the two-part launches below use it).
+        shape, tune_ms = mod.autotune(p, dc.ptr, n, hc.ptr, stream=s.handle)
+        out[name + "_autotuned_shape"] = {"blocks_per_cu": shape[0], "queue_bands": shape[1],
+                                          "closest_half_ms": round(tune_ms, 4)}
         ms = time_it(lambda: mod.query_parts_device(p, parts, stream=s.handle), steps if name == "fp16" else 3)
         out[name] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
         ps = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, sort_rays=True)
