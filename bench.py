@@ -120,7 +120,8 @@ def wavefront_bench(nb, hip, ctx, steps):
         return float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(k)]))
 
     out = {"workload": "64 displaced geodesic spheres, 2.17M triangles, N-BVH 4096 leaves; 4,194,304 rays in one "
-                       "launch: 2M incoherent closest-hit + 2M shadow any-hit to 8 point lights",
+                       "launch: 2M incoherent closest-hit + 2M shadow any-hit to 8 point lights; launch shape "
+                       "from the autotuner",
            "rays": 2 * n}
     for name, prec in (("fp16", nb.PREC_FP16), ("fp32", nb.PREC_FP32)):
         p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
