@@ -164,7 +164,15 @@ enum nbvh_query_flags {
      * ray's own index and are identical; worthwhile for incoherent batches (bounces, shadow rays).
      * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray, so a
      * sorted query is not meant for HIP graph capture. */
-    NBVH_QUERY_SORT_RAYS = 1u
+    NBVH_QUERY_SORT_RAYS = 1u,
+    /* Neural kinds: evaluate in leaf-sorted wavefront rounds instead of the persistent kernel.
+     * Each round traverses every active ray to its next leaf (one thread per ray), sorts the
+     * round's segments by N-BVH leaf and evaluates them in that order, so the hash-grid gathers of
+     * a wave fall in one leaf box. Same hit records as the persistent kernel; pays off for
+     * incoherent batches (bounces, shadow rays of a wavefront path tracer). Synchronises the
+     * stream once per round (two 4-byte reads) and uses stream-ordered transient memory of about
+     * 300 B per ray; not for graph capture. */
+    NBVH_QUERY_WAVEFRONT = 2u
 };
 
 /* Pinhole camera in the reference's CameraParams form (origin, du, dv, dir_top_left). */

@@ -285,16 +285,18 @@ class NeuralBVHModule:
 
     @staticmethod
     def params(kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0, max_traversal_steps=0,
-               visibility_threshold=0.0, image_width=0, sort_rays=False):
-        return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width,
-                           _ffi.QUERY_SORT_RAYS if sort_rays else 0)
+               visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False):
+        flags = (_ffi.QUERY_SORT_RAYS if sort_rays else 0) | (_ffi.QUERY_WAVEFRONT if wavefront else 0)
+        return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, flags)
 
     def neural_bvh_traversal(self, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0,
-                             max_traversal_steps=0, visibility_threshold=0.0, image_width=0, sort_rays=False):
+                             max_traversal_steps=0, visibility_threshold=0.0, image_width=0, sort_rays=False,
+                             wavefront=False):
         """Host rays (RAY_DTYPE) -> host hits (HIT_DTYPE); synchronous."""
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
-        p = self.params(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, sort_rays)
+        p = self.params(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, sort_rays,
+                        wavefront)
         self.ctx.check(_lib().nbvh_query_host(self.ctx.handle, self.scene.handle, ctypes.byref(p), _ptr(rays),
                                               len(rays), _ptr(hits)))
         return hits

@@ -26,6 +26,7 @@ ERR_CAPACITY = 9
 # ---- enums ----
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
 QUERY_SORT_RAYS = 1
+QUERY_WAVEFRONT = 2
 QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2

@@ -43,25 +43,32 @@ struct LdsStack {
 // SEG: holds a leaf segment to evaluate; DONE: the queue is drained.
 enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DONE = 3, SLOT_NEW = 4 };
 
-struct Slot {
-    uint32_t state;
-    uint32_t part;    // part of a multi-part launch (0 for a single array)
-    uint32_t ray_id;  // index within the part
-    bool any_hit;     // stop at the first accepted segment (shadow ray)
-    Ray ray;
-    float best_t;
-    float seg_mint, seg_maxt;
-    uint32_t seg_node;
+// Per-ray result of a neural query, built up segment by segment (shared by the persistent wave
+// scheduler and the leaf-sorted wavefront rounds, so both make identical decisions).
+struct HitState {
+    float best_t;  // closest decoded depth so far (starts at the ray's tmax)
     uint32_t segments, flags, kind, hit_node;
     float hit_logit;
     float nrm[3], alb[3];
 };
 
-template <bool PARTS>
-__device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, const RayBatch& b)
+__device__ __forceinline__ void hit_state_init(HitState& h, float tmax)
+{
+    h.best_t    = tmax;
+    h.segments  = 0;
+    h.flags     = 0;
+    h.kind      = 0;
+    h.hit_node  = 0xFFFFFFFFu;
+    h.hit_logit = 0.f;
+    h.nrm[0] = h.nrm[1] = h.nrm[2] = 0.f;
+    h.alb[0] = h.alb[1] = h.alb[2] = 0.f;
+}
+
+// The hit record of a finished ray.
+__device__ __forceinline__ nbvh_hit make_hit(const HitState& s, bool any_hit)
 {
     nbvh_hit h;
-    h.t       = (s.kind != 0u && !s.any_hit) ? s.best_t : FLT_MAX;
+    h.t       = (s.kind != 0u && !any_hit) ? s.best_t : FLT_MAX;
     h.prim_id = 0xFFFFFFFFu;
     h.node_id = s.kind ? s.hit_node : 0xFFFFFFFFu;
     h.info    = s.kind | (s.flags << 8) | ((s.segments < 65535u ? s.segments : 65535u) << 16);
@@ -73,7 +80,63 @@ __device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, const RayB
     h.albedo[1] = s.alb[1];
     h.albedo[2] = s.alb[2];
     h.reserved  = 0.f;
-    store_hit(batch_hits<PARTS>(b, s.part), s.ray_id, h);
+    return h;
+}
+
+// One evaluated segment (outputs z[0..7], pre-sigmoid) applied to the ray's result
+// (neural_bvh_module_common.cuh:160-209,497-550, path_inference_kernels.cu:61-131): visibility
+// decision on the logit, LocalT depth decode, closest depth kept with its normal and albedo.
+// Returns true when the ray is finished (an any-hit query found an occluding segment).
+__device__ __forceinline__ bool apply_segment(const NetArgs& a, HitState& s, bool any_hit, float seg_mint,
+                                              float seg_maxt, uint32_t seg_node, const float z[8])
+{
+    s.segments++;
+    if (!(z[0] > a.thr_logit)) return false;
+    const bool inside = seg_mint < 0.f;
+    if (any_hit) {
+        bool h = true;
+        if (inside) h = decode_depth(sigmoidf_cw(z[1]), seg_mint, seg_maxt) != FLT_MAX;
+        if (h) {
+            s.kind      = NBVH_HIT_NEURAL;
+            s.hit_node  = seg_node;
+            s.hit_logit = z[0];
+        }
+        return h;
+    }
+    const float depth = decode_depth(sigmoidf_cw(z[1]), seg_mint, seg_maxt);
+    if (depth != FLT_MAX && depth < s.best_t) {
+        s.best_t    = depth;
+        s.kind      = NBVH_HIT_NEURAL;
+        s.hit_node  = seg_node;
+        s.hit_logit = z[0];
+        float v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c]     = sigmoidf_cw(z[2 + c]) * 2.f - 1.f;
+            s.alb[c] = sigmoidf_cw(z[5 + c]);
+        }
+        const float inv = 1.f / sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s.nrm[c] = v[c] * inv;
+    }
+    return false;
+}
+
+struct Slot {
+    uint32_t state;
+    uint32_t part;    // part of a multi-part launch (0 for a single array)
+    uint32_t ray_id;  // index within the part
+    bool any_hit;     // stop at the first accepted segment (shadow ray)
+    Ray ray;
+    float seg_mint, seg_maxt;
+    uint32_t seg_node;
+    HitState h;
+};
+
+template <bool PARTS>
+__device__ __forceinline__ void finish_ray(const NetArgs& a, Slot& s, const RayBatch& b)
+{
+    store_hit(batch_hits<PARTS>(b, s.part), s.ray_id, make_hit(s.h, s.any_hit));
     s.state = SLOT_EMPTY;
 }
 
@@ -181,14 +244,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes,
             s.state    = SLOT_RAY;
             s.ray      = load_ray(batch_rays<PARTS>(b, s.part), s.ray_id);
             s.any_hit  = batch_any_hit<PARTS>(b, s.part, a.any_hit);
-            s.best_t   = s.ray.tmax;
-            s.segments = 0;
-            s.flags    = 0;
-            s.kind     = 0;
-            s.hit_node = 0xFFFFFFFFu;
-            s.hit_logit = 0.f;
-            s.nrm[0] = s.nrm[1] = s.nrm[2] = 0.f;
-            s.alb[0] = s.alb[1] = s.alb[2] = 0.f;
+            hit_state_init(s.h, s.ray.tmax);
             st.n = 0;
             st.push(0u);
         }
@@ -196,19 +252,19 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes,
             if (st.empty()) {
                 finish_ray<PARTS>(a, s, b);
             } else {
-                const int r = nbvh_next_leaf(nodes, a.lod, s.ray, st, s.best_t, s.seg_mint, s.seg_maxt, s.seg_node,
-                                             kTraversalBudget);
+                const int r = nbvh_next_leaf(nodes, a.lod, s.ray, st, s.h.best_t, s.seg_mint, s.seg_maxt,
+                                             s.seg_node, kTraversalBudget);
                 if (r == 2) {
                     // suspended: stays SLOT_RAY
                 } else if (r == 1) {
-                    if (s.segments == a.max_steps + 1u) {
-                        s.flags |= NBVH_FLAG_STEP_CAP;
+                    if (s.h.segments == a.max_steps + 1u) {
+                        s.h.flags |= NBVH_FLAG_STEP_CAP;
                         finish_ray<PARTS>(a, s, b);
                     } else {
                         s.state = SLOT_SEG;
                     }
                 } else {
-                    if (r < 0) s.flags |= NBVH_FLAG_STACK_OVERFLOW;
+                    if (r < 0) s.h.flags |= NBVH_FLAG_STACK_OVERFLOW;
                     finish_ray<PARTS>(a, s, b);
                 }
             }
@@ -225,37 +281,8 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes,
 template <bool PARTS>
 __device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const float z[8], const RayBatch& b)
 {
-    s.segments++;
     s.state = SLOT_RAY;
-    if (!(z[0] > a.thr_logit)) return;
-    const bool inside = s.seg_mint < 0.f;
-    if (s.any_hit) {
-        bool h = true;
-        if (inside) h = decode_depth(sigmoidf_cw(z[1]), s.seg_mint, s.seg_maxt) != FLT_MAX;
-        if (h) {
-            s.kind      = NBVH_HIT_NEURAL;
-            s.hit_node  = s.seg_node;
-            s.hit_logit = z[0];
-            finish_ray<PARTS>(a, s, b);
-        }
-        return;
-    }
-    const float depth = decode_depth(sigmoidf_cw(z[1]), s.seg_mint, s.seg_maxt);
-    if (depth != FLT_MAX && depth < s.best_t) {
-        s.best_t    = depth;
-        s.kind      = NBVH_HIT_NEURAL;
-        s.hit_node  = s.seg_node;
-        s.hit_logit = z[0];
-        float v[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            v[c]     = sigmoidf_cw(z[2 + c]) * 2.f - 1.f;
-            s.alb[c] = sigmoidf_cw(z[5 + c]);
-        }
-        const float inv = 1.f / sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
-#pragma unroll
-        for (int c = 0; c < 3; c++) s.nrm[c] = v[c] * inv;
-    }
+    if (apply_segment(a, s.h, s.any_hit, s.seg_mint, s.seg_maxt, s.seg_node, z)) finish_ray<PARTS>(a, s, b);
 }
 
 }  // namespace dev

@@ -186,12 +186,37 @@ uint32_t counter_slot(nbvh_ctx* ctx)
 
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                        hipStream_t stream);
+void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+                         hipStream_t stream);
+
+// Kernel arguments of a neural query: the scene's network plus the query parameters.
+NetArgs neural_args(const nbvh_scene* s, const nbvh_query_params* p)
+{
+    require(p->kind <= NBVH_QUERY_TRIANGLE_ANY, "unknown query kind");
+    if (s->nbvh_nodes.empty()) fail(NBVH_ERR_STATE, "neural query on a scene without an N-BVH");
+    if (!s->has_net) fail(NBVH_ERR_STATE, "neural query on a scene without network weights");
+    require(p->lod < NBVH_NUM_LODS, "lod must be 0..6");
+    require(p->precision <= NBVH_PREC_FP8, "unknown precision");
+    NetArgs a = s->args;
+    a.lod       = p->lod;
+    a.max_steps = p->max_traversal_steps ? p->max_traversal_steps : kDefaultMaxSteps;
+    a.any_hit   = p->kind == NBVH_QUERY_NEURAL_ANY ? 1u : 0u;
+    const float thr = p->visibility_threshold == 0.f ? 0.5f : p->visibility_threshold;
+    require(thr > 0.f && thr < 1.f, "visibility_threshold must be in (0, 1)");
+    a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
+    return a;
+}
 
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                hipStream_t stream)
 {
     require(p != nullptr, "query params are NULL");
-    require((p->flags & ~(uint32_t)NBVH_QUERY_SORT_RAYS) == 0u, "unknown query flags");
+    require((p->flags & ~(uint32_t)(NBVH_QUERY_SORT_RAYS | NBVH_QUERY_WAVEFRONT)) == 0u, "unknown query flags");
+    if ((p->flags & NBVH_QUERY_WAVEFRONT) != 0u &&
+        (p->kind == NBVH_QUERY_NEURAL_CLOSEST || p->kind == NBVH_QUERY_NEURAL_ANY)) {
+        run_query_wavefront(ctx, s, p, batch, stream);
+        return;
+    }
     if ((p->flags & NBVH_QUERY_SORT_RAYS) == 0u || batch.n < 2 || batch.order != nullptr) {
         run_query_ordered(ctx, s, p, batch, stream);
         return;
@@ -217,6 +242,83 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
 }
 
+// Leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT): see kernels/wavefront.hip. Transient
+// device memory comes from the stream-ordered allocator; two small counters are read back per round.
+void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
+                         hipStream_t stream)
+{
+    const uint32_t n = batch.n;
+    if (n == 0) return;
+    require(batch.parts != nullptr || (batch.rays && batch.hits), "ray or hit buffer is NULL");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    NetArgs a = neural_args(s, p);
+    const uint32_t in_floats = 3 * a.n_points;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t sort_tmp = wf_sort_scratch_bytes(n);
+    size_t off = 0;
+    const size_t o_st = off;      off += al((size_t)n * sizeof(WfRayState));
+    const size_t o_stack = off;   off += al((size_t)n * kNeuralStackSize * sizeof(uint32_t));
+    const size_t o_act0 = off;    off += al((size_t)n * 4);
+    const size_t o_act1 = off;    off += al((size_t)n * 4);
+    const size_t o_segs = off;    off += al((size_t)n * sizeof(WfSeg));
+    const size_t o_keys = off;    off += al((size_t)n * 4);
+    const size_t o_vals = off;    off += al((size_t)n * 4);
+    const size_t o_kout = off;    off += al((size_t)n * 4);
+    const size_t o_order = off;   off += al((size_t)n * 4);
+    const size_t o_in = off;      off += al((size_t)n * in_floats * 4);
+    const size_t o_logits = off;  off += al((size_t)n * 8 * 4);
+    const size_t o_cnt = off;     off += 256;
+    const size_t o_tmp = off;     off += al(sort_tmp);
+    char* mem = nullptr;
+    hip_check(hipMallocAsync((void**)&mem, off, stream), "hipMallocAsync");
+    auto U32 = [&](size_t o) { return reinterpret_cast<uint32_t*>(mem + o); };
+    WfRayState* st = reinterpret_cast<WfRayState*>(mem + o_st);
+    WfSeg* segs    = reinterpret_cast<WfSeg*>(mem + o_segs);
+    float* in      = reinterpret_cast<float*>(mem + o_in);
+    float* logits  = reinterpret_cast<float*>(mem + o_logits);
+    uint32_t* cnt  = U32(o_cnt);
+    uint32_t* active[2] = {U32(o_act0), U32(o_act1)};
+    int end_bit = 1;
+    while (end_bit < 32 && (1ull << end_bit) < (uint64_t)a.n_nodes) end_bit++;
+    auto read_u32 = [&](const uint32_t* d) {
+        uint32_t v = 0;
+        hip_check(hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        return v;
+    };
+    try {
+        hip_check(wf_init(batch, st, U32(o_stack), active[0], stream), "wavefront init");
+        uint32_t n_active = n;
+        int cur = 0;
+        while (n_active > 0) {
+            hip_check(zero_u32(cnt, 2, stream), "counter reset");
+            hip_check(wf_traverse(a, batch, st, U32(o_stack), active[cur], n_active, segs, U32(o_keys), U32(o_vals),
+                                  cnt, stream),
+                      "wavefront traverse");
+            const uint32_t m = read_u32(cnt);
+            if (m == 0) break;
+            hip_check(wf_sort(mem + o_tmp, al(sort_tmp), U32(o_keys), U32(o_kout), U32(o_vals), U32(o_order), m,
+                              end_bit, stream),
+                      "wavefront sort");
+            hip_check(wf_encode(a, batch, segs, U32(o_order), m, in, stream), "wavefront encode");
+            bool supported = false;
+            if (p->precision == NBVH_PREC_FP32)
+                hip_check(network_fp32(a, in, m, nullptr, logits, stream, &supported), "fp32 network");
+            else
+                hip_check(network_mfma(a, p->precision, in, m, nullptr, logits, (uint32_t)ctx->n_cus, stream, &supported), "mfma network");
+            if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no network kernel for this network shape / precision");
+            hip_check(wf_consume(a, batch, segs, U32(o_order), logits, m, st, active[1 - cur], cnt + 1, stream),
+                      "wavefront consume");
+            n_active = read_u32(cnt + 1);
+            cur      = 1 - cur;
+        }
+    } catch (...) {
+        (void)hipFreeAsync(mem, stream);
+        throw;
+    }
+    hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
+}
+
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                        hipStream_t stream)
 {
@@ -234,18 +336,7 @@ void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_para
         hip_check(query_triangles(t, batch, stream), "triangle query launch");
         return;
     }
-    require(p->kind <= NBVH_QUERY_TRIANGLE_ANY, "unknown query kind");
-    if (s->nbvh_nodes.empty()) fail(NBVH_ERR_STATE, "neural query on a scene without an N-BVH");
-    if (!s->has_net) fail(NBVH_ERR_STATE, "neural query on a scene without network weights");
-    require(p->lod < NBVH_NUM_LODS, "lod must be 0..6");
-    require(p->precision <= NBVH_PREC_FP8, "unknown precision");
-    NetArgs a = s->args;
-    a.lod       = p->lod;
-    a.max_steps = p->max_traversal_steps ? p->max_traversal_steps : kDefaultMaxSteps;
-    a.any_hit   = p->kind == NBVH_QUERY_NEURAL_ANY ? 1u : 0u;
-    const float thr = p->visibility_threshold == 0.f ? 0.5f : p->visibility_threshold;
-    require(thr > 0.f && thr < 1.f, "visibility_threshold must be in (0, 1)");
-    a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
+    NetArgs a = neural_args(s, p);
     // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
     const uint64_t w = p->image_width;
     a.image_width = (batch.parts == nullptr && batch.order == nullptr && w != 0 && w % 8 == 0 &&
@@ -279,7 +370,8 @@ void run_network(nbvh_ctx* ctx, const nbvh_scene* s, uint32_t precision, const f
     if (precision == NBVH_PREC_FP32)
         hip_check(network_fp32(s->args, d_in, n, d_out, d_logits, stream, &supported), "fp32 network");
     else
-        hip_check(network_mfma(s->args, precision, d_in, n, d_out, d_logits, stream, &supported), "mfma network");
+        hip_check(network_mfma(s->args, precision, d_in, n, d_out, d_logits, (uint32_t)ctx->n_cus, stream,
+                               &supported), "mfma network");
     if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no network kernel for this shape / precision");
 }
 

@@ -32,6 +32,21 @@ hipError_t ray_order(const RayBatch& rb, const float enc_min[3], const float enc
 hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream);
 hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream);
 
+// Leaf-sorted wavefront rounds (wavefront.hip): state init, traversal to the next leaf per active
+// ray, segment sort by leaf, encoding in leaf order, and the consume step after the network.
+hipError_t wf_init(const RayBatch& b, WfRayState* st, uint32_t* stacks, uint32_t* active, hipStream_t stream);
+hipError_t wf_traverse(const NetArgs& a, const RayBatch& b, WfRayState* st, uint32_t* stacks, const uint32_t* active,
+                       uint32_t n_active, WfSeg* segs, uint32_t* keys, uint32_t* vals, uint32_t* counter,
+                       hipStream_t stream);
+size_t wf_sort_scratch_bytes(uint32_t n);
+hipError_t wf_sort(void* tmp, size_t tmp_bytes, const uint32_t* keys, uint32_t* keys_out, const uint32_t* vals,
+                   uint32_t* order, uint32_t m, int end_bit, hipStream_t stream);
+hipError_t wf_encode(const NetArgs& a, const RayBatch& b, const WfSeg* segs, const uint32_t* order, uint32_t m,
+                     float* in, hipStream_t stream);
+hipError_t wf_consume(const NetArgs& a, const RayBatch& b, const WfSeg* segs, const uint32_t* order,
+                      const float* logits, uint32_t m, WfRayState* st, uint32_t* next, uint32_t* next_count,
+                      hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 
@@ -45,7 +60,9 @@ hipError_t network_fp32(const NetArgs& a, const float* in, uint32_t n, float* ou
 // MFMA paths: precision = NBVH_PREC_FP16 / BF16 / FP8.
 hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
                       hipStream_t stream, bool* supported);
+// The network kernels are persistent too: at most n_cus x (resident workgroups per CU) blocks,
+// each staging the weights once.
 hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
-                        hipStream_t stream, bool* supported);
+                        uint32_t n_cus, hipStream_t stream, bool* supported);
 
 }  // namespace nbvh

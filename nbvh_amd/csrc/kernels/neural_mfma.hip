@@ -521,17 +521,21 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetAr
     constexpr int NF = MlpShape<P * L * F>::N_FRAG;
     __shared__ uint4 wfrag[NF * 64];
     stage_weights<P, L, F, BF16>(a, wfrag);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float own[3 * P];
+    // Persistent over chunks of blockDim.x inputs (the chunk start is block-uniform, so every wave
+    // takes part in each MFMA); the weights are staged once per resident workgroup.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        float own[3 * P];
 #pragma unroll
-    for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
-    float z[8];
-    mfma_network<P, L, F, BF16>(a, wfrag, own, z);
-    if (i < n) {
+        for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
+        float z[8];
+        mfma_network<P, L, F, BF16>(a, wfrag, own, z);
+        if (i < n) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
-            if (logits) logits[(size_t)i * 8 + j] = z[j];
+            for (int j = 0; j < 8; j++) {
+                if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
+                if (logits) logits[(size_t)i * 8 + j] = z[j];
+            }
         }
     }
 }
@@ -574,17 +578,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_fp8_kernel(NetArg
 {
     __shared__ Fp8Lds<P * L * F> w;
     stage_weights_fp8<P * L * F>(a, w);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    float own[3 * P];
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        float own[3 * P];
 #pragma unroll
-    for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
-    float z[8];
-    mfma_network_fp8<P, L, F>(a, w, own, z);
-    if (i < n) {
+        for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
+        float z[8];
+        mfma_network_fp8<P, L, F>(a, w, own, z);
+        if (i < n) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
-            if (logits) logits[(size_t)i * 8 + j] = z[j];
+            for (int j = 0; j < 8; j++) {
+                if (out) out[(size_t)i * 8 + j] = sigmoidf_cw(z[j]);
+                if (logits) logits[(size_t)i * 8 + j] = z[j];
+            }
         }
     }
 }
@@ -625,12 +631,15 @@ hipError_t launch_query(const NetArgs& a, const RayBatch& rb, uint32_t* queue, u
 }
 
 template <int P, int L, int F, bool BF16>
-hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits,
+hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
                           hipStream_t stream)
 {
+    auto k = dev::network_mfma_kernel<P, L, F, BF16>;
+    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
-    hipLaunchKernelGGL((dev::network_mfma_kernel<P, L, F, BF16>), dim3((n + per_block - 1) / per_block),
-                       dim3(per_block), 0, stream, a, in, n, out, logits);
+    uint32_t blocks = (n + per_block - 1) / per_block;
+    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, in, n, out, logits);
     return hipGetLastError();
 }
 
@@ -656,12 +665,15 @@ hipError_t launch_query_fp8(const NetArgs& a, const RayBatch& rb, uint32_t* queu
 }
 
 template <int P, int L, int F>
-hipError_t launch_network_fp8(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits,
+hipError_t launch_network_fp8(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
                               hipStream_t stream)
 {
+    auto k = dev::network_fp8_kernel<P, L, F>;
+    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
-    hipLaunchKernelGGL((dev::network_fp8_kernel<P, L, F>), dim3((n + per_block - 1) / per_block), dim3(per_block),
-                       0, stream, a, in, n, out, logits);
+    uint32_t blocks = (n + per_block - 1) / per_block;
+    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, in, n, out, logits);
     return hipGetLastError();
 }
 
@@ -691,18 +703,18 @@ hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, 
 }
 
 hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
-                        hipStream_t stream, bool* supported)
+                        uint32_t n_cus, hipStream_t stream, bool* supported)
 {
     *supported = true;
     if (a.n_hidden_layers == 4 && a.mlp_frag != nullptr && a.tables_f16_bytes != 0) {
 #define X(P_, L_, F_)                                                                 \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                \
         if (precision == NBVH_PREC_FP16)                                             \
-            return launch_network<P_, L_, F_, false>(a, in, n, out, logits, stream);  \
+            return launch_network<P_, L_, F_, false>(a, in, n, out, logits, n_cus, stream);  \
         if (precision == NBVH_PREC_BF16)                                             \
-            return launch_network<P_, L_, F_, true>(a, in, n, out, logits, stream);   \
+            return launch_network<P_, L_, F_, true>(a, in, n, out, logits, n_cus, stream);   \
         if (precision == NBVH_PREC_FP8 && a.mlp_frag_fp8 != nullptr)                 \
-            return launch_network_fp8<P_, L_, F_>(a, in, n, out, logits, stream);     \
+            return launch_network_fp8<P_, L_, F_>(a, in, n, out, logits, n_cus, stream);     \
     }
         NBVH_MFMA_SHAPES(X)
 #undef X

@@ -87,4 +87,16 @@ struct RayBatch {
     const uint32_t*  order;  // optional claim order: claim i traces ray order[i] (a permutation of [0, n))
 };
 
+// One leaf segment of the leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT).
+struct WfSeg {
+    uint32_t ray;   // claim index of the ray in its launch
+    uint32_t node;  // N-BVH leaf
+    float    mint, maxt;
+};
+
+// Per-ray state of the wavefront rounds (64 B; the layout belongs to the kernels).
+struct WfRayState {
+    uint32_t words[16];
+};
+
 }  // namespace nbvh

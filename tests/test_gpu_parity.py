@@ -410,3 +410,29 @@ def test_hybrid_query_matches_oracle(nb, orc, ctx, bench, any_hit):
                            mode=orc.FP64)
     assert (nb.hit_kind(g16) != nb.hit_kind(e64)).mean() <= 0.005
     env.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16", "fp8"])
+def test_wavefront_mode_same_results(nb, orc, bench, precision):
+    """Leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT) give the persistent kernel's hit records
+    bit for bit: closest and any hit, step cap, threshold, and a multi-part launch."""
+    from nbvh_amd import hip
+    mod = bench["module"]
+    prec = {"fp32": nb.PREC_FP32, "fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp8": nb.PREC_FP8}[precision]
+    _, cam = camera_rays(nb, orc, 96, 64)
+    rays = np.concatenate([cam, random_rays(9000, 103, inside_frac=0.4)])
+    for kw in ({}, {"kind": nb.QUERY_NEURAL_ANY}, {"max_traversal_steps": 1}, {"visibility_threshold": 0.50001}):
+        a = mod.neural_bvh_traversal(rays, precision=prec, **kw)
+        b = mod.neural_bvh_traversal(rays, precision=prec, wavefront=True, **kw)
+        assert hits_equal(a, b), kw
+    if precision == "fp32":
+        assert hits_equal(b, bench["net"].neural_query(bench["nbvh"], rays, threshold=0.50001, mode=orc.FP32))
+    r2 = random_rays(3000, 104, inside_frac=0.5)
+    d1, d2 = hip.Buffer.from_array(rays), hip.Buffer.from_array(r2)
+    h1, h2 = hip.Buffer(len(rays) * 48), hip.Buffer(len(r2) * 48)
+    mod.query_parts_device(mod.params(precision=prec, wavefront=True),
+                           [(d1.ptr, len(rays), h1.ptr, False), (d2.ptr, len(r2), h2.ptr, True)])
+    hip.synchronize()
+    assert hits_equal(h1.download(nb.HIT_DTYPE), mod.neural_bvh_traversal(rays, precision=prec))
+    assert hits_equal(h2.download(nb.HIT_DTYPE),
+                      mod.neural_bvh_traversal(r2, kind=nb.QUERY_NEURAL_ANY, precision=prec))
