@@ -172,6 +172,24 @@ __device__ __forceinline__ float expf_cw(float x)
 }
 __device__ __forceinline__ float sigmoidf_cw(float z) { return 1.f / (1.f + expf_cw(-z)); }
 
+// 30-bit Morton code of a point given in [0,1]^3 (10 bits per axis, x in the lowest bit;
+// coordinates clamped, NaN -> 0).
+__device__ __forceinline__ uint32_t morton30(float x, float y, float z)
+{
+    const float q[3] = {x, y, z};
+    uint32_t m = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        uint32_t v = (uint32_t)(fminf(fmaxf(q[c], 0.f), 1.f) * 1023.f);
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        m |= v << c;
+    }
+    return m;
+}
+
 // Segment -> P encoded points in the neural scene box
 // (src/cuda/neural/neural_bvh_module_kernels.cu:10-97,266-303; EquidistantExterior/Interior).
 __device__ __forceinline__ void encode_segment(const NetArgs& a, const Ray& r, float mint, float maxt, float* pts)

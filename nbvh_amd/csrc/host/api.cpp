@@ -278,8 +278,7 @@ void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_pa
     float* logits  = reinterpret_cast<float*>(mem + o_logits);
     uint32_t* cnt  = U32(o_cnt);
     uint32_t* active[2] = {U32(o_act0), U32(o_act1)};
-    int end_bit = 1;
-    while (end_bit < 32 && (1ull << end_bit) < (uint64_t)a.n_nodes) end_bit++;
+    const int end_bit = 30;  // Morton keys of segment midpoints
     auto read_u32 = [&](const uint32_t* d) {
         uint32_t v = 0;
         hip_check(hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");

@@ -15,16 +15,6 @@
 namespace nbvh {
 namespace dev {
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v)
-{
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __global__ __launch_bounds__(256) void ray_keys_kernel(RayBatch b, float3 lo, float3 inv_ext,
                                                        unsigned long long* __restrict__ keys,
                                                        uint32_t* __restrict__ ids)
@@ -38,12 +28,7 @@ __global__ __launch_bounds__(256) void ray_keys_kernel(RayBatch b, float3 lo, fl
     }
     const Ray r = load_ray(b.parts ? b.parts[part].rays : b.rays, i);
     const float q[3] = {(r.ox - lo.x) * inv_ext.x, (r.oy - lo.y) * inv_ext.y, (r.oz - lo.z) * inv_ext.z};
-    uint32_t m = 0;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float t = fminf(fmaxf(q[c], 0.f), 1.f) * 1023.f;  // NaN -> 0
-        m |= spread10((uint32_t)t) << c;
-    }
+    const uint32_t m = morton30(q[0], q[1], q[2]);
     const uint32_t oct = (r.dx < 0.f ? 1u : 0u) | (r.dy < 0.f ? 2u : 0u) | (r.dz < 0.f ? 4u : 0u);
     keys[g] = ((unsigned long long)part << 33) | ((unsigned long long)oct << 30) | m;
     ids[g]  = g;

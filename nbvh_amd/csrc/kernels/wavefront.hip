@@ -4,12 +4,14 @@
 // rays those segments lie all over the scene and the hash-grid gathers miss in L1 and L2 (about
 // 3.5x the per-segment gather cost of coherent primary rays). Here a query runs in rounds, like
 // the reference's step loop (NeuralPathModule::neural_bvh_traversal,
-// src/cuda/neural/path/path_module.cu:54-137), but the round's segments are evaluated grouped by
-// N-BVH leaf, so the 64 segments of a wave share a leaf box and most of its grid cells:
+// src/cuda/neural/path/path_module.cu:54-137), but the round's segments are evaluated in the
+// order of their midpoints' Morton codes, so the 64 segments of a wave lie close together and
+// share coarse grid cells (measured on 4M incoherent rays: network time 28.5 ms unsorted,
+// 16.6 ms grouped by leaf, 14.5 ms in Morton order):
 //   traverse  every active ray advances to its next accepted leaf (one thread per ray, global
-//             SoA stack, high occupancy) and appends the segment, keyed by its leaf;
-//   sort      segment indices by leaf (radix sort);
-//   encode    the segments in leaf order (9 floats each);
+//             SoA stack, high occupancy) and appends the segment with its sort key;
+//   sort      segment indices by key (radix sort over 30 bits);
+//   encode    the segments in key order (9 floats each);
 //   network   the network kernels on the encoded segments (the same MLP code as the query);
 //   consume   apply_segment on each ray's segment (the same decision code as the persistent
 //             scheduler); finished rays write their hit record, the others stay active.
@@ -104,7 +106,11 @@ __global__ __launch_bounds__(256) void wf_traverse_kernel(NetArgs a, RayBatch b,
     if (res == 1 && w.h.segments != a.max_steps + 1u) {
         const uint32_t k = atomicAdd(counter, 1u);
         segs[k] = WfSeg{g, node, mint, maxt};
-        keys[k] = node;
+        // Sort key: Morton code of the segment's midpoint in the encoding box.
+        const float tm = 0.5f * (mint + maxt);
+        keys[k] = morton30((r.ox + tm * r.dx - a.enc_center[0]) / a.enc_extent[0] + 0.5f,
+                           (r.oy + tm * r.dy - a.enc_center[1]) / a.enc_extent[1] + 0.5f,
+                           (r.oz + tm * r.dz - a.enc_center[2]) / a.enc_extent[2] + 0.5f);
         vals[k] = k;
         st[g].stack_n = (uint32_t)stk.n;
         return;

@@ -18,7 +18,10 @@ CHESS = os.path.join(GOLDEN, "chess_nodes_32940_hash_18_lvl_8.nbvh")
 
 def hits_equal(a, b):
     """Bitwise equality of hit records (NaN-safe)."""
-    return np.array_equal(a.view(np.uint32).reshape(len(a), -1), b.view(np.uint32).reshape(len(b), -1))
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    w = a.dtype.itemsize // 4
+    return a.dtype == b.dtype and np.array_equal(a.view(np.uint32).reshape(len(a), w),
+                                                 b.view(np.uint32).reshape(len(b), w))
 
 
 @pytest.fixture(scope="module")
@@ -436,3 +439,84 @@ def test_wavefront_mode_same_results(nb, orc, bench, precision):
     assert hits_equal(h1.download(nb.HIT_DTYPE), mod.neural_bvh_traversal(rays, precision=prec))
     assert hits_equal(h2.download(nb.HIT_DTYPE),
                       mod.neural_bvh_traversal(r2, kind=nb.QUERY_NEURAL_ANY, precision=prec))
+
+
+def edge_rays(seed):
+    """Degenerate and boundary rays: axis-aligned directions (two zero components), one zero
+    component, origins exactly on the scene box faces, short tmax (stops before, inside or just
+    past the surface), empty intervals (tmin > tmax) and origins inside the object."""
+    from oracle import RAY_DTYPE
+    rng = np.random.default_rng(seed)
+    out = []
+    axes = np.concatenate([np.eye(3), -np.eye(3)])
+    for k in range(6):  # axis-aligned, origins off-axis in the box
+        o = rng.uniform(-1.2, 1.2, (300, 3))
+        o[:, k % 3] = 1.3 * (1 if k < 3 else -1) * -1
+        d = np.tile(axes[k], (300, 1))
+        out.append((o, d, 1e-4, np.float32(3.4028234663852886e38)))
+    o = rng.uniform(-1.5, 1.5, (600, 3))
+    d = rng.normal(size=(600, 3))
+    d[np.arange(600), rng.integers(0, 3, 600)] = 0.0  # one zero component
+    out.append((o, d, 1e-4, np.float32(3.4028234663852886e38)))
+    o = rng.uniform(-1.5, 1.5, (600, 3))
+    d = -o + rng.normal(scale=0.2, size=(600, 3))
+    out.append((o, d, 1e-4, rng.uniform(0.05, 2.5, 600)))  # short rays
+    out.append((o[:200], d[:200], 1.0, 0.5))  # empty interval
+    o = rng.uniform(-0.3, 0.3, (400, 3))  # inside the object
+    out.append((o, rng.normal(size=(400, 3)), 1e-4, np.float32(3.4028234663852886e38)))
+    rays = []
+    for o, d, tmin, tmax in out:
+        r = np.zeros(len(o), RAY_DTYPE)
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        r["o"], r["d"] = o.astype(np.float32), d.astype(np.float32)
+        r["tmin"], r["tmax"] = tmin, tmax
+        rays.append(r)
+    return np.concatenate(rays)
+
+
+def test_edge_rays_bit_exact(nb, orc, bench):
+    rays = edge_rays(31)
+    # Origins exactly on the scene box faces, moving along the face plane and into it.
+    bmin, bmax = bench["scene"].bbox()
+    face = np.zeros(300, orc.RAY_DTYPE)
+    rng = np.random.default_rng(3)
+    o = rng.uniform(bmin, bmax, (300, 3))
+    o[:150, 0], o[150:, 1] = bmin[0], bmax[1]
+    d = np.zeros((300, 3))
+    d[:150, 0], d[150:, 1] = 1.0, -1.0
+    d[::2, 2] = 0.3
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    face["o"], face["d"] = o.astype(np.float32), d.astype(np.float32)
+    face["tmin"], face["tmax"] = 0.0, np.float32(3.4028234663852886e38)
+    rays = np.concatenate([rays, face])
+    mod = bench["module"]
+    for any_hit in (False, True):
+        kind = nb.QUERY_NEURAL_ANY if any_hit else nb.QUERY_NEURAL_CLOSEST
+        got = mod.neural_bvh_traversal(rays, kind=kind, precision=nb.PREC_FP32)
+        exp = bench["net"].neural_query(bench["nbvh"], rays, any_hit=any_hit, mode=orc.FP32)
+        assert hits_equal(got, exp)
+        tkind = nb.QUERY_TRIANGLE_ANY if any_hit else nb.QUERY_TRIANGLE_CLOSEST
+        tgot = mod.neural_bvh_traversal(rays, kind=tkind)
+        texp = orc.tri_query(bench["v"], bench["t"], bench["nodes"], bench["prims"], rays, any_hit=any_hit)
+        assert np.array_equal(tgot["info"], texp["info"])
+        if not any_hit:
+            assert hits_equal(tgot, texp)
+    # Empty intervals never hit; the short rays end both ways.
+    k = nb.hit_kind(got)
+    assert 0.05 < (k != 0).mean() < 0.95
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp8"])
+def test_batch_size_independence(nb, orc, bench, precision):
+    """A ray's record does not depend on the batch around it: the first n rays of a batch, traced
+    alone for empty, single, ragged and wave-multiple n, give the records of the full batch."""
+    prec = {"fp32": nb.PREC_FP32, "fp16": nb.PREC_FP16, "fp8": nb.PREC_FP8}[precision]
+    _, cam = camera_rays(nb, orc, 64, 48)
+    rays = np.concatenate([random_rays(1500, 12, inside_frac=0.3), cam])
+    mod = bench["module"]
+    full = mod.neural_bvh_traversal(rays, precision=prec)
+    for n in (0, 1, 2, 63, 64, 65, 255, 257, 1000):
+        got = mod.neural_bvh_traversal(rays[:n], precision=prec)
+        assert len(got) == n and hits_equal(got, full[:n]), n
+    rev = mod.neural_bvh_traversal(rays[::-1].copy(), precision=prec)
+    assert hits_equal(rev[::-1], full)
