@@ -137,7 +137,7 @@ def wavefront_bench(nb, hip, ctx, steps):
         out[name + "_sorted"] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
         pw = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, wavefront=True)
         ms = time_it(lambda: mod.query_parts_device(pw, parts, stream=s.handle), 3)
-        out[name + "_leaf_sorted_wavefront"] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
+        out[name + "_wavefront_rounds"] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
         if name == "fp16":
             h1, h2 = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
             out["segments_per_ray"] = round(float((nb.hit_segments(h1).astype(np.int64).sum()
@@ -672,9 +672,9 @@ def main():
         wf = wavefront_bench(nb, hip, ctx, max(5, args.steps // 2))
         result["wavefront"] = wf
         log(f"wavefront: fp16 {wf['fp16']['mrays_per_s']} Mrays/s (graph {wf['fp16_graph']['mrays_per_s']}, "
-            f"sorted {wf['fp16_sorted']['mrays_per_s']}, leaf-sorted rounds "
-            f"{wf['fp16_leaf_sorted_wavefront']['mrays_per_s']}), fp32 {wf['fp32']['mrays_per_s']} (sorted "
-            f"{wf['fp32_sorted']['mrays_per_s']}, rounds {wf['fp32_leaf_sorted_wavefront']['mrays_per_s']}), "
+            f"sorted {wf['fp16_sorted']['mrays_per_s']}, Morton-sorted rounds "
+            f"{wf['fp16_wavefront_rounds']['mrays_per_s']}), fp32 {wf['fp32']['mrays_per_s']} (sorted "
+            f"{wf['fp32_sorted']['mrays_per_s']}, rounds {wf['fp32_wavefront_rounds']['mrays_per_s']}), "
             f"triangles {wf['triangle_fallback']['mrays_per_s']}")
 
     if not args.no_serving and prec != nb.PREC_FP32:

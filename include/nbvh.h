@@ -165,13 +165,16 @@ enum nbvh_query_flags {
      * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray, so a
      * sorted query is not meant for HIP graph capture. */
     NBVH_QUERY_SORT_RAYS = 1u,
-    /* Neural kinds: evaluate in leaf-sorted wavefront rounds instead of the persistent kernel.
-     * Each round traverses every active ray to its next leaf (one thread per ray), sorts the
-     * round's segments by N-BVH leaf and evaluates them in that order, so the hash-grid gathers of
-     * a wave fall in one leaf box. Same hit records as the persistent kernel; pays off for
-     * incoherent batches (bounces, shadow rays of a wavefront path tracer). Synchronises the
-     * stream once per round (two 4-byte reads) and uses stream-ordered transient memory of about
-     * 300 B per ray; not for graph capture. */
+    /* Neural kinds: evaluate in sorted wavefront rounds instead of the persistent kernel, the
+     * step-loop schedule of NeuralPathModule::neural_bvh_traversal
+     * (src/cuda/neural/path/path_module.cu:54-137). Each round traverses every active ray to its
+     * next leaf (one thread per ray), sorts the round's segments by the Morton code of their
+     * midpoints and evaluates them in that order. Same hit records as the persistent kernel. On
+     * 4M incoherent rays it is slower than NBVH_QUERY_SORT_RAYS (about 29 ms vs 21 ms: one round
+     * per segment of the longest ray, each with a host synchronisation); it is kept for callers
+     * that want the reference's round structure. Synchronises the stream once per round (two
+     * 4-byte reads) and uses stream-ordered transient memory of about 300 B per ray; not for
+     * graph capture. */
     NBVH_QUERY_WAVEFRONT = 2u
 };
 

@@ -44,7 +44,7 @@ struct LdsStack {
 enum SlotState : uint32_t { SLOT_EMPTY = 0, SLOT_RAY = 1, SLOT_SEG = 2, SLOT_DONE = 3, SLOT_NEW = 4 };
 
 // Per-ray result of a neural query, built up segment by segment (shared by the persistent wave
-// scheduler and the leaf-sorted wavefront rounds, so both make identical decisions).
+// scheduler and the sorted wavefront rounds, so both make identical decisions).
 struct HitState {
     float best_t;  // closest decoded depth so far (starts at the ray's tmax)
     uint32_t segments, flags, kind, hit_node;

@@ -1,4 +1,4 @@
-// wavefront.hip -- leaf-sorted wavefront rounds of the neural query (NBVH_QUERY_WAVEFRONT).
+// wavefront.hip -- sorted wavefront rounds of the neural query (NBVH_QUERY_WAVEFRONT).
 //
 // The persistent kernels evaluate the segments of whatever 64 rays a wave holds; for incoherent
 // rays those segments lie all over the scene and the hash-grid gathers miss in L1 and L2 (about

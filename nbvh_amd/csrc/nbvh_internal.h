@@ -87,7 +87,7 @@ struct RayBatch {
     const uint32_t*  order;  // optional claim order: claim i traces ray order[i] (a permutation of [0, n))
 };
 
-// One leaf segment of the leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT).
+// One leaf segment of the sorted wavefront rounds (NBVH_QUERY_WAVEFRONT).
 struct WfSeg {
     uint32_t ray;   // claim index of the ray in its launch
     uint32_t node;  // N-BVH leaf
