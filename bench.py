@@ -365,6 +365,46 @@ def hybrid_path_traced_bench(nb, hip, ctx, scene, W=1920, H=1080, bounces=4):
             "per_bounce": rows}
 
 
+def path_trace_driver_bench(nb, hip, ctx, scene, W=1920, H=1080, max_depth=5, rr_start_depth=3, frames=3):
+    """Native path-trace driver (nbvh_path_trace): 1920x1080 at 1 spp, 5 path vertices (4 diffuse
+    bounces), Russian roulette from depth 3, sky background, device-side compaction, bounces in
+    coherence order. Neural icosphere alone and inside the triangle environment (hybrid).
+    Frame time with HIP events around the whole call (it synchronises once per depth)."""
+    env = nb.Scene(ctx, *hybrid_environment(nb))
+    mod = nb.NeuralBVHModule(ctx, scene)
+    acc = hip.Buffer(W * H * 16)
+    s = hip.Stream()
+    out = {"workload": f"{W}x{H}, 1 spp, max_depth {max_depth} (4 diffuse bounces), Russian roulette from depth "
+                       f"{rr_start_depth}, sky background; 'neural': icosphere 81,920 tris as N-BVH 256 + headline "
+                       "network; 'hybrid': the same inside a floor quad and 4 geodesic spheres (triangles)"}
+    for scene_name, cam, environment in (
+            ("neural", nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H), None),
+            ("hybrid", nb.camera_look_at((0.5, 0.6, 3.2), (0, -0.2, 0), (0, 1, 0), 45.0, W, H), env)):
+        for name, prec in (("fp16", nb.PREC_FP16), ("fp32", nb.PREC_FP32)):
+            kw = dict(environment=environment, precision=prec, max_depth=max_depth, rr_start_depth=rr_start_depth,
+                      stream=s.handle)
+            mod.path_trace_device(cam, W, H, acc.ptr, **kw)
+            s.synchronize()
+            ms, counts = [], None
+            for f in range(frames):
+                e0, e1 = hip.Event(), hip.Event()
+                e0.record(s)
+                counts = mod.path_trace_device(cam, W, H, acc.ptr, sample=f, **kw)
+                e1.record(s)
+                s.synchronize()
+                ms.append(e0.elapsed_ms(e1))
+            img = acc.download(np.float32).reshape(-1, 4)
+            m = float(np.median(ms))
+            out[f"{scene_name}_{name}"] = {"ms_per_frame": round(m, 3), "rays_per_depth": counts,
+                                           "mrays_per_s": round(sum(counts) / (m * 1e-3) / 1e6, 1),
+                                           "mean_radiance": round(float((img[:, :3] / img[:, 3:4]).mean()), 5)}
+    s.destroy()
+    env.close()
+    log("path-trace driver: " + ", ".join(f"{k} {v['ms_per_frame']} ms ({v['mrays_per_s']} Mrays/s)"
+                                          for k, v in out.items() if isinstance(v, dict)))
+    return out
+
+
 def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     """Continuous-batching serving run over three resident scenes.
 
@@ -635,6 +675,7 @@ def main():
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)
             result["path_traced"] = pt
             result["hybrid_path_traced"] = hybrid_path_traced_bench(nb, hip, ctx, scene)
+            result["path_trace_driver"] = path_trace_driver_bench(nb, hip, ctx, scene)
 
         if not args.no_cpu_baseline:
             # CPU baseline: the oracle's FP32 path (same traversal + hash grid + FP32 MLP), all

@@ -360,6 +360,57 @@ int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_sc
                           const nbvh_query_params* params, const nbvh_ray* d_rays, uint32_t n_rays, nbvh_hit* d_hits,
                           void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Path tracing: the reference's NeuralPathModule::neural_path_tracing
+ * (src/cuda/neural/path/path_module.cu:139-245) with its scatter step
+ * hybrid_scatter_and_get_next_tlas_traversal_step (src/cuda/neural/path/path_inference_kernels.cu:
+ * 224-429), restricted to diffuse materials and a procedural background.
+ * ------------------------------------------------------------------------------------------- */
+
+/* What an escaped ray sees (background_hit_eval, path_inference_kernels.cu:37-58). */
+enum nbvh_background {
+    NBVH_BACKGROUND_CONSTANT = 0, /* background_color (black = the reference's envmap_eval off) */
+    NBVH_BACKGROUND_SKY      = 1  /* skylike_background: a = (d.y + 1)/2, (1-a)*(1,1,1) + a*(0.5,0.7,1) */
+};
+
+typedef struct nbvh_path_params {
+    nbvh_query_params query;      /* neural query of the primary rays (kind: NBVH_QUERY_NEURAL_CLOSEST) */
+    uint32_t bounce_query_flags;  /* nbvh_query_flags of the bounce rays (query.flags is for depth 1) */
+    uint32_t max_depth;           /* path vertices, 1..16; primary hits have depth 1 (m_max_depth) */
+    uint32_t rr_start_depth;      /* Russian roulette at depths >= this (rr_terminate,
+                                   * include/cuda/utils/rendering.cuh:24-40); 0: off */
+    uint32_t sample;              /* 0: the frame overwrites d_accum; > 0: it adds to it */
+    uint32_t background;          /* nbvh_background */
+    float background_color[3];
+    float environment_albedo[3];  /* diffuse albedo of the triangle environment's meshes */
+    float normal_offset;          /* next origin = hit + offset * shading normal; 0: 1e-4
+                                   * (normal_epsilon_offset, include/cuda/neural/path/path_module.h:220) */
+    uint32_t reserved[4];
+} nbvh_path_params;
+
+typedef struct nbvh_path_stats {
+    uint32_t rays[16];     /* rays traced at depth d in rays[d - 1] */
+    uint32_t reserved[16];
+} nbvh_path_stats;
+
+/* One frame of 1-spp diffuse path tracing into d_accum (width*height float4, pixel y*width + x:
+ * rgb radiance sum and the sample count in w, like accumulate_pixel_value,
+ * include/cuda/neural/neural_bvh_renderer_kernels.cuh:81-110; divide by w to display).
+ * Per depth: closest-hit query over the live paths (hybrid with `environment` when it is not
+ * NULL: its triangle meshes are diffuse with environment_albedo), then one shading kernel. A path
+ * that escapes adds throughput * background and ends; a hit at max_depth ends; otherwise the
+ * throughput is multiplied by the hit's albedo (the network's albedo for neural hits), Russian
+ * roulette may end the path, and the next ray leaves in a cosine-distributed direction about the
+ * shading normal. Live paths are compacted on the device. Every path writes its pixel exactly
+ * once, so the frame is deterministic. Random numbers: the raygen LCG with frame = depth and
+ * linear = pixel (two draws for the direction, one for Russian roulette), independent of the
+ * compaction order. Primary rays: nbvh_raygen with (sample, frame 0).
+ * Synchronises `stream` once per depth (the live-path count); uses transient stream-ordered
+ * device memory of about 180 B per pixel. stats may be NULL. */
+int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
+                        const nbvh_path_params* params, const nbvh_camera* camera, uint32_t width, uint32_t height,
+                        float* d_accum, nbvh_path_stats* stats, void* stream);
+
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                         const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);

@@ -14,7 +14,7 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8, QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY,
                    QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY, HIT_NONE, HIT_NEURAL, HIT_TRIANGLE,
                    FLAG_STACK_OVERFLOW, FLAG_STEP_CAP, NO_NEURAL_CHILD, SEGMENT_EQUIDISTANT_EXTERIOR,
-                   SEGMENT_EQUIDISTANT_INTERIOR)
+                   SEGMENT_EQUIDISTANT_INTERIOR, BACKGROUND_CONSTANT, BACKGROUND_SKY)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
 __all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "NbvhError", "network_config", "icosphere", "bvh_build",
@@ -364,6 +364,40 @@ class NeuralBVHModule:
         self.query_hybrid_device(triangles, self.params(kind, precision, **kw), dr.ptr, len(rays), dh.ptr)
         hip.synchronize()
         return dh.download(HIT_DTYPE)
+
+    def path_trace_device(self, camera, width, height, d_accum_ptr, environment=None, precision=PREC_FP32,
+                          max_depth=5, rr_start_depth=0, sample=0, background=BACKGROUND_SKY,
+                          background_color=(0.0, 0.0, 0.0), environment_albedo=(0.8, 0.8, 0.8), normal_offset=0.0,
+                          sort_bounces=True, lod=0, max_traversal_steps=0, visibility_threshold=0.0, stream=None):
+        """One frame of 1-spp diffuse path tracing into the device float4 buffer d_accum_ptr
+        (width*height; rgb sum and sample count). `environment`: a triangle Scene traced with the
+        neural object (hybrid), or None. Returns the rays traced per depth (list)."""
+        p = _ffi.PathParams()
+        p.query = self.params(QUERY_NEURAL_CLOSEST, precision, lod, max_traversal_steps, visibility_threshold,
+                              image_width=width if width % 8 == 0 and height % 8 == 0 else 0)
+        p.bounce_query_flags = _ffi.QUERY_SORT_RAYS if sort_bounces else 0
+        p.max_depth, p.rr_start_depth, p.sample, p.background = max_depth, rr_start_depth, sample, background
+        p.background_color[:] = [float(c) for c in background_color]
+        p.environment_albedo[:] = [float(c) for c in environment_albedo]
+        p.normal_offset = normal_offset
+        stats = _ffi.PathStats()
+        self.ctx.check(_lib().nbvh_path_trace(self.ctx.handle, self.scene.handle,
+                                              environment.handle if environment is not None else None,
+                                              ctypes.byref(p), ctypes.byref(camera), width, height,
+                                              ctypes.c_void_p(d_accum_ptr), ctypes.byref(stats),
+                                              ctypes.c_void_p(stream or 0)))
+        return [int(r) for r in stats.rays[:max_depth]]
+
+    def path_trace(self, camera, width, height, samples=1, **kw):
+        """Host image [height, width, 4] (rgb sums, sample count) of `samples` frames."""
+        from . import hip
+        accum = hip.Buffer(max(1, width * height) * 16)
+        rays = []
+        for s in range(samples):
+            rays.append(self.path_trace_device(camera, width, height, accum.ptr, sample=s, **kw))
+        hip.synchronize()
+        img = accum.download(np.float32, width * height * 4).reshape(height, width, 4)
+        return img, rays
 
     def query_parts_device(self, params, parts, stream=None):
         """One launch over several device ray arrays: parts = [(d_rays_ptr, n, d_hits_ptr, any_hit)].

@@ -27,6 +27,7 @@ ERR_CAPACITY = 9
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
 QUERY_SORT_RAYS = 1
 QUERY_WAVEFRONT = 2
+BACKGROUND_CONSTANT, BACKGROUND_SKY = 0, 1
 QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
@@ -72,6 +73,17 @@ class CreateInfo(ctypes.Structure):
 class QueryPart(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p), ("n_rays", ctypes.c_uint32),
                 ("any_hit", ctypes.c_uint32)]
+
+
+class PathParams(ctypes.Structure):
+    _fields_ = [("query", QueryParams), ("bounce_query_flags", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("rr_start_depth", ctypes.c_uint32), ("sample", ctypes.c_uint32), ("background", ctypes.c_uint32),
+                ("background_color", ctypes.c_float * 3), ("environment_albedo", ctypes.c_float * 3),
+                ("normal_offset", ctypes.c_float), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class PathStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint32 * 16), ("reserved", ctypes.c_uint32 * 16)]
 
 
 class LaunchShape(ctypes.Structure):
@@ -152,6 +164,8 @@ SIGNATURES = {
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
     "nbvh_query_hybrid": [P, P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
+    "nbvh_path_trace": [P, P, P, ctypes.POINTER(PathParams), ctypes.POINTER(Camera), U32, U32, P,
+                        ctypes.POINTER(PathStats), P],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_scene_get_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_autotune": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P, ctypes.POINTER(LaunchShape),

@@ -242,7 +242,7 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
 }
 
-// Leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT): see kernels/wavefront.hip. Transient
+// Sorted wavefront rounds (NBVH_QUERY_WAVEFRONT): see kernels/wavefront.hip. Transient
 // device memory comes from the stream-ordered allocator; two small counters are read back per round.
 void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                          hipStream_t stream)
@@ -961,6 +961,35 @@ NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_pa
     });
 }
 
+namespace nbvh {
+
+// Hybrid closest/any hit (see nbvh_query_hybrid): triangles first, the neural query in front of
+// each triangle hit, the neural record replacing the triangle's when there is one.
+void run_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles, const nbvh_query_params* p,
+                const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits, hipStream_t st)
+{
+    const bool any = p->kind == NBVH_QUERY_NEURAL_ANY;
+    nbvh_query_params tp = *p;
+    tp.kind  = any ? NBVH_QUERY_TRIANGLE_ANY : NBVH_QUERY_TRIANGLE_CLOSEST;
+    tp.flags = p->flags & NBVH_QUERY_SORT_RAYS;
+    run_query(ctx, triangles, &tp, RayBatch{d_rays, d_hits, nullptr, 0, n}, st);
+    void* mem = nullptr;
+    hip_check(hipMallocAsync(&mem, (size_t)n * (sizeof(nbvh_ray) + sizeof(nbvh_hit)), st), "hipMallocAsync");
+    nbvh_ray* clipped = static_cast<nbvh_ray*>(mem);
+    nbvh_hit* nh      = reinterpret_cast<nbvh_hit*>(clipped + n);
+    try {
+        hip_check(hybrid_clip(d_rays, d_hits, n, clipped, st), "hybrid clip");
+        run_query(ctx, neural, p, RayBatch{clipped, nh, nullptr, 0, n}, st);
+        hip_check(hybrid_merge(d_hits, nh, n, any, st), "hybrid merge");
+    } catch (...) {
+        (void)hipFreeAsync(mem, st);
+        throw;
+    }
+    hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
+}
+
+}  // namespace nbvh
+
 NBVH_API int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles,
                                    const nbvh_query_params* p, const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits,
                                    void* stream)
@@ -973,20 +1002,81 @@ NBVH_API int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, cons
         if (n == 0) return;
         require(d_rays && d_hits, "NULL buffer");
         if (triangles->bvh.empty()) fail(NBVH_ERR_STATE, "the triangle scene has no triangles");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        run_hybrid(ctx, neural, triangles, p, d_rays, n, d_hits, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
+                                 const nbvh_path_params* p, const nbvh_camera* cam, uint32_t width, uint32_t height,
+                                 float* d_accum, nbvh_path_stats* stats, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && neural && p && cam, "NULL argument");
+        require(neural->ctx == ctx && (!environment || environment->ctx == ctx), "scene belongs to another context");
+        require(p->query.kind == NBVH_QUERY_NEURAL_CLOSEST, "path tracing takes NBVH_QUERY_NEURAL_CLOSEST");
+        require(p->max_depth >= 1 && p->max_depth <= 16, "max_depth must be 1..16");
+        require(p->background <= NBVH_BACKGROUND_SKY, "unknown background");
+        require((uint64_t)width * height <= 0xFFFFFFFFull, "image too large");
+        if (environment && environment->bvh.empty()) fail(NBVH_ERR_STATE, "the environment has no triangles");
+        if (stats) *stats = nbvh_path_stats{};
+        const uint32_t n = width * height;
+        if (n == 0) return;
+        require(d_accum != nullptr, "NULL accumulation buffer");
         hipStream_t st = (hipStream_t)stream;
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        const bool any = p->kind == NBVH_QUERY_NEURAL_ANY;
-        nbvh_query_params tp = *p;
-        tp.kind = any ? NBVH_QUERY_TRIANGLE_ANY : NBVH_QUERY_TRIANGLE_CLOSEST;
-        run_query(ctx, triangles, &tp, RayBatch{d_rays, d_hits, nullptr, 0, n}, st);
-        void* mem = nullptr;
-        hip_check(hipMallocAsync(&mem, (size_t)n * (sizeof(nbvh_ray) + sizeof(nbvh_hit)), st), "hipMallocAsync");
-        nbvh_ray* clipped = static_cast<nbvh_ray*>(mem);
-        nbvh_hit* nh      = reinterpret_cast<nbvh_hit*>(clipped + n);
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        size_t off = 0;
+        const size_t o_rays0 = off;  off += al((size_t)n * sizeof(nbvh_ray));
+        const size_t o_rays1 = off;  off += al((size_t)n * sizeof(nbvh_ray));
+        const size_t o_st0 = off;    off += al((size_t)n * sizeof(PathState));
+        const size_t o_st1 = off;    off += al((size_t)n * sizeof(PathState));
+        const size_t o_hits = off;   off += al((size_t)n * sizeof(nbvh_hit));
+        const size_t o_cnt = off;    off += 256;
+        char* mem = nullptr;
+        hip_check(hipMallocAsync((void**)&mem, off, st), "hipMallocAsync");
+        nbvh_ray* rays[2] = {reinterpret_cast<nbvh_ray*>(mem + o_rays0), reinterpret_cast<nbvh_ray*>(mem + o_rays1)};
+        PathState* state[2] = {reinterpret_cast<PathState*>(mem + o_st0), reinterpret_cast<PathState*>(mem + o_st1)};
+        nbvh_hit* hits = reinterpret_cast<nbvh_hit*>(mem + o_hits);
+        uint32_t* cnt  = reinterpret_cast<uint32_t*>(mem + o_cnt);
+        PathArgs a{};
+        if (environment)
+            a.tri = TriArgs{environment->d_verts.as<float>(), environment->d_tris.as<uint32_t>(),
+                            environment->d_bvh.as<nbvh_node>(), environment->d_prims.as<uint32_t>(), 0u};
+        a.accum          = d_accum;
+        a.max_depth      = p->max_depth;
+        a.rr_start_depth = p->rr_start_depth;
+        a.sample         = p->sample;
+        a.background     = p->background;
+        for (int c = 0; c < 3; c++) {
+            a.background_color[c] = p->background_color[c];
+            a.env_albedo[c]       = p->environment_albedo[c];
+        }
+        a.normal_offset = p->normal_offset > 0.f ? p->normal_offset : 1e-4f;
         try {
-            hip_check(hybrid_clip(d_rays, d_hits, n, clipped, st), "hybrid clip");
-            run_query(ctx, neural, p, RayBatch{clipped, nh, nullptr, 0, n}, st);
-            hip_check(hybrid_merge(d_hits, nh, n, any, st), "hybrid merge");
+            hip_check(raygen(*cam, width, height, p->sample, 0u, rays[0], st), "raygen launch");
+            hip_check(path_init(state[0], n, st), "path init");
+            uint32_t cur = n;
+            int c = 0;
+            for (uint32_t depth = 1; depth <= p->max_depth && cur > 0; depth++) {
+                if (stats) stats->rays[depth - 1] = cur;
+                nbvh_query_params q = p->query;
+                if (depth > 1) {
+                    q.flags       = p->bounce_query_flags;
+                    q.image_width = 0;
+                }
+                if (environment)
+                    run_hybrid(ctx, neural, environment, &q, rays[c], cur, hits, st);
+                else
+                    run_query(ctx, neural, &q, RayBatch{rays[c], hits, nullptr, 0, cur}, st);
+                hip_check(zero_u32(cnt, 1, st), "counter reset");
+                hip_check(path_shade(a, depth, cur, rays[c], hits, state[c], rays[1 - c], state[1 - c], cnt, st),
+                          "path shade");
+                if (depth == p->max_depth) break;
+                hip_check(hipMemcpyAsync(&cur, cnt, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                c = 1 - c;
+            }
         } catch (...) {
             (void)hipFreeAsync(mem, st);
             throw;

@@ -2,34 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include "../device/common.h"
+#include "../device/sampling.h"
 #include "launch.h"
 
 namespace nbvh {
 namespace dev {
-
-// get_lcg_rng + lcg_randomf (include/cuda/utils/lcg_rng.cuh:12-72).
-__device__ __forceinline__ uint32_t murmur_mix(uint32_t hash, uint32_t k)
-{
-    k *= 0xcc9e2d51u;
-    k = (k << 15) | (k >> 17);
-    k *= 0x1b873593u;
-    hash ^= k;
-    return ((hash << 13) | (hash >> 19)) * 5u + 0xe6546b64u;
-}
-__device__ __forceinline__ uint32_t murmur_fmix(uint32_t h)
-{
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
-}
-__device__ __forceinline__ float lcg_float(uint32_t& state)
-{
-    state = state * 1664525u + 1013904223u;
-    return ldexpf((float)state, -32);
-}
 
 // camera_raygen_and_intersect_neural_bvh's ray (neural_bvh_module_common.cuh:230-236) and
 // camera_raygen (neural_bvh_renderer_kernels.cuh:74-79).
@@ -39,7 +16,7 @@ __global__ void raygen_kernel(nbvh_camera cam, uint32_t w, uint32_t h, uint32_t 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
     const uint32_t px = i % w, py = i / w;
-    uint32_t st = murmur_fmix(murmur_mix(murmur_mix(0u, i), 512u * frame + sample));
+    uint32_t st = lcg_seed(sample, frame, i);
     const float xf = ((float)px + lcg_float(st)) / (float)w;
     const float yf = ((float)py + lcg_float(st)) / (float)h;
     const float dx = (xf * cam.du[0] + yf * cam.dv[0]) + cam.dir_top_left[0];
@@ -57,29 +34,6 @@ __global__ void raygen_kernel(nbvh_camera cam, uint32_t w, uint32_t h, uint32_t 
 // (path_module.h:220). The shading normal is the hit's predicted normal (neural) or the
 // triangle's geometric normal, turned toward the incoming ray. Random numbers: the reference's
 // LCG seeded like raygen with frame = bounce + 1. Rays without a hit come out inactive (tmax = -1).
-__device__ __forceinline__ void ortho_basis(const float n[3], float t[3], float b[3])
-{
-    float bt[3] = {0.f, 0.f, 0.f};
-    if (n[0] < 0.6f && n[0] > -0.6f)
-        bt[0] = 1.f;
-    else if (n[1] < 0.6f && n[1] > -0.6f)
-        bt[1] = 1.f;
-    else if (n[2] < 0.6f && n[2] > -0.6f)
-        bt[2] = 1.f;
-    else
-        bt[0] = 1.f;
-    t[0] = bt[1] * n[2] - bt[2] * n[1];
-    t[1] = bt[2] * n[0] - bt[0] * n[2];
-    t[2] = bt[0] * n[1] - bt[1] * n[0];
-    float inv = 1.f / sqrtf((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
-    for (int c = 0; c < 3; c++) t[c] *= inv;
-    b[0] = n[1] * t[2] - n[2] * t[1];
-    b[1] = n[2] * t[0] - n[0] * t[2];
-    b[2] = n[0] * t[1] - n[1] * t[0];
-    inv = 1.f / sqrtf((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
-    for (int c = 0; c < 3; c++) b[c] *= inv;
-}
-
 __global__ void scatter_diffuse_kernel(TriArgs tri, const nbvh_ray* __restrict__ rays, const nbvh_hit* __restrict__ hits,
                                        uint32_t n, uint32_t sample, uint32_t bounce, nbvh_ray* __restrict__ out)
 {
@@ -87,56 +41,17 @@ __global__ void scatter_diffuse_kernel(TriArgs tri, const nbvh_ray* __restrict__
     if (i >= n) return;
     const Ray r = load_ray(rays, i);
     const nbvh_hit h = hits[i];
-    const uint32_t kind = h.info & 0xFFu;
     float4* o = reinterpret_cast<float4*>(out + i);
     float nrm[3];
-    if (kind == NBVH_HIT_NEURAL) {
-        nrm[0] = h.normal_or_uv[0];
-        nrm[1] = h.normal_or_uv[1];
-        nrm[2] = h.normal_or_uv[2];
-    } else if (kind == NBVH_HIT_TRIANGLE && tri.verts != nullptr) {
-        const float* a = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id];
-        const float* b = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id + 1];
-        const float* c = tri.verts + 3 * (size_t)tri.indices[3 * (size_t)h.prim_id + 2];
-        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
-        nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
-        nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
-    } else {
+    if (!shading_normal(tri, h, r.dx, r.dy, r.dz, nrm)) {
         o[0] = make_float4(0.f, 0.f, 0.f, 0.f);
         o[1] = make_float4(0.f, 0.f, 1.f, -1.f);
         return;
     }
-    const float len = sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
-    const float sgn = ((nrm[0] * r.dx + nrm[1] * r.dy) + nrm[2] * r.dz) > 0.f ? -1.f : 1.f;
-    const float inv = len > 0.f ? sgn / len : 0.f;
-    for (int c = 0; c < 3; c++) nrm[c] *= inv;
-    if (!(len > 0.f)) nrm[2] = 1.f;
-    uint32_t st = murmur_fmix(murmur_mix(murmur_mix(0u, i), 512u * (bounce + 1u) + sample));
+    uint32_t st = lcg_seed(sample, bounce + 1u, i);
     const float u0 = lcg_float(st), u1 = lcg_float(st);
-    // cos_sample_hemisphere: concentric disk mapping, z = sqrt(1 - r^2).
-    const float sx = 2.f * u0 - 1.f, sy = 2.f * u1 - 1.f;
-    float dx = 0.f, dy = 0.f;
-    if (sx != 0.f || sy != 0.f) {
-        float rad, th;
-        if (fabsf(sx) > fabsf(sy)) {
-            rad = sx;
-            th  = 0.785398163397448310f * (sy / sx);
-        } else {
-            rad = sy;
-            th  = 1.570796326794896619f - 0.785398163397448310f * (sx / sy);
-        }
-        dx = rad * cosf(th);
-        dy = rad * sinf(th);
-    }
-    float hz = sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy));
-    const float hinv = 1.f / sqrtf((dx * dx + dy * dy) + hz * hz);
-    const float hx = dx * hinv, hy = dy * hinv;
-    hz *= hinv;
-    float t[3], b[3];
-    ortho_basis(nrm, t, b);
     float wi[3];
-    for (int c = 0; c < 3; c++) wi[c] = (hx * t[c] + hy * b[c]) + hz * nrm[c];
+    (void)cosine_direction(u0, u1, nrm, wi);
     float p[3] = {r.ox + h.t * r.dx, r.oy + h.t * r.dy, r.oz + h.t * r.dz};
     for (int c = 0; c < 3; c++) p[c] = p[c] + nrm[c] * 1e-4f;
     o[0] = make_float4(p[0], p[1], p[2], 1e-4f);

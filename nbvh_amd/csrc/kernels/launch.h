@@ -47,6 +47,13 @@ hipError_t wf_consume(const NetArgs& a, const RayBatch& b, const WfSeg* segs, co
                       const float* logits, uint32_t m, WfRayState* st, uint32_t* next, uint32_t* next_count,
                       hipStream_t stream);
 
+// Path-trace driver: per-path state init (pixel = index, throughput 1) and one depth's shading
+// and compaction (live paths appended at *counter).
+hipError_t path_init(PathState* st, uint32_t n, hipStream_t stream);
+hipError_t path_shade(const PathArgs& a, uint32_t depth, uint32_t n, const nbvh_ray* rays, const nbvh_hit* hits,
+                      const PathState* st, nbvh_ray* next_rays, PathState* next_st, uint32_t* counter,
+                      hipStream_t stream);
+
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);
 

@@ -99,4 +99,25 @@ struct WfRayState {
     uint32_t words[16];
 };
 
+// One live path of the path-trace driver (32 B): its pixel, throughput and radiance so far.
+struct PathState {
+    uint32_t pixel;
+    float    thr[3];
+    float    illum[3];
+    uint32_t pad;
+};
+
+// Shading parameters of the path-trace driver.
+struct PathArgs {
+    TriArgs  tri;                  // triangle environment (normals of triangle hits); verts NULL: none
+    float*   accum;                // width*height float4
+    uint32_t max_depth;
+    uint32_t rr_start_depth;       // 0: no Russian roulette
+    uint32_t sample;
+    uint32_t background;           // nbvh_background
+    float    background_color[3];
+    float    env_albedo[3];
+    float    normal_offset;
+};
+
 }  // namespace nbvh

@@ -56,6 +56,7 @@ def lib():
             "oracle_num_threads": ([], ctypes.c_int),
             "oracle_raygen": ([vp, u32, u32, u32, u32, vp], None),
             "oracle_scatter_diffuse": ([vp, vp, vp, vp, u32, u32, u32, vp], None),
+            "oracle_path_shade": ([vp, vp, u32, u32, u32, u32, vp, vp, f32, u32, vp, vp, vp, u32, vp, vp, vp], u32),
             "oracle_slab": ([vp, vp, vp, vp, vp], ctypes.c_int),
             "oracle_dilate": ([vp, f32, vp, vp], None),
             "oracle_nbvh_leaf_sequence": ([vp, u32, vp, f32, vp, vp, vp, u32], ctypes.c_int),
@@ -248,6 +249,53 @@ def hybrid_query(net, nbvh_nodes, verts, tris, nodes, prims, rays, any_hit=False
     out = th.copy()
     out[take] = nh[take]
     return out
+
+
+PATH_STATE_DTYPE = np.dtype([("pixel", "<u4"), ("thr", "<f4", 3), ("illum", "<f4", 3), ("pad", "<u4")])
+
+
+def path_shade(depth, rays, hits, state, accum, max_depth=5, rr_start_depth=0, sample=0, background=1,
+               background_color=(0.0, 0.0, 0.0), env_albedo=(0.8, 0.8, 0.8), normal_offset=1e-4, verts=None,
+               tris=None):
+    """One depth of the path-trace restatement (oracle_path_shade): ending paths are written to
+    accum [n_pixels, 4] in place; returns the live paths' next rays and states."""
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    state = np.ascontiguousarray(state, PATH_STATE_DTYPE)
+    nr, ns = np.zeros(len(rays), RAY_DTYPE), np.zeros(len(rays), PATH_STATE_DTYPE)
+    bg = np.asarray(background_color, np.float32)
+    alb = np.asarray(env_albedo, np.float32)
+    v = None if verts is None else np.ascontiguousarray(verts, np.float32)
+    t = None if tris is None else np.ascontiguousarray(tris, np.uint32)
+    assert accum.dtype == np.float32 and accum.flags.c_contiguous
+    m = lib().oracle_path_shade(_p(v), _p(t), max_depth, rr_start_depth, sample, background, _p(bg), _p(alb),
+                                normal_offset, depth, _p(rays), _p(hits), _p(state), len(rays), _p(nr), _p(ns),
+                                _p(accum))
+    return nr[:m].copy(), ns[:m].copy()
+
+
+def path_trace(query, cam12, w, h, max_depth=5, rr_start_depth=0, sample=0, background=1,
+               background_color=(0.0, 0.0, 0.0), env_albedo=(0.8, 0.8, 0.8), normal_offset=1e-4, verts=None,
+               tris=None, accum=None):
+    """One frame of 1-spp diffuse path tracing (NeuralPathModule::neural_path_tracing,
+    src/cuda/neural/path/path_module.cu:139-245, diffuse materials): raygen (sample, frame 0), then
+    per depth query(rays) -> hits (the oracle's neural or hybrid closest hit) and path_shade.
+    Returns (accum [w*h, 4], rays traced per depth)."""
+    if accum is None:
+        accum = np.zeros((w * h, 4), np.float32)
+    rays = raygen(cam12, w, h, sample, 0)
+    state = np.zeros(w * h, PATH_STATE_DTYPE)
+    state["pixel"] = np.arange(w * h, dtype=np.uint32)
+    state["thr"] = 1.0
+    counts = []
+    for depth in range(1, max_depth + 1):
+        if len(rays) == 0:
+            break
+        counts.append(len(rays))
+        hits = query(rays)
+        rays, state = path_shade(depth, rays, hits, state, accum, max_depth, rr_start_depth, sample, background,
+                                 background_color, env_albedo, normal_offset, verts, tris)
+    return accum, counts
 
 
 def tri_brute(verts, tris, rays):

@@ -18,6 +18,9 @@
 //                                src/cuda/neural/path/path_inference_kernels.cu:61-131
 //   per-ray step loop            include/cuda/neural/neural_bvh_module_common.cuh:497-550,
 //                                src/cuda/neural/path/path_module.cu:54-137
+//   diffuse bounce, path shading src/cuda/neural/path/path_inference_kernels.cu:224-429 (diffuse
+//                                materials), include/cuda/base/material.cuh:101-120,193-201,
+//                                include/cuda/utils/rendering.cuh:11-40
 //   triangle BVH closest/any     include/cuda/accel/bvh2.cuh:13-88 (with the watertight test of
 //                                Woop, Benthin, Wald, JCGT 2013 in place of Moller-Trumbore,
 //                                include/cuda/accel/bvh.cuh:98-136, as the build specifies)
@@ -703,11 +706,81 @@ ORACLE_API void oracle_raygen(const float *cam, uint32_t w, uint32_t h, uint32_t
     }
 }
 
+// Shading normal of a hit turned toward the incoming direction d (shading_frame,
+// include/cuda/utils/rendering.cuh:11-22): predicted for a neural hit, geometric for a triangle
+// hit. False when the hit has no normal source.
+static bool shading_normal(const float *verts, const uint32_t *tris, const Hit &h, const float d[3], float nrm[3])
+{
+    const uint32_t kind = h.info & 0xFFu;
+    if (kind == 1u) {
+        for (int c = 0; c < 3; c++) nrm[c] = h.normal_or_uv[c];
+    } else if (kind == 2u && verts) {
+        const float *a = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id];
+        const float *b = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 1];
+        const float *c = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 2];
+        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    } else {
+        return false;
+    }
+    const float len = std::sqrt((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+    const float sgn = ((nrm[0] * d[0] + nrm[1] * d[1]) + nrm[2] * d[2]) > 0.f ? -1.f : 1.f;
+    const float inv = len > 0.f ? sgn / len : 0.f;
+    for (int c = 0; c < 3; c++) nrm[c] *= inv;
+    if (!(len > 0.f)) nrm[2] = 1.f;
+    return true;
+}
+
+// sample_lambertian_dir(n, ortho_basis(n), u) (material.cuh:101-120,193-201; utils/common.cuh:
+// 21-35): concentric disk mapping, z = sqrt(1 - r^2), into the frame of n. Returns the local z.
+static float lambertian_direction(float u0, float u1, const float nrm[3], float wi[3])
+{
+    const float sx = 2.f * u0 - 1.f, sy = 2.f * u1 - 1.f;
+    float dx = 0.f, dy = 0.f;
+    if (sx != 0.f || sy != 0.f) {
+        float rad, th;
+        if (std::fabs(sx) > std::fabs(sy)) {
+            rad = sx;
+            th  = 0.785398163397448310f * (sy / sx);
+        } else {
+            rad = sy;
+            th  = 1.570796326794896619f - 0.785398163397448310f * (sx / sy);
+        }
+        dx = rad * std::cos(th);
+        dy = rad * std::sin(th);
+    }
+    float hz         = std::sqrt(std::max(0.f, 1.f - dx * dx - dy * dy));
+    const float hinv = 1.f / std::sqrt((dx * dx + dy * dy) + hz * hz);
+    const float hx = dx * hinv, hy = dy * hinv;
+    hz *= hinv;
+    float bt[3] = {0.f, 0.f, 0.f}, t[3], b[3];
+    if (nrm[0] < 0.6f && nrm[0] > -0.6f)
+        bt[0] = 1.f;
+    else if (nrm[1] < 0.6f && nrm[1] > -0.6f)
+        bt[1] = 1.f;
+    else if (nrm[2] < 0.6f && nrm[2] > -0.6f)
+        bt[2] = 1.f;
+    else
+        bt[0] = 1.f;
+    t[0] = bt[1] * nrm[2] - bt[2] * nrm[1];
+    t[1] = bt[2] * nrm[0] - bt[0] * nrm[2];
+    t[2] = bt[0] * nrm[1] - bt[1] * nrm[0];
+    float ti = 1.f / std::sqrt((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
+    for (int c = 0; c < 3; c++) t[c] *= ti;
+    b[0] = nrm[1] * t[2] - nrm[2] * t[1];
+    b[1] = nrm[2] * t[0] - nrm[0] * t[2];
+    b[2] = nrm[0] * t[1] - nrm[1] * t[0];
+    ti = 1.f / std::sqrt((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
+    for (int c = 0; c < 3; c++) b[c] *= ti;
+    for (int c = 0; c < 3; c++) wi[c] = (hx * t[c] + hy * b[c]) + hz * nrm[c];
+    return hz;
+}
+
 // One diffuse bounce (path_inference_kernels.cu:380-393): the next ray leaves p = o + t*d + 1e-4*n
-// (normal_epsilon_offset, path_module.h:220) in direction sample_lambertian_dir(n, ortho_basis(n),
-// u) (material.cuh:101-120,193-201; utils/common.cuh:21-35), n the shading normal (predicted for a
-// neural hit, geometric for a triangle hit) turned toward the incoming ray, u two LCG floats
-// seeded with (sample, bounce + 1, ray). Hit-less rays come out with tmax = -1.
+// (normal_epsilon_offset, path_module.h:220) in a Lambertian direction about the shading normal
+// n, u two LCG floats seeded with (sample, bounce + 1, ray). Hit-less rays come out with tmax = -1.
 ORACLE_API void oracle_scatter_diffuse(const float *verts, const uint32_t *tris, const void *rays_in,
                                        const void *hits_in, uint32_t n, uint32_t sample, uint32_t bounce,
                                        void *rays_out)
@@ -716,78 +789,116 @@ ORACLE_API void oracle_scatter_diffuse(const float *verts, const uint32_t *tris,
     const Hit *hits = (const Hit *)hits_in;
     Ray *out        = (Ray *)rays_out;
     for (uint32_t i = 0; i < n; i++) {
-        const Ray &r   = rays[i];
-        const Hit &h   = hits[i];
-        const uint32_t kind = h.info & 0xFFu;
-        Ray &q = out[i];
+        const Ray &r = rays[i];
+        Ray &q       = out[i];
         float nrm[3];
-        if (kind == 1u) {
-            for (int c = 0; c < 3; c++) nrm[c] = h.normal_or_uv[c];
-        } else if (kind == 2u && verts) {
-            const float *a = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id];
-            const float *b = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 1];
-            const float *c = verts + 3 * (size_t)tris[3 * (size_t)h.prim_id + 2];
-            const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-            nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
-            nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
-            nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
-        } else {
+        if (!shading_normal(verts, tris, hits[i], r.d, nrm)) {
             q = Ray{{0.f, 0.f, 0.f}, 0.f, {0.f, 0.f, 1.f}, -1.f};
             continue;
         }
-        const float len = std::sqrt((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
-        const float sgn = ((nrm[0] * r.d[0] + nrm[1] * r.d[1]) + nrm[2] * r.d[2]) > 0.f ? -1.f : 1.f;
-        const float inv = len > 0.f ? sgn / len : 0.f;
-        for (int c = 0; c < 3; c++) nrm[c] *= inv;
-        if (!(len > 0.f)) nrm[2] = 1.f;
         uint32_t st    = lcg_seed(sample, bounce + 1u, i);
         const float u0 = lcg_float(st), u1 = lcg_float(st);
-        // cos_sample_hemisphere: concentric disk mapping.
-        const float sx = 2.f * u0 - 1.f, sy = 2.f * u1 - 1.f;
-        float dx = 0.f, dy = 0.f;
-        if (sx != 0.f || sy != 0.f) {
-            float rad, th;
-            if (std::fabs(sx) > std::fabs(sy)) {
-                rad = sx;
-                th  = 0.785398163397448310f * (sy / sx);
-            } else {
-                rad = sy;
-                th  = 1.570796326794896619f - 0.785398163397448310f * (sx / sy);
-            }
-            dx = rad * std::cos(th);
-            dy = rad * std::sin(th);
-        }
-        float hz         = std::sqrt(std::max(0.f, 1.f - dx * dx - dy * dy));
-        const float hinv = 1.f / std::sqrt((dx * dx + dy * dy) + hz * hz);
-        const float hx = dx * hinv, hy = dy * hinv;
-        hz *= hinv;
-        // ortho_basis
-        float bt[3] = {0.f, 0.f, 0.f}, t[3], b[3];
-        if (nrm[0] < 0.6f && nrm[0] > -0.6f)
-            bt[0] = 1.f;
-        else if (nrm[1] < 0.6f && nrm[1] > -0.6f)
-            bt[1] = 1.f;
-        else if (nrm[2] < 0.6f && nrm[2] > -0.6f)
-            bt[2] = 1.f;
-        else
-            bt[0] = 1.f;
-        t[0] = bt[1] * nrm[2] - bt[2] * nrm[1];
-        t[1] = bt[2] * nrm[0] - bt[0] * nrm[2];
-        t[2] = bt[0] * nrm[1] - bt[1] * nrm[0];
-        float ti = 1.f / std::sqrt((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]);
-        for (int c = 0; c < 3; c++) t[c] *= ti;
-        b[0] = nrm[1] * t[2] - nrm[2] * t[1];
-        b[1] = nrm[2] * t[0] - nrm[0] * t[2];
-        b[2] = nrm[0] * t[1] - nrm[1] * t[0];
-        ti = 1.f / std::sqrt((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
-        for (int c = 0; c < 3; c++) b[c] *= ti;
-        for (int c = 0; c < 3; c++) {
-            q.d[c] = (hx * t[c] + hy * b[c]) + hz * nrm[c];
-            q.o[c] = (r.o[c] + h.t * r.d[c]) + nrm[c] * 1e-4f;
-        }
+        (void)lambertian_direction(u0, u1, nrm, q.d);
+        for (int c = 0; c < 3; c++) q.o[c] = (r.o[c] + hits[i].t * r.d[c]) + nrm[c] * 1e-4f;
         q.tmin = 1e-4f;
         q.tmax = FLT_MAX;
     }
+}
+
+// Path state of the path-trace restatement: pixel, throughput, radiance so far (32 B).
+struct PathState {
+    uint32_t pixel;
+    float thr[3];
+    float illum[3];
+    uint32_t pad;
+};
+
+// One depth of diffuse path tracing (hybrid_scatter_and_get_next_tlas_traversal_step,
+// src/cuda/neural/path/path_inference_kernels.cu:224-429, diffuse materials only):
+//   no hit: illum += thr * background (background_hit_eval, :37-58; skylike or constant) and the
+//     path ends; depth == max_depth: the path ends;
+//   else thr *= albedo (Lambertian bsdf * |cos| / pdf), a zero pdf or albedo ends the path,
+//     Russian roulette at depth >= rr_start (rr_terminate, include/cuda/utils/rendering.cuh:24-40:
+//     survive with p = min(1, max(thr)), thr /= p), and the next ray leaves
+//     hit + normal_offset * n in a Lambertian direction; LCG seeded (sample, depth, pixel): two
+//     floats for the direction, then one for the roulette.
+// An ending path adds (illum, 1) to accum[pixel] (sample 0 overwrites; accumulate_pixel_value,
+// include/cuda/neural/neural_bvh_renderer_kernels.cuh:81-110). Live paths are appended in order
+// to next_rays / next_state; returns their count.
+ORACLE_API uint32_t oracle_path_shade(const float *verts, const uint32_t *tris, uint32_t max_depth, uint32_t rr_start,
+                                      uint32_t sample, uint32_t background, const float *bg_color,
+                                      const float *env_albedo, float normal_offset, uint32_t depth,
+                                      const void *rays_in, const void *hits_in, const void *state_in, uint32_t n,
+                                      void *next_rays_out, void *next_state_out, float *accum)
+{
+    const Ray *rays       = (const Ray *)rays_in;
+    const Hit *hits       = (const Hit *)hits_in;
+    const PathState *sts  = (const PathState *)state_in;
+    Ray *next_rays        = (Ray *)next_rays_out;
+    PathState *next_state = (PathState *)next_state_out;
+    uint32_t m            = 0;
+    auto splat = [&](const PathState &s) {
+        float *px = accum + 4 * (size_t)s.pixel;
+        for (int c = 0; c < 3; c++) px[c] = (sample ? px[c] : 0.f) + s.illum[c];
+        px[3] = (sample ? px[3] : 0.f) + 1.f;
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        const Ray &r = rays[i];
+        const Hit &h = hits[i];
+        PathState s  = sts[i];
+        const uint32_t kind = h.info & 0xFFu;
+        if (kind == 0u) {
+            float bg[3];
+            if (background == 1u) {
+                const float t = 0.5f * (r.d[1] + 1.f);
+                bg[0] = (1.f - t) * 1.f + t * 0.5f;
+                bg[1] = (1.f - t) * 1.f + t * 0.7f;
+                bg[2] = (1.f - t) * 1.f + t * 1.f;
+            } else {
+                for (int c = 0; c < 3; c++) bg[c] = bg_color[c];
+            }
+            for (int c = 0; c < 3; c++) s.illum[c] = s.illum[c] + s.thr[c] * bg[c];
+            splat(s);
+            continue;
+        }
+        if (depth >= max_depth) {
+            splat(s);
+            continue;
+        }
+        float nrm[3];
+        if (!shading_normal(verts, tris, h, r.d, nrm)) {
+            splat(s);
+            continue;
+        }
+        const float *alb = kind == 1u ? h.albedo : env_albedo;
+        uint32_t rng     = lcg_seed(sample, depth, s.pixel);
+        const float u0 = lcg_float(rng), u1 = lcg_float(rng);
+        float wi[3];
+        const float cos_i = lambertian_direction(u0, u1, nrm, wi);
+        for (int c = 0; c < 3; c++) s.thr[c] = s.thr[c] * alb[c];
+        if (!(cos_i > 0.f) || (alb[0] == 0.f && alb[1] == 0.f && alb[2] == 0.f)) {
+            splat(s);
+            continue;
+        }
+        if (rr_start != 0u && depth >= rr_start) {
+            const float survive = std::min(1.f, std::max(std::max(s.thr[0], s.thr[1]), s.thr[2]));
+            if (lcg_float(rng) < survive) {
+                for (int c = 0; c < 3; c++) s.thr[c] = s.thr[c] / survive;
+            } else {
+                splat(s);
+                continue;
+            }
+        }
+        Ray &q = next_rays[m];
+        for (int c = 0; c < 3; c++) {
+            q.o[c] = (r.o[c] + h.t * r.d[c]) + nrm[c] * normal_offset;
+            q.d[c] = wi[c];
+        }
+        q.tmin          = 1e-4f;
+        q.tmax          = FLT_MAX;
+        next_state[m++] = s;
+    }
+    return m;
 }
 
 // Slab test of one box (check_ray_bounds false): returns hit, writes tmin/tmax.
