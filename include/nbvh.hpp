@@ -227,11 +227,40 @@ public:
               ctx_);
         return out;
     }
+    // NeuralPathModule::neural_path_tracing: one 1-spp diffuse frame into d_accum (width*height
+    // float4: rgb sum, sample count). `environment` (may be NULL): triangle meshes traced with the
+    // neural object. Returns the rays traced per depth.
+    nbvh_path_stats neural_path_tracing(const nbvh_camera& camera, uint32_t width, uint32_t height, float* d_accum,
+                                        const nbvh_path_params& params, const Scene* environment = nullptr,
+                                        void* stream = nullptr)
+    {
+        nbvh_path_stats stats{};
+        check(nbvh_path_trace(ctx_, scene_, environment ? environment->get() : nullptr, &params, &camera, width,
+                              height, d_accum, &stats, stream),
+              ctx_);
+        return stats;
+    }
 
 private:
     nbvh_ctx* ctx_;
     nbvh_scene* scene_;
 };
+
+// Path-trace defaults of the reference's path module: max depth 5 (4 bounces), sky background.
+inline nbvh_path_params path_params(uint32_t precision = NBVH_PREC_FP16, uint32_t max_depth = 5,
+                                    uint32_t rr_start_depth = 0, uint32_t sample = 0)
+{
+    nbvh_path_params p{};
+    p.query.kind          = NBVH_QUERY_NEURAL_CLOSEST;
+    p.query.precision     = precision;
+    p.bounce_query_flags  = NBVH_QUERY_SORT_RAYS;
+    p.max_depth           = max_depth;
+    p.rr_start_depth      = rr_start_depth;
+    p.sample              = sample;
+    p.background          = NBVH_BACKGROUND_SKY;
+    p.environment_albedo[0] = p.environment_albedo[1] = p.environment_albedo[2] = 0.8f;
+    return p;
+}
 
 // Continuous-batching request loop (nbvh_server_*).
 class Server {

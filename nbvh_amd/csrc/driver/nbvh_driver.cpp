@@ -10,6 +10,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -142,17 +143,74 @@ int query(int argc, char** argv)
     return 0;
 }
 
+// Path-traced frames of the benchmark icosphere (nbvh_path_trace); writes the averaged image as
+// a binary PPM when --out is given.
+int path(int argc, char** argv)
+{
+    uint32_t W = 640, H = 480, spp = 4, depth = 5, rr = 3;
+    std::string prec = "fp16", out;
+    for (int i = 2; i + 1 < argc; i += 2) {
+        const std::string a = argv[i], v = argv[i + 1];
+        if (a == "--width") W = (uint32_t)std::stoul(v);
+        else if (a == "--height") H = (uint32_t)std::stoul(v);
+        else if (a == "--precision") prec = v;
+        else if (a == "--spp") spp = (uint32_t)std::stoul(v);
+        else if (a == "--max-depth") depth = (uint32_t)std::stoul(v);
+        else if (a == "--rr-start") rr = (uint32_t)std::stoul(v);
+        else if (a == "--out") out = v;
+        else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 2;
+        }
+    }
+    nbvh::Context ctx;
+    nbvh::Mesh m = nbvh::icosphere(6, 7, 0.1f);
+    nbvh::Scene scene(ctx, m);
+    scene.build_neural_bvh(256);
+    scene.init_network(nbvh::network_config(), 1, 1e-4f);
+    nbvh::NeuralBVHModule module(ctx, scene);
+    const float eye[3] = {0, 0, 3}, target[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    const nbvh_camera cam = nbvh::camera_look_at(eye, target, up, 45.f, W, H);
+    const uint32_t prec_id = prec == "fp32" ? NBVH_PREC_FP32 : prec == "bf16" ? NBVH_PREC_BF16 : NBVH_PREC_FP16;
+    float* d_accum = nullptr;
+    hip_ok(hipMalloc((void**)&d_accum, (size_t)W * H * 4 * sizeof(float)), "hipMalloc accum");
+    uint64_t rays = 0;
+    for (uint32_t s = 0; s < spp; s++) {
+        const nbvh_path_stats st = module.neural_path_tracing(cam, W, H, d_accum, nbvh::path_params(prec_id, depth, rr, s));
+        for (uint32_t d = 0; d < depth; d++) rays += st.rays[d];
+    }
+    std::vector<float> img((size_t)W * H * 4);
+    hip_ok(hipMemcpy(img.data(), d_accum, img.size() * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    double sum = 0.0;
+    for (size_t i = 0; i < (size_t)W * H; i++) sum += (img[4 * i] + img[4 * i + 1] + img[4 * i + 2]) / (3.0 * img[4 * i + 3]);
+    std::printf("%ux%u, %u spp, %s: %llu rays traced, mean radiance %.5f\n", W, H, spp, prec.c_str(),
+                (unsigned long long)rays, sum / ((double)W * H));
+    if (!out.empty()) {
+        std::ofstream f(out, std::ios::binary);
+        f << "P6\n" << W << " " << H << "\n255\n";
+        for (size_t i = 0; i < (size_t)W * H; i++)
+            for (int c = 0; c < 3; c++) {
+                const float v = img[4 * i + c] / img[4 * i + 3];
+                const float g = v <= 0.f ? 0.f : v >= 1.f ? 1.f : std::pow(v, 1.f / 2.2f);
+                f.put((char)(unsigned char)(g * 255.f + 0.5f));
+            }
+    }
+    (void)hipFree(d_accum);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: nbvh_driver host | query [options]\n");
+        std::fprintf(stderr, "usage: nbvh_driver host | query [options] | path [options]\n");
         return 2;
     }
     try {
         if (std::strcmp(argv[1], "host") == 0) return host_selftest();
         if (std::strcmp(argv[1], "query") == 0) return query(argc, argv);
+        if (std::strcmp(argv[1], "path") == 0) return path(argc, argv);
     } catch (const nbvh::Error& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

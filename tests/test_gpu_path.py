@@ -116,3 +116,25 @@ def test_path_trace_errors(nb, setup):
         setup["mod"].path_trace(cam, W, H, max_depth=17)
     img, counts = setup["mod"].path_trace(cam, 0, 0)
     assert img.size == 0 and counts == [[0] * 5]
+
+
+def test_cpp_driver_path_mode_matches_python(nb, setup, tmp_path):
+    """The C++ host driver's path mode (include/nbvh.hpp -> nbvh_path_trace) renders the same
+    frames as the Python host path: 4 spp, max depth 5, roulette from depth 3, FP16."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    out = tmp_path / "frame.ppm"
+    r = subprocess.run([os.path.join(ROOT, "nbvh_amd", "nbvh_driver"), "path", "--width", "128", "--height", "96",
+                        "--spp", "4", "--out", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = out.read_bytes()
+    header = b"P6\n128 96\n255\n"
+    assert data.startswith(header)
+    got = np.frombuffer(data[len(header):], np.uint8).reshape(96, 128, 3).astype(np.int32)
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, 128, 96)
+    img, _ = setup["mod"].path_trace(cam, 128, 96, samples=4, precision=nb.PREC_FP16, max_depth=5, rr_start_depth=3)
+    v = np.clip(img[..., :3] / img[..., 3:4], 0, 1) ** (1 / 2.2)
+    exp = np.floor(v * 255 + 0.5).astype(np.int32)
+    assert np.abs(got - exp).max() <= 1
+    assert got.mean() > 20
