@@ -248,6 +248,21 @@ int32_t nbvh_network_param_counts(const nbvh_network_config* cfg, uint64_t* n_ta
 int32_t nbvh_network_init_params(const nbvh_network_config* cfg, uint64_t seed, float table_range, float* tables,
                                  float* mlp);
 
+/* Weights in tiny-cuda-nn's flat parameter order, as the reference's trainer serialises them
+ * (NeuralBVHRenderer::save_network_model / load_network_model,
+ * src/cuda/neural/neural_bvh_renderer_init.cu:385-404: the "params_binary" float array of
+ * Trainer::serialize). Assumed order (public tiny-cuda-nn; no reference weights file or test
+ * exists to confirm it): NetworkWithInputEncoding puts the FullyFusedMLP's matrices first, each
+ * row-major [out x in] with the input width padded to a multiple of 16 and the output matrix padded
+ * to a multiple of 16 rows, then the grid's tables (level-major, entry-major, the order of
+ * nbvh_scene_load_network's tables). to_network drops the padding; to_tcnn writes it as zeros.
+ * n_params must equal the count for cfg (NBVH_ERR_INVALID_ARGUMENT otherwise). */
+int32_t nbvh_tcnn_params_to_network(const nbvh_network_config* cfg, const float* params, uint64_t n_params,
+                                    float* tables, float* mlp);
+/* params NULL: only *n_params is written. */
+int32_t nbvh_network_to_tcnn_params(const nbvh_network_config* cfg, const float* tables, const float* mlp,
+                                    float* params, uint64_t capacity, uint64_t* n_params);
+
 /* Pinhole camera from eye/target/up, vertical fov (degrees) and image size (update_camera). */
 int32_t nbvh_camera_look_at(const float eye[3], const float target[3], const float up[3], float fovy_degrees,
                             uint32_t width, uint32_t height, nbvh_camera* out);

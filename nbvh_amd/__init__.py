@@ -19,7 +19,7 @@ from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
 __all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "NbvhError", "network_config", "icosphere", "bvh_build",
            "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
-           "camera_look_at", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
+           "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
 
 def hit_kind(hits):
@@ -145,6 +145,50 @@ def network_init_params(cfg, seed, table_range=1e-4):
     return tables, mlp
 
 
+def tcnn_params_to_network(cfg, params):
+    """tiny-cuda-nn flat parameters (the reference's serialised model) -> (tables, mlp)."""
+    p = np.ascontiguousarray(params, np.float32)
+    nt, nm = network_param_counts(cfg)
+    tables, mlp = np.empty(nt, np.float32), np.empty(nm, np.float32)
+    _check(_lib().nbvh_tcnn_params_to_network(ctypes.byref(cfg), _ptr(p), len(p), _ptr(tables), _ptr(mlp)))
+    return tables, mlp
+
+
+def network_to_tcnn_params(cfg, tables, mlp):
+    """(tables, mlp) -> tiny-cuda-nn flat parameters (padding rows and columns zero)."""
+    n = ctypes.c_uint64()
+    _check(_lib().nbvh_network_to_tcnn_params(ctypes.byref(cfg), None, None, None, 0, ctypes.byref(n)))
+    t = np.ascontiguousarray(tables, np.float32)
+    m = np.ascontiguousarray(mlp, np.float32)
+    out = np.empty(n.value, np.float32)
+    _check(_lib().nbvh_network_to_tcnn_params(ctypes.byref(cfg), _ptr(t), _ptr(m), _ptr(out), n.value,
+                                              ctypes.byref(n)))
+    return out
+
+
+def read_tcnn_model(path):
+    """Flat float32 parameters of a model file written by the reference's save_network_model
+    (src/cuda/neural/neural_bvh_renderer_init.cu:385-392: the JSON text of tiny-cuda-nn's
+    Trainer::serialize, whose "params_binary" is a JSON binary value, dumped as
+    {"bytes": [...], "subtype": ...}). A MessagePack file of the same object is read too."""
+    import json
+    with open(path, "rb") as f:
+        raw = f.read()
+    try:
+        data = json.loads(raw.decode("utf-8"))
+    except (UnicodeDecodeError, ValueError):
+        import msgpack
+        data = msgpack.unpackb(raw, raw=False)
+    blob = data["params_binary"]
+    if isinstance(blob, dict):
+        blob = bytes(blob["bytes"])
+    params = np.frombuffer(bytes(blob), "<f4").copy()
+    n = int(data.get("n_params", len(params)))
+    if len(params) != n:
+        raise NbvhError(_ffi.ERR_FORMAT, f"params_binary holds {len(params)} floats, n_params says {n}")
+    return params
+
+
 def camera_look_at(eye, target, up, fovy_degrees, width, height):
     cam = Camera()
     e, t, u = (np.asarray(x, np.float32) for x in (eye, target, up))
@@ -257,6 +301,14 @@ class Scene:
         m = np.ascontiguousarray(mlp, np.float32)
         self.ctx.check(_lib().nbvh_scene_load_network(self.handle, ctypes.byref(cfg), _ptr(t), len(t), _ptr(m),
                                                       len(m)))
+
+    def load_tcnn_model(self, cfg, path_or_params):
+        """load_network_model from the reference's serialised model (a file path) or from a flat
+        tiny-cuda-nn parameter array."""
+        params = read_tcnn_model(path_or_params) if isinstance(path_or_params, (str, bytes)) or hasattr(
+            path_or_params, "__fspath__") else path_or_params
+        tables, mlp = tcnn_params_to_network(cfg, params)
+        self.load_network_model(cfg, tables, mlp)
 
     def init_network(self, cfg, seed, table_range=1e-4):
         self.ctx.check(_lib().nbvh_scene_init_network(self.handle, ctypes.byref(cfg), seed, table_range))

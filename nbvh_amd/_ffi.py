@@ -143,6 +143,8 @@ SIGNATURES = {
     "nbvh_nbvh_serialize": [P, U32, P, SZ, PSZ],
     "nbvh_network_param_counts": [ctypes.POINTER(NetworkConfig), PU64, PU64],
     "nbvh_network_init_params": [ctypes.POINTER(NetworkConfig), U64, F32, P, P],
+    "nbvh_tcnn_params_to_network": [ctypes.POINTER(NetworkConfig), P, U64, P, P],
+    "nbvh_network_to_tcnn_params": [ctypes.POINTER(NetworkConfig), P, P, P, U64, PU64],
     "nbvh_camera_look_at": [P, P, P, F32, U32, U32, ctypes.POINTER(Camera)],
     "nbvh_scene_create": [P, P, U32, P, U32, ctypes.POINTER(P)],
     "nbvh_scene_create_empty": [P, ctypes.POINTER(P)],

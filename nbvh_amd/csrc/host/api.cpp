@@ -574,6 +574,34 @@ NBVH_API int32_t nbvh_network_param_counts(const nbvh_network_config* cfg, uint6
     });
 }
 
+NBVH_API int32_t nbvh_tcnn_params_to_network(const nbvh_network_config* cfg, const float* params, uint64_t n_params,
+                                             float* tables, float* mlp)
+{
+    return guarded(nullptr, [&] {
+        require(cfg && params && tables && mlp, "NULL argument");
+        network_validate(*cfg);
+        if (n_params != tcnn_param_count(*cfg))
+            fail(NBVH_ERR_INVALID_ARGUMENT, "tcnn parameter count " + std::to_string(n_params) + " != " +
+                                                std::to_string(tcnn_param_count(*cfg)) + " for this network");
+        tcnn_params_split(*cfg, params, tables, mlp);
+    });
+}
+
+NBVH_API int32_t nbvh_network_to_tcnn_params(const nbvh_network_config* cfg, const float* tables, const float* mlp,
+                                             float* params, uint64_t capacity, uint64_t* n_params)
+{
+    return guarded(nullptr, [&] {
+        require(cfg != nullptr, "cfg is NULL");
+        network_validate(*cfg);
+        const uint64_t n = tcnn_param_count(*cfg);
+        if (n_params) *n_params = n;
+        if (!params) return;
+        require(tables && mlp, "NULL argument");
+        if (capacity < n) fail(NBVH_ERR_CAPACITY, "parameter buffer too small");
+        tcnn_params_join(*cfg, tables, mlp, params);
+    });
+}
+
 NBVH_API int32_t nbvh_network_init_params(const nbvh_network_config* cfg, uint64_t seed, float table_range,
                                           float* tables, float* mlp)
 {

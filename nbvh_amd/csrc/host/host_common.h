@@ -46,6 +46,10 @@ std::vector<GridLevel> grid_levels(const nbvh_grid_config& g);
 uint64_t table_param_count(const nbvh_network_config& cfg);
 uint64_t mlp_param_count(const nbvh_network_config& cfg);
 void network_init(const nbvh_network_config& cfg, uint64_t seed, float table_range, float* tables, float* mlp);
+// tcnn's flat parameter order (MLP matrices with padding, then tables) <-> tables + MLP.
+uint64_t tcnn_param_count(const nbvh_network_config& cfg);
+void tcnn_params_split(const nbvh_network_config& cfg, const float* params, float* tables, float* mlp);
+void tcnn_params_join(const nbvh_network_config& cfg, const float* tables, const float* mlp, float* params);
 
 // Half / bfloat16 / FP8 e4m3 conversions (round to nearest even).
 uint16_t float_to_half(float f);

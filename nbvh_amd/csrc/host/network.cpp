@@ -106,6 +106,52 @@ void network_init(const nbvh_network_config& cfg, uint64_t seed, float table_ran
     }
 }
 
+// tcnn's flat parameter vector of the reference's model (NetworkWithInputEncoding: the
+// FullyFusedMLP's matrices first, then the grid's tables; each matrix row-major [out x in]; the
+// output matrix padded to a multiple of 16 rows). Public tiny-cuda-nn convention; see the header.
+uint64_t tcnn_param_count(const nbvh_network_config& cfg)
+{
+    const uint64_t K0 = (uint64_t)cfg.n_points * cfg.grid.n_levels * cfg.grid.n_features_per_level;
+    const uint64_t K0p = (K0 + 15) / 16 * 16, H = cfg.n_neurons, Op = ((uint64_t)cfg.n_outputs + 15) / 16 * 16;
+    return H * K0p + (uint64_t)(cfg.n_hidden_layers - 1) * H * H + Op * H + table_param_count(cfg);
+}
+
+void tcnn_params_split(const nbvh_network_config& cfg, const float* params, float* tables, float* mlp)
+{
+    const uint32_t K0 = cfg.n_points * cfg.grid.n_levels * cfg.grid.n_features_per_level;
+    const uint32_t K0p = (K0 + 15) / 16 * 16, H = cfg.n_neurons, Op = (cfg.n_outputs + 15) / 16 * 16;
+    const float* src = params;
+    float* dst = mlp;
+    for (uint32_t layer = 0; layer <= cfg.n_hidden_layers; layer++) {
+        const uint32_t in = layer == 0 ? K0 : H, in_p = layer == 0 ? K0p : H;
+        const uint32_t out = layer == cfg.n_hidden_layers ? cfg.n_outputs : H;
+        const uint32_t out_p = layer == cfg.n_hidden_layers ? Op : H;
+        for (uint32_t r = 0; r < out; r++)
+            for (uint32_t c = 0; c < in; c++) *dst++ = src[(size_t)r * in_p + c];
+        src += (size_t)out_p * in_p;
+    }
+    const uint64_t nt = table_param_count(cfg);
+    for (uint64_t i = 0; i < nt; i++) tables[i] = src[i];
+}
+
+void tcnn_params_join(const nbvh_network_config& cfg, const float* tables, const float* mlp, float* params)
+{
+    const uint32_t K0 = cfg.n_points * cfg.grid.n_levels * cfg.grid.n_features_per_level;
+    const uint32_t K0p = (K0 + 15) / 16 * 16, H = cfg.n_neurons, Op = (cfg.n_outputs + 15) / 16 * 16;
+    const float* src = mlp;
+    float* dst = params;
+    for (uint32_t layer = 0; layer <= cfg.n_hidden_layers; layer++) {
+        const uint32_t in = layer == 0 ? K0 : H, in_p = layer == 0 ? K0p : H;
+        const uint32_t out = layer == cfg.n_hidden_layers ? cfg.n_outputs : H;
+        const uint32_t out_p = layer == cfg.n_hidden_layers ? Op : H;
+        for (uint32_t r = 0; r < out_p; r++)
+            for (uint32_t c = 0; c < in_p; c++) dst[(size_t)r * in_p + c] = (r < out && c < in) ? *src++ : 0.f;
+        dst += (size_t)out_p * in_p;
+    }
+    const uint64_t nt = table_param_count(cfg);
+    for (uint64_t i = 0; i < nt; i++) dst[i] = tables[i];
+}
+
 // ----- number formats -----
 uint16_t float_to_half(float f)
 {

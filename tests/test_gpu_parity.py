@@ -520,3 +520,23 @@ def test_batch_size_independence(nb, orc, bench, precision):
         assert len(got) == n and hits_equal(got, full[:n]), n
     rev = mod.neural_bvh_traversal(rays[::-1].copy(), precision=prec)
     assert hits_equal(rev[::-1], full)
+
+
+def test_tcnn_model_file_loads_the_same_network(nb, orc, ctx, bench, tmp_path):
+    """A model written in the reference's serialised form (tiny-cuda-nn flat order, JSON text)
+    and loaded with Scene.load_tcnn_model traces the same records as the original weights."""
+    import json
+    cfg, tables, mlp = bench["scene"].network()
+    params = nb.network_to_tcnn_params(cfg, tables, mlp)
+    path = tmp_path / "model.json"
+    path.write_text(json.dumps({"n_params": len(params), "params_binary": {"bytes": list(params.tobytes())}}))
+    v, t = bench["v"], bench["t"]
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    scene.load_tcnn_model(cfg, str(path))
+    _, rays = camera_rays(nb, orc, 64, 64)
+    for prec in (nb.PREC_FP32, nb.PREC_FP16):
+        a = bench["module"].neural_bvh_traversal(rays, precision=prec)
+        b = nb.NeuralBVHModule(ctx, scene).neural_bvh_traversal(rays, precision=prec)
+        assert hits_equal(a, b)
+    scene.close()

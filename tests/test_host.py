@@ -226,3 +226,50 @@ def test_cpp_driver_host_side(nb):
     assert r.returncode == 0, r.stderr
     assert "host ok: 1280 triangles" in r.stdout
     assert "N-BVH nodes" in r.stdout and "device:" in r.stdout
+
+
+def test_tcnn_parameter_order_round_trip(nb, tmp_path):
+    """tiny-cuda-nn's flat parameter order (MLP matrices first, input width and output rows padded
+    to 16, then the tables): the counts, the padding and the round trip through the reference's
+    serialised-model file format. The order itself is tiny-cuda-nn's public convention and stays
+    unpinned: no reference weights file or test exists."""
+    import json
+    import msgpack
+    cfg = nb.network_config()
+    tables, mlp = nb.network_init_params(cfg, seed=3)
+    params = nb.network_to_tcnn_params(cfg, tables, mlp)
+    assert len(params) == 64 * 96 + 3 * 64 * 64 + 16 * 64 + len(tables)
+    # Output matrix rows 8..15 are padding (zero); its rows 0..7 are the MLP's last matrix.
+    out = params[64 * 96 + 3 * 64 * 64: 64 * 96 + 3 * 64 * 64 + 16 * 64].reshape(16, 64)
+    assert not out[8:].any() and np.array_equal(out[:8], mlp[-8 * 64:].reshape(8, 64))
+    assert np.array_equal(params[:64 * 96].reshape(64, 96), mlp[:64 * 96].reshape(64, 96))
+    assert np.array_equal(params[-len(tables):], tables)
+    t2, m2 = nb.tcnn_params_to_network(cfg, params)
+    assert np.array_equal(t2, tables) and np.array_equal(m2, mlp)
+    # Padding is dropped on the way in.
+    p2 = params.copy()
+    p2[64 * 96 + 3 * 64 * 64 + 8 * 64: 64 * 96 + 3 * 64 * 64 + 16 * 64] = 7.0
+    assert np.array_equal(nb.tcnn_params_to_network(cfg, p2)[1], mlp)
+    with pytest.raises(nb.NbvhError):
+        nb.tcnn_params_to_network(cfg, params[:-1])
+    # A network whose input width is not a multiple of 16 (P*L*F = 3*2*2 = 12 -> 16).
+    small = nb.network_config(n_levels=2, n_features_per_level=2, log2_hashmap_size=10)
+    ts, ms = nb.network_init_params(small, seed=4)
+    ps = nb.network_to_tcnn_params(small, ts, ms)
+    first = ps[:64 * 16].reshape(64, 16)
+    assert not first[:, 12:].any() and np.array_equal(first[:, :12], ms[:64 * 12].reshape(64, 12))
+    assert np.array_equal(np.concatenate(nb.tcnn_params_to_network(small, ps)), np.concatenate([ts, ms]))
+    # The serialised-model file: nlohmann::json text of Trainer::serialize (binary value dumped as
+    # {"bytes": [...], "subtype": null}), and the same object as MessagePack.
+    model = {"n_params": len(params), "params_type": "__half",
+             "params_binary": {"bytes": list(params.astype("<f4").tobytes()), "subtype": None}}
+    path = tmp_path / "model.json"
+    path.write_text(json.dumps(model))
+    assert np.array_equal(nb.read_tcnn_model(str(path)), params)
+    mp = tmp_path / "model.msgpack"
+    mp.write_bytes(msgpack.packb({"n_params": len(params), "params_binary": params.astype("<f4").tobytes()}))
+    assert np.array_equal(nb.read_tcnn_model(str(mp)), params)
+    model["n_params"] += 1
+    path.write_text(json.dumps(model))
+    with pytest.raises(nb.NbvhError):
+        nb.read_tcnn_model(str(path))
