@@ -540,3 +540,55 @@ def test_tcnn_model_file_loads_the_same_network(nb, orc, ctx, bench, tmp_path):
         b = nb.NeuralBVHModule(ctx, scene).neural_bvh_traversal(rays, precision=prec)
         assert hits_equal(a, b)
     scene.close()
+
+
+@pytest.fixture(scope="module")
+def chess(nb, orc, ctx):
+    """The reference's chess N-BVH (25,249 nodes) with a seeded network of the shipped chess
+    configuration: 8 levels x 4 features, 2^18 entries, base 8, scale 2 (the (3, 8, 4) MFMA
+    kernels, 8-byte corner loads)."""
+    cfg = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=18, base_resolution=8,
+                            per_level_scale=2.0)
+    scene = nb.Scene(ctx)
+    scene.load_neural_bvh(CHESS)
+    scene.init_network(cfg, seed=9, table_range=1e-2)
+    _, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    rng = np.random.default_rng(17)
+    n = 6000
+    rays = np.zeros(n, orc.RAY_DTYPE)
+    o = rng.uniform(bmin - 0.2, bmax + 0.2, (n, 3))
+    d = rng.uniform(bmin, bmax, (n, 3)) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
+    yield dict(scene=scene, net=orc.Net(cfg, tables, mlp, bmin, bmax), nodes=scene.nbvh(), rays=rays,
+               mod=nb.NeuralBVHModule(ctx, scene))
+    scene.close()
+
+
+@pytest.mark.parametrize("precision,lod", [("fp16", 0), ("fp16", 6), ("bf16", 0)])
+def test_chess_mfma_query_vs_fp64(nb, orc, chess, precision, lod):
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16}[precision]
+    got = chess["mod"].neural_bvh_traversal(chess["rays"], precision=prec, lod=lod)
+    exp = chess["net"].neural_query(chess["nodes"], chess["rays"], lod=lod, mode=orc.FP64)
+    kg, ke = nb.hit_kind(got), nb.hit_kind(exp)
+    assert 0.05 < (ke == 1).mean() < 0.95
+    assert (kg != ke).mean() <= 0.005
+    same = (kg == 1) & (ke == 1) & (got["node_id"] == exp["node_id"])
+    rel = np.abs(got["t"][same] - exp["t"][same]) / np.maximum(exp["t"][same], 1e-3)
+    assert np.percentile(rel, 99) < 2e-2
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+def test_chess_network_mfma_vs_fp64(nb, orc, chess, precision):
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp8": nb.PREC_FP8}[precision]
+    _, log = chess["net"].neural_query(chess["nodes"], chess["rays"], mode=orc.FP64, log_segments=True)
+    x = chess["net"].encode(chess["rays"][log["ray"]], log["mint"], log["maxt"], mode=orc.FP32)
+    assert len(x) > 4000
+    _, z = chess["mod"].network_inference(x, precision=prec)
+    _, z64 = chess["net"].network(x, mode=orc.FP64)
+    err = np.abs(z - z64) / np.abs(z64).max(axis=0)
+    assert err.max() < {"fp16": 2e-2, "bf16": 8e-2, "fp8": 0.25}[precision], err.max(axis=0)
+    flips = np.mean(np.sign(z[:, 0]) != np.sign(z64[:, 0]))
+    assert flips <= {"fp16": 0.005, "bf16": 0.005, "fp8": 0.05}[precision], flips
