@@ -156,6 +156,59 @@ def wavefront_bench(nb, hip, ctx, steps):
     return out
 
 
+def chess_bench(nb, hip, ctx, oracle, W=1024, H=1024, steps=20):
+    """The reference's own N-BVH: the shipped chess .nbvh (25,249 nodes, 12,625 leaves; a test
+    fixture of this repository) with a seeded network of the shipped chess configuration (8 levels
+    x 4 features, 2^18 entries, base 8, scale 2). 1024x1024 primary rays at the board; FP16, BF16
+    and FP32 query time, and visibility flips of the FP16 network vs FP64 on every 16th row."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "chess_nodes_32940_hash_18_lvl_8.nbvh")
+    cfg = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=18, base_resolution=8,
+                            per_level_scale=2.0)
+    sc = nb.Scene(ctx)
+    sc.load_neural_bvh(path)
+    sc.init_network(cfg, seed=9, table_range=1e-2)
+    lo, hi = sc.bbox()
+    c = (lo + hi) * 0.5
+    ext = float(np.max(hi - lo))
+    cam = nb.camera_look_at(tuple(c + np.array([0.0, 0.9, 1.1]) * ext), tuple(c), (0, 1, 0), 45.0, W, H)
+    d_rays, d_hits = hip.Buffer(W * H * 32), hip.Buffer(W * H * 48)
+    ctx.raygen(cam, W, H, d_rays.ptr)
+    mod = nb.NeuralBVHModule(ctx, sc)
+    s = hip.Stream()
+    out = {"workload": f"reference chess N-BVH (25,249 nodes), hash grid 8x4 2^18 base 8 scale 2, MLP 96-64x4-8, "
+                       f"{W}x{H} primary rays"}
+    for name, prec, k in (("fp16", nb.PREC_FP16, steps), ("bf16", nb.PREC_BF16, steps), ("fp32", nb.PREC_FP32, 5)):
+        p = mod.params(precision=prec, image_width=W)
+        mod.query_device(p, d_rays.ptr, W * H, d_hits.ptr, stream=s.handle)
+        ev = [hip.Event() for _ in range(k + 1)]
+        ev[0].record(s)
+        for i in range(k):
+            mod.query_device(p, d_rays.ptr, W * H, d_hits.ptr, stream=s.handle)
+            ev[i + 1].record(s)
+        s.synchronize()
+        ms = float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(k)]))
+        hits = d_hits.download(nb.HIT_DTYPE)
+        out[name] = {"ms": round(ms, 4), "mrays_per_s": round(W * H / (ms * 1e-3) / 1e6, 1),
+                     "segments_per_ray": round(float(nb.hit_segments(hits).mean()), 3),
+                     "hit_fraction": round(float((nb.hit_kind(hits) != 0).mean()), 4)}
+    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
+    cfg2, tables, mlp = sc.network()
+    net = oracle.Net(cfg2, tables, mlp, lo, hi)
+    sample = d_rays.download(nb.RAY_DTYPE).reshape(H, W)[::16].reshape(-1)
+    _, seglog = net.neural_query(sc.nbvh(), sample, mode=oracle.FP64, log_segments=True)
+    x = net.encode(sample[seglog["ray"]], seglog["mint"], seglog["maxt"], mode=oracle.FP32)
+    _, z = mod.network_inference(x, precision=nb.PREC_FP16)
+    out["fp16_vis_flip_rate"] = round(float(np.mean((z[:, 0] > 0) != (seglog["logit"][:, 0] > 0))), 5)
+    out["flip_sample_segments"] = int(len(x))
+    s.destroy()
+    sc.close()
+    log(f"chess N-BVH: fp16 {out['fp16']['mrays_per_s']} / bf16 {out['bf16']['mrays_per_s']} / fp32 "
+        f"{out['fp32']['mrays_per_s']} Mrays/s, speedup {out['fp16_speedup_vs_fp32']}x, "
+        f"flips {out['fp16_vis_flip_rate']}")
+    return out
+
+
 def plain_triangles(nb, hip, ctx, scene, oracle, W=512, H=512):
     """The plain non-neural workload: 512x512 jittered primary rays, closest hit on the triangle BVH
     (binned SAH, leaf <= 2) of the 81,920-triangle icosphere. GPU kernel vs the oracle's CPU
@@ -670,6 +723,8 @@ def main():
 
         if not args.no_cpu_baseline:
             result["plain_triangles_512"] = plain_triangles(nb, hip, ctx, scene, oracle)
+        if prec != nb.PREC_FP32:
+            result["chess_nbvh"] = chess_bench(nb, hip, ctx, oracle)
 
         if not args.no_path_traced and prec != nb.PREC_FP32:
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)
