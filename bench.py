@@ -458,7 +458,7 @@ def path_trace_driver_bench(nb, hip, ctx, scene, W=1920, H=1080, max_depth=5, rr
     return out
 
 
-def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
+def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, max_batch_rays=1 << 21):
     """Continuous-batching serving run over three resident scenes.
 
     Scenes: the 81,920-triangle icosphere (N-BVH 256), a field of 64 geodesic spheres (2.17M
@@ -530,7 +530,7 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
     log(f"serving: capacity of the mix {capacity / 1e6:.0f} Mrays/s, mean request {mean_size:.0f} rays, "
         f"{rate:.0f} requests/s for {seconds:.1f} s ({n_req} requests, load {load})")
 
-    srv = nb.Server(ctx, n_streams=4, max_batch_rays=1 << 21)
+    srv = nb.Server(ctx, n_streams=n_streams, max_batch_rays=max_batch_rays)
     # Warm-up: every scene x kind once through the server.
     for i in range(3):
         for kind in (nb.QUERY_NEURAL_CLOSEST, nb.QUERY_NEURAL_ANY):
@@ -594,7 +594,7 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11):
         "requests_per_launch": round(st["n_batched_requests"] / max(1, st["n_launches"]), 2),
         "workload": "3 resident scenes (81,920 / 2.17M / 8.4M triangles; N-BVH 256 / 4096 / 16384), FP16, "
                     "Poisson arrivals, lognormal sizes median 64k in [4k, 2M], 70% closest / 30% shadow any-hit, "
-                    "device-resident rays, 4 streams",
+                    f"device-resident rays, {n_streams} streams, launches of at most {max_batch_rays} rays",
     }
 
 
