@@ -32,8 +32,9 @@ hipError_t ray_order(const RayBatch& rb, const float enc_min[3], const float enc
 hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream);
 hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream);
 
-// Leaf-sorted wavefront rounds (wavefront.hip): state init, traversal to the next leaf per active
-// ray, segment sort by leaf, encoding in leaf order, and the consume step after the network.
+// Sorted wavefront rounds (wavefront.hip): state init, traversal to the next leaf per active ray
+// (segments keyed by the Morton code of their midpoints), segment sort by key, encoding in key
+// order, and the consume step after the network.
 hipError_t wf_init(const RayBatch& b, WfRayState* st, uint32_t* stacks, uint32_t* active, hipStream_t stream);
 hipError_t wf_traverse(const NetArgs& a, const RayBatch& b, WfRayState* st, uint32_t* stacks, const uint32_t* active,
                        uint32_t n_active, WfSeg* segs, uint32_t* keys, uint32_t* vals, uint32_t* counter,

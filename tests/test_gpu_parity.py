@@ -417,7 +417,7 @@ def test_hybrid_query_matches_oracle(nb, orc, ctx, bench, any_hit):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16", "fp8"])
 def test_wavefront_mode_same_results(nb, orc, bench, precision):
-    """Leaf-sorted wavefront rounds (NBVH_QUERY_WAVEFRONT) give the persistent kernel's hit records
+    """Sorted wavefront rounds (NBVH_QUERY_WAVEFRONT) give the persistent kernel's hit records
     bit for bit: closest and any hit, step cap, threshold, and a multi-part launch."""
     from nbvh_amd import hip
     mod = bench["module"]
