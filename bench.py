@@ -209,6 +209,28 @@ def chess_bench(nb, hip, ctx, oracle, W=1024, H=1024, steps=20):
     return out
 
 
+def gather_ceiling(sizes_mib=(2, 4, 8, 16, 32, 64)):
+    """Random-gather line rate of the GPU (nbvh_amd/libnbvh_bench.so, csrc/bench/gather_ceiling.hip):
+    8 waves per CU, every lane of a load instruction in its own random 128-byte line of a table of
+    the given size. The ceiling the query's fine-level hash-grid gathers run against."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "nbvh_amd", "libnbvh_bench.so"))
+    fn = lib.nbvh_bench_gather_lines
+    fn.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_int
+    out = {"workload": "8 waves per CU, 64 distinct random 128-B lines per load instruction, 512 loads per lane"}
+    for mib in sizes_mib:
+        ms, rate = ctypes.c_float(), ctypes.c_double()
+        st = fn(mib << 20, 1, 8, 512, ctypes.byref(ms), ctypes.byref(rate))
+        if st != 0:
+            raise RuntimeError(f"gather ceiling failed with HIP error {st}")
+        out[f"{mib}MiB_glines_per_s"] = round(rate.value / 1e9, 1)
+    log("random-gather line rate (G lines/s): " + ", ".join(f"{k.split('MiB')[0]} MiB {v}" for k, v in out.items()
+                                                           if k.endswith("per_s")))
+    return out
+
+
 def plain_triangles(nb, hip, ctx, scene, oracle, W=512, H=512):
     """The plain non-neural workload: 512x512 jittered primary rays, closest hit on the triangle BVH
     (binned SAH, leaf <= 2) of the 81,920-triangle icosphere. GPU kernel vs the oracle's CPU
@@ -607,6 +629,7 @@ def main():
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gather-ceiling", action="store_true", help="skip the random-gather line-rate measurement")
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (continuous batching) run")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M mixed-ray wavefront run")
@@ -725,6 +748,8 @@ def main():
             result["plain_triangles_512"] = plain_triangles(nb, hip, ctx, scene, oracle)
         if prec != nb.PREC_FP32:
             result["chess_nbvh"] = chess_bench(nb, hip, ctx, oracle)
+        if not args.no_gather_ceiling:
+            result["gather_ceiling"] = gather_ceiling()
 
         if not args.no_path_traced and prec != nb.PREC_FP32:
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)

@@ -13,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnbvh_hip.so")
 DRIVER = os.path.join(HERE, "nbvh_driver")
+BENCH_LIB = os.path.join(HERE, "libnbvh_bench.so")
 ROOT = os.path.dirname(HERE)
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
@@ -61,6 +62,11 @@ def build(verbose=False, jobs=None):
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # Measurement helper of the benchmark (random-gather line rate), a library of its own.
+    bench_src = os.path.join(CSRC, "bench", "gather_ceiling.hip")
+    tmp = BENCH_LIB + ".tmp"
+    _run([hipcc] + HIP_FLAGS + ["-shared", bench_src, "-o", tmp])
+    os.replace(tmp, BENCH_LIB)
     # C++ host driver over include/nbvh.hpp, linked against the library just built.
     tmp = DRIVER + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),

@@ -273,3 +273,16 @@ def test_tcnn_parameter_order_round_trip(nb, tmp_path):
     path.write_text(json.dumps(model))
     with pytest.raises(nb.NbvhError):
         nb.read_tcnn_model(str(path))
+
+
+def test_gather_ceiling_helper_validates_arguments():
+    """The benchmark's line-rate helper library loads and rejects invalid shapes before any GPU
+    call (hipErrorInvalidValue = 1)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "nbvh_amd", "libnbvh_bench.so"))
+    fn = lib.nbvh_bench_gather_lines
+    fn.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_int
+    ms, rate = ctypes.c_float(), ctypes.c_double()
+    for args in ((64, 1, 8, 512), (1 << 20, 3, 8, 512), (1 << 20, 64, 8, 512), (1 << 20, 1, 8, 7), (1 << 20, 1, 6, 512)):
+        assert fn(*args, ctypes.byref(ms), ctypes.byref(rate)) == 1, args
