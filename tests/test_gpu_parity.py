@@ -189,17 +189,21 @@ def test_network_mfma_vs_fp64(nb, orc, bench, precision):
     assert len(x) > 5000
 
 
-def test_neural_query_fp16_vs_fp64(nb, orc, bench):
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+def test_neural_query_vs_fp64(nb, orc, bench, precision):
+    # Bars: hit/miss disagreement with the FP64 oracle <= 0.5 % (FP16, BF16; the headline bar) or
+    # <= 5 % (FP8 e4m3 hidden layers); p99 relative depth error on agreeing hits.
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp8": nb.PREC_FP8}[precision]
     _, rays = camera_rays(nb, orc, 128, 128)
     rays = np.concatenate([rays, random_rays(4000, 3, inside_frac=0.3)])
-    got = bench["module"].neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    got = bench["module"].neural_bvh_traversal(rays, precision=prec)
     exp = bench["net"].neural_query(bench["nbvh"], rays, mode=orc.FP64)
     kg, ke = nb.hit_kind(got), nb.hit_kind(exp)
-    assert (kg != ke).mean() <= 0.005
+    assert (kg != ke).mean() <= {"fp16": 0.005, "bf16": 0.005, "fp8": 0.05}[precision], (kg != ke).mean()
     same = (kg == 1) & (ke == 1) & (got["node_id"] == exp["node_id"])
     assert same.mean() > 0.3
     rel = np.abs(got["t"][same] - exp["t"][same]) / np.maximum(exp["t"][same], 1e-3)
-    assert np.percentile(rel, 99) < 1e-2
+    assert np.percentile(rel, 99) < {"fp16": 1e-2, "bf16": 3e-2, "fp8": 0.1}[precision], np.percentile(rel, 99)
 
 
 def test_errors_through_abi(nb, ctx):
