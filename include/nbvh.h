@@ -400,7 +400,11 @@ typedef struct nbvh_path_params {
     float environment_albedo[3];  /* diffuse albedo of the triangle environment's meshes */
     float normal_offset;          /* next origin = hit + offset * shading normal; 0: 1e-4
                                    * (normal_epsilon_offset, include/cuda/neural/path/path_module.h:220) */
-    uint32_t reserved[4];
+    uint32_t path_lod_switch;     /* nonzero: rays at depth > 1 use N-BVH LoD lod_switch instead of
+                                   * query.lod (path_lod_switch / lod_switch,
+                                   * include/cuda/neural/neural_bvh_module_common.cuh:333-335) */
+    uint32_t lod_switch;
+    uint32_t reserved[2];
 } nbvh_path_params;
 
 typedef struct nbvh_path_stats {

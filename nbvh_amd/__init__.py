@@ -420,10 +420,12 @@ class NeuralBVHModule:
     def path_trace_device(self, camera, width, height, d_accum_ptr, environment=None, precision=PREC_FP32,
                           max_depth=5, rr_start_depth=0, sample=0, background=BACKGROUND_SKY,
                           background_color=(0.0, 0.0, 0.0), environment_albedo=(0.8, 0.8, 0.8), normal_offset=0.0,
-                          sort_bounces=True, lod=0, max_traversal_steps=0, visibility_threshold=0.0, stream=None):
+                          sort_bounces=True, lod=0, bounce_lod=None, max_traversal_steps=0, visibility_threshold=0.0,
+                          stream=None):
         """One frame of 1-spp diffuse path tracing into the device float4 buffer d_accum_ptr
         (width*height; rgb sum and sample count). `environment`: a triangle Scene traced with the
-        neural object (hybrid), or None. Returns the rays traced per depth (list)."""
+        neural object (hybrid), or None. bounce_lod: N-BVH LoD of the rays at depth > 1 (None: lod).
+        Returns the rays traced per depth (list)."""
         p = _ffi.PathParams()
         p.query = self.params(QUERY_NEURAL_CLOSEST, precision, lod, max_traversal_steps, visibility_threshold,
                               image_width=width if width % 8 == 0 and height % 8 == 0 else 0)
@@ -432,6 +434,8 @@ class NeuralBVHModule:
         p.background_color[:] = [float(c) for c in background_color]
         p.environment_albedo[:] = [float(c) for c in environment_albedo]
         p.normal_offset = normal_offset
+        if bounce_lod is not None:
+            p.path_lod_switch, p.lod_switch = 1, bounce_lod
         stats = _ffi.PathStats()
         self.ctx.check(_lib().nbvh_path_trace(self.ctx.handle, self.scene.handle,
                                               environment.handle if environment is not None else None,

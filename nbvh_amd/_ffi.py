@@ -79,7 +79,8 @@ class PathParams(ctypes.Structure):
     _fields_ = [("query", QueryParams), ("bounce_query_flags", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("rr_start_depth", ctypes.c_uint32), ("sample", ctypes.c_uint32), ("background", ctypes.c_uint32),
                 ("background_color", ctypes.c_float * 3), ("environment_albedo", ctypes.c_float * 3),
-                ("normal_offset", ctypes.c_float), ("reserved", ctypes.c_uint32 * 4)]
+                ("normal_offset", ctypes.c_float), ("path_lod_switch", ctypes.c_uint32),
+                ("lod_switch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class PathStats(ctypes.Structure):

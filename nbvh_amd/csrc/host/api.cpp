@@ -1092,6 +1092,7 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
                 if (depth > 1) {
                     q.flags       = p->bounce_query_flags;
                     q.image_width = 0;
+                    if (p->path_lod_switch) q.lod = p->lod_switch;
                 }
                 if (environment)
                     run_hybrid(ctx, neural, environment, &q, rays[c], cur, hits, st);

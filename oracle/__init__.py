@@ -276,10 +276,11 @@ def path_shade(depth, rays, hits, state, accum, max_depth=5, rr_start_depth=0, s
 
 def path_trace(query, cam12, w, h, max_depth=5, rr_start_depth=0, sample=0, background=1,
                background_color=(0.0, 0.0, 0.0), env_albedo=(0.8, 0.8, 0.8), normal_offset=1e-4, verts=None,
-               tris=None, accum=None):
+               tris=None, accum=None, query_takes_depth=False):
     """One frame of 1-spp diffuse path tracing (NeuralPathModule::neural_path_tracing,
     src/cuda/neural/path/path_module.cu:139-245, diffuse materials): raygen (sample, frame 0), then
-    per depth query(rays) -> hits (the oracle's neural or hybrid closest hit) and path_shade.
+    per depth query(rays) -> hits (the oracle's neural or hybrid closest hit; query(rays, depth)
+    with query_takes_depth, e.g. for a LoD switch on bounces) and path_shade.
     Returns (accum [w*h, 4], rays traced per depth)."""
     if accum is None:
         accum = np.zeros((w * h, 4), np.float32)
@@ -292,7 +293,7 @@ def path_trace(query, cam12, w, h, max_depth=5, rr_start_depth=0, sample=0, back
         if len(rays) == 0:
             break
         counts.append(len(rays))
-        hits = query(rays)
+        hits = query(rays, depth) if query_takes_depth else query(rays)
         rays, state = path_shade(depth, rays, hits, state, accum, max_depth, rr_start_depth, sample, background,
                                  background_color, env_albedo, normal_offset, verts, tris)
     return accum, counts

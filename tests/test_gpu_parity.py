@@ -596,3 +596,46 @@ def test_chess_network_mfma_vs_fp64(nb, orc, chess, precision):
     assert err.max() < {"fp16": 2e-2, "bf16": 8e-2, "fp8": 0.25}[precision], err.max(axis=0)
     flips = np.mean(np.sign(z[:, 0]) != np.sign(z64[:, 0]))
     assert flips <= {"fp16": 0.005, "bf16": 0.005, "fp8": 0.05}[precision], flips
+
+
+def multi_lod_nbvh(nb, nodes, n_leaves=2048, marks=((3, 6), (5, 3))):
+    """An N-BVH cut whose interior nodes at the given depths are also LoD leaves, children kept
+    (the form the reference's LoD assignment leaves): marks = ((depth, lod), ...)."""
+    nbvh = nb.nbvh_cut(nodes, n_leaves)
+    depth = np.full(len(nbvh), -1)
+    depth[0] = 0
+    for i in range(len(nbvh)):
+        if nbvh["prim_count"][i] == 0 and nbvh["left_or_prim_offset"][i] != nb.NO_NEURAL_CHILD:
+            c = int(nbvh["left_or_prim_offset"][i])
+            depth[c] = depth[c + 1] = depth[i] + 1
+            for d, lod in marks:
+                if depth[i] == d:
+                    nbvh["prim_count"][i] = 2 + lod
+    assert np.all(depth >= 0)
+    return nbvh
+
+
+@pytest.mark.parametrize("lod", [0, 3, 6])
+def test_lod_selection_parity(nb, orc, ctx, bench, lod):
+    """Traversal at LoD l stops at interior nodes marked LEAF_LOD(<= l): FP32 bit-exact with the
+    oracle and FP16 within the visibility bar, on an N-BVH with LoD-marked interior nodes at two
+    depths; the leaf sets differ between LoDs."""
+    scene = nb.Scene(ctx, bench["v"], bench["t"])
+    scene.set_nbvh(multi_lod_nbvh(nb, bench["nodes"]))
+    scene.init_network(nb.network_config(), seed=4, table_range=1e-3)
+    cfg, tables, mlp = scene.network()
+    lo, hi = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    nodes = scene.nbvh()
+    _, cam = camera_rays(nb, orc, 96, 96)
+    rays = np.concatenate([cam, random_rays(3000, 71, inside_frac=0.3)])
+    mod = nb.NeuralBVHModule(ctx, scene)
+    got = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP32, lod=lod)
+    assert hits_equal(got, net.neural_query(nodes, rays, lod=lod, mode=orc.FP32))
+    g16 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16, lod=lod)
+    e64 = net.neural_query(nodes, rays, lod=lod, mode=orc.FP64)
+    assert (nb.hit_kind(g16) != nb.hit_kind(e64)).mean() <= 0.005
+    if lod > 0:
+        g0 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP32, lod=0)
+        assert not np.array_equal(g0["node_id"], got["node_id"])
+    scene.close()

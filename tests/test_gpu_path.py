@@ -138,3 +138,41 @@ def test_cpp_driver_path_mode_matches_python(nb, setup, tmp_path):
     exp = np.floor(v * 255 + 0.5).astype(np.int32)
     assert np.abs(got - exp).max() <= 1
     assert got.mean() > 20
+
+
+def test_path_trace_lod_switch_on_bounces(nb, orc, ctx):
+    """path_lod_switch: rays at depth > 1 traverse the N-BVH at LoD lod_switch. An N-BVH of the
+    benchmark mesh cut at 2048 leaves whose interior nodes at depth 5 are also marked LEAF_LOD(6)
+    (children kept, as the reference's LoD assignment leaves them): primary rays at LoD 0, bounces
+    at LoD 6, FP32, against the oracle's path trace with the same per-depth LoD; the frame differs
+    from bounces at LoD 0."""
+    v, t, nodes, prims, _ = bench_scene(nb)
+    nbvh = nb.nbvh_cut(nodes, 2048)
+    depth = np.full(len(nbvh), -1)
+    depth[0] = 0
+    marked = 0
+    for i in range(len(nbvh)):
+        if nbvh["prim_count"][i] == 0 and nbvh["left_or_prim_offset"][i] != nb.NO_NEURAL_CHILD:
+            c = int(nbvh["left_or_prim_offset"][i])
+            depth[c] = depth[c + 1] = depth[i] + 1
+            if depth[i] == 5:
+                nbvh["prim_count"][i] = 8  # LEAF_LOD(6), children kept
+                marked += 1
+    assert marked > 8 and np.all(depth >= 0)
+    scene = nb.Scene(ctx, v, t)
+    scene.set_nbvh(nbvh)
+    scene.init_network(nb.network_config(), seed=4, table_range=1e-3)
+    cfg, tables, mlp = scene.network()
+    lo, hi = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    nodes_d = scene.nbvh()
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    img, counts = mod.path_trace(cam, W, H, precision=nb.PREC_FP32, max_depth=4, bounce_lod=6)
+    ref, rc = orc.path_trace(lambda r, d: net.neural_query(nodes_d, r, lod=6 if d > 1 else 0, mode=orc.FP32),
+                             nb.camera_array(cam), W, H, max_depth=4, query_takes_depth=True)
+    compare(img, ref, counts[0], rc)
+    assert counts[0][1] > 100
+    img0, _ = mod.path_trace(cam, W, H, precision=nb.PREC_FP32, max_depth=4)
+    assert not np.array_equal(img0, img)
+    scene.close()
