@@ -1,0 +1,106 @@
+"""Pins of the training restatement (oracle/train.py): gradients against central finite
+differences of the loss, the composite loss's special cases, and the Adam step's closed form."""
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def small(nb, orc):
+    from oracle import train
+    cfg = nb.network_config(n_levels=3, n_features_per_level=2, log2_hashmap_size=7, base_resolution=3,
+                            per_level_scale=2.0)
+    tables, mlp = nb.network_init_params(cfg, seed=5, table_range=0.5)
+    net = orc.Net(cfg, tables, mlp, (-1, -1, -1), (1, 1, 1))
+    lv = net.levels()
+    assert lv["dense"][0] == 1 and lv["dense"][2] == 0  # one dense and one hashed level at least
+    rng = np.random.default_rng(3)
+    n = 40
+    x = rng.uniform(0.02, 0.98, (n, 9))
+    t = rng.uniform(0, 1, (n, 8))
+    t[:, 0] = rng.integers(0, 2, n)
+    return train, cfg, lv, tables.astype(np.float64), mlp.astype(np.float64), x, t
+
+
+def test_forward_matches_the_query_oracle(nb, orc, small):
+    train, cfg, lv, tables, mlp, x, t = small
+    net = orc.Net(cfg, tables.astype(np.float32), mlp.astype(np.float32), (-1, -1, -1), (1, 1, 1))
+    out, _ = net.network(x.astype(np.float32), mode=orc.FP64)
+    _, _, _, y = train.train_gradients(lv, cfg, tables, mlp, x.astype(np.float32).astype(np.float64), t)
+    assert np.allclose(y, out, rtol=0, atol=1e-9)
+
+
+def test_gradients_match_finite_differences(small):
+    # The relative-L2 (albedo) gradient holds its denominator constant, so the finite differences
+    # are of the loss with the denominators frozen at the base point.
+    train, cfg, lv, tables, mlp, x, t = small
+    loss, dtab, dmlp, y = train.train_gradients(lv, cfg, tables, mlp, x, t)
+    den = y[:, 5:8] ** 2 + 1e-2
+
+    def L(tb, ml):
+        return train.train_loss(lv, cfg, tb, ml, x, t, albedo_den=den)
+    rng = np.random.default_rng(8)
+    touched = np.nonzero(dtab)[0]
+    assert len(touched) > 50
+    h = 1e-6
+    for i in rng.choice(touched, 40, replace=False):
+        tp, tm = tables.copy(), tables.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (L(tp, mlp) - L(tm, mlp)) / (2 * h)
+        assert abs(fd - dtab[i]) <= 1e-6 * max(1.0, abs(dtab[i])) + 1e-9, (i, fd, dtab[i])
+    for i in rng.choice(len(mlp), 60, replace=False):
+        mp, mm = mlp.copy(), mlp.copy()
+        mp[i] += h
+        mm[i] -= h
+        fd = (L(tables, mp) - L(tables, mm)) / (2 * h)
+        assert abs(fd - dmlp[i]) <= 1e-6 * max(1.0, abs(dmlp[i])) + 1e-9, (i, fd, dmlp[i])
+
+
+def test_relative_l2_gradient_holds_the_denominator(small):
+    train = small[0]
+    y = np.full((1, 8), 0.5)
+    y[0, 5] = 0.3
+    t = np.ones((1, 8))
+    _, g = train.composite_loss(y, t)
+    h = 1e-7
+    den = y[:, 5:8] ** 2 + 1e-2
+    yp, ym = y.copy(), y.copy()
+    yp[0, 5] += h
+    ym[0, 5] -= h
+    fd_frozen = (train.composite_loss(yp, t, den)[0][0, 5] - train.composite_loss(ym, t, den)[0][0, 5]) / (2 * h)
+    fd_full = (train.composite_loss(yp, t)[0][0, 5] - train.composite_loss(ym, t)[0][0, 5]) / (2 * h)
+    assert np.isclose(g[0, 5], fd_frozen, rtol=1e-6) and not np.isclose(g[0, 5], fd_full, rtol=1e-2)
+
+
+def test_composite_loss_cases(small):
+    train = small[0]
+    y = np.array([[0.7, 0.4, 0.55, 0.5, 0.45, 0.2, 0.3, 0.9],
+                  [0.2, 0.4, 0.55, 0.5, 0.45, 0.2, 0.3, 0.9]])
+    t = np.array([[1.0, 0.5, 0.5, 0.5, 0.5, 0.1, 0.3, 0.5],
+                  [0.0, 0.5, 0.5, 0.5, 0.5, 0.1, 0.3, 0.5]])
+    loss, g = train.composite_loss(y, t)
+    n8 = 16
+    assert np.isclose(loss[0, 0] * n8, -np.log(0.7)) and np.isclose(g[0, 0] * n8, -(1 - 0.7) / (0.7 - 0.49))
+    assert np.isclose(loss[0, 1] * n8, 0.1) and np.isclose(g[0, 1] * n8, -1.0)
+    # normal: linearised ((y*2-1)/0.1)*0.5+0.5 -> 1.0 for y = 0.55, L1 against 0.5, gradient x 10
+    assert np.isclose(loss[0, 2] * n8, 0.5) and np.isclose(g[0, 2] * n8, 10.0)
+    assert np.isclose(loss[0, 3] * n8, 0.0)
+    assert np.isclose(loss[0, 5] * n8, 0.01 / (0.04 + 1e-2)) and np.isclose(g[0, 5] * n8, 0.2 / 0.05)
+    # target visibility < 0.5 masks every other output
+    assert np.all(loss[1, 1:] == 0) and np.all(g[1, 1:] == 0) and np.isclose(loss[1, 0] * n8, -np.log(0.8))
+    # BCE clamps the prediction to [1e-5, 1 - 1e-5]
+    l2, g2 = train.composite_loss(np.array([[1.0] + [0.5] * 7]), np.array([[0.0] + [0.5] * 7]))
+    assert np.isclose(l2[0, 0] * 8, -np.log(1e-5)) and np.isfinite(g2[0, 0])
+
+
+def test_adam_closed_form(small):
+    train = small[0]
+    rng = np.random.default_rng(1)
+    w, g = rng.normal(size=100), rng.normal(size=100)
+    w1, m1, v1 = train.adam_step(w, g, np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=0.0, eps=0.0)
+    assert np.allclose(w1, w - 0.01 * np.sign(g))  # first step: m^ = g, v^ = g^2
+    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
+    w2, _, _ = train.adam_step(w1, g, m1, v1, 2, lr=0.01, l2_reg=0.0, eps=0.0)
+    assert np.allclose(w2, w1 - 0.01 * np.sign(g))  # a constant gradient keeps m^/sqrt(v^) = sign
+    w3, _, _ = train.adam_step(w, np.zeros(100), np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=1.0, eps=0.0)
+    assert np.allclose(w3, w - 0.01 * np.sign(w))  # L2 regularisation enters as g + l2 * w
