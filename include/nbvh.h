@@ -430,6 +430,59 @@ int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scen
                         const nbvh_path_params* params, const nbvh_camera* camera, uint32_t width, uint32_t height,
                         float* d_accum, nbvh_path_stats* stats, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Training: the replacement of the reference's trainer->training_step(stream, inputs[9 x n],
+ * targets[8 x n]) (src/cuda/neural/neural_bvh_renderer.cu:170-173) for its default
+ * configuration, in FP32:
+ *   loss   the composite loss of include/cuda/neural/utils/pt_composite_loss.cuh:19-123 with the
+ *          defaults of include/cuda/neural/neural_bvh_module.h:250-266: visibility binary cross
+ *          entropy; depth L1, normal L1 on the linearised output, albedo relative L2 (denominator
+ *          held constant, include/cuda/neural/utils/loss_gradient.cuh), these three masked where
+ *          the target visibility is < 0.5; each element divided by n * 8;
+ *   update Adam with L2 regularisation added to the gradient (defaults of
+ *          include/cuda/neural/utils/tcnn_config.h:153-166; the learning-rate decay schedule is
+ * the caller.s: change learning_rate between steps with a new trainer config). Parameters
+ *          whose gradient is exactly zero (table entries no sample touched) are left unchanged.
+ * The segment sampling and tree-cut learning of the reference's training loop are not part of
+ * this: inputs are encoded segments (as for nbvh_network_inference) and targets are the 8 outputs
+ * (nbvh_segment_labels gives them from the scene's triangles). The trainer holds its own FP32 copy
+ * of the scene's network; nbvh_trainer_commit writes it back into the scene (and re-derives the
+ * FP16/BF16/FP8 forms). Shapes: the MFMA query's, 96 inputs, 4 x 64 hidden, 8 outputs.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct nbvh_trainer nbvh_trainer;
+
+typedef struct nbvh_train_config {
+    float    learning_rate; /* 1e-2 */
+    float    beta1;         /* 0.9 */
+    float    beta2;         /* 0.999 */
+    float    epsilon;       /* 1e-8 */
+    float    l2_reg;        /* 1e-8 */
+    uint32_t max_batch;     /* samples per step, at most 2^24 (624 floats of device scratch each) */
+    uint32_t reserved[4];
+} nbvh_train_config;
+
+enum nbvh_train_flags {
+    NBVH_TRAIN_GRADIENTS_ONLY = 1u /* compute loss and gradients, leave the parameters */
+};
+
+enum nbvh_train_array {
+    NBVH_TRAIN_TABLES      = 0, /* the trained tables (tcnn order) */
+    NBVH_TRAIN_MLP         = 1, /* the trained matrices (row-major, nbvh_network_init_params order) */
+    NBVH_TRAIN_GRAD_TABLES = 2, /* the last step's gradients */
+    NBVH_TRAIN_GRAD_MLP    = 3
+};
+
+int32_t nbvh_trainer_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_train_config* cfg, nbvh_trainer** out);
+int32_t nbvh_trainer_destroy(nbvh_trainer* trainer);
+/* One step on device buffers: d_inputs n x 9 (encoded segments), d_targets n x 8. Stream-ordered;
+ * with loss != NULL the call waits for the step and returns its loss (sum of the element values). */
+int32_t nbvh_trainer_step(nbvh_trainer* trainer, const float* d_inputs, const float* d_targets, uint32_t n,
+                          uint32_t flags, float* loss, void* stream);
+/* Host copy of one of the trainer's arrays (nbvh_train_array); out NULL: only *count. Synchronous. */
+int32_t nbvh_trainer_get(nbvh_trainer* trainer, uint32_t what, float* out, uint64_t capacity, uint64_t* count);
+/* Write the trained network into the scene. Synchronous; no query may run on the scene meanwhile. */
+int32_t nbvh_trainer_commit(nbvh_trainer* trainer);
+
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                         const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);

@@ -14,10 +14,11 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8, QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY,
                    QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY, HIT_NONE, HIT_NEURAL, HIT_TRIANGLE,
                    FLAG_STACK_OVERFLOW, FLAG_STEP_CAP, NO_NEURAL_CHILD, SEGMENT_EQUIDISTANT_EXTERIOR,
-                   SEGMENT_EQUIDISTANT_INTERIOR, BACKGROUND_CONSTANT, BACKGROUND_SKY)
+                   SEGMENT_EQUIDISTANT_INTERIOR, BACKGROUND_CONSTANT, BACKGROUND_SKY, TRAIN_TABLES, TRAIN_MLP,
+                   TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
-__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "NbvhError", "network_config", "icosphere", "bvh_build",
+__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "NbvhError", "network_config", "icosphere", "bvh_build",
            "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
            "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
@@ -478,6 +479,55 @@ class NeuralBVHModule:
         self.ctx.check(_lib().nbvh_network_inference(self.ctx.handle, self.scene.handle, precision,
                                                      ctypes.c_void_p(d_in_ptr), n, ctypes.c_void_p(d_out_ptr),
                                                      ctypes.c_void_p(d_logits_ptr or 0), ctypes.c_void_p(stream or 0)))
+
+
+class Trainer:
+    """Training of a scene's network (nbvh_trainer_*): the replacement of the reference's
+    trainer->training_step on encoded segments and their 8 targets, composite loss and Adam with
+    the reference's defaults. commit() writes the trained network into the scene."""
+
+    def __init__(self, ctx, scene, learning_rate=1e-2, beta1=0.9, beta2=0.999, epsilon=1e-8, l2_reg=1e-8,
+                 max_batch=1 << 18):
+        self.ctx, self.scene = ctx, scene
+        cfg = _ffi.TrainConfig(learning_rate, beta1, beta2, epsilon, l2_reg, max_batch)
+        self.handle = ctypes.c_void_p()
+        self.ctx.check(_lib().nbvh_trainer_create(ctx.handle, scene.handle, ctypes.byref(cfg),
+                                                  ctypes.byref(self.handle)))
+        self.max_batch = max_batch
+
+    def step_device(self, d_inputs_ptr, d_targets_ptr, n, gradients_only=False, stream=None):
+        """One step on device buffers (n x 9 inputs, n x 8 targets); returns the loss."""
+        loss = ctypes.c_float()
+        self.ctx.check(_lib().nbvh_trainer_step(self.handle, ctypes.c_void_p(d_inputs_ptr),
+                                                ctypes.c_void_p(d_targets_ptr), n,
+                                                _ffi.TRAIN_GRADIENTS_ONLY if gradients_only else 0,
+                                                ctypes.byref(loss), ctypes.c_void_p(stream or 0)))
+        return loss.value
+
+    def step(self, inputs, targets, gradients_only=False):
+        from . import hip
+        x = np.ascontiguousarray(inputs, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        if len(x) == 0:
+            return self.step_device(0, 0, 0, gradients_only)
+        dx, dt = hip.Buffer.from_array(x), hip.Buffer.from_array(t)
+        return self.step_device(dx.ptr, dt.ptr, len(x), gradients_only)
+
+    def get(self, what):
+        """One of the trainer's arrays: _ffi.TRAIN_TABLES / TRAIN_MLP / TRAIN_GRAD_TABLES / TRAIN_GRAD_MLP."""
+        n = ctypes.c_uint64()
+        self.ctx.check(_lib().nbvh_trainer_get(self.handle, what, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        self.ctx.check(_lib().nbvh_trainer_get(self.handle, what, _ptr(out), n.value, ctypes.byref(n)))
+        return out
+
+    def commit(self):
+        self.ctx.check(_lib().nbvh_trainer_commit(self.handle))
+
+    def close(self):
+        if self.handle:
+            _lib().nbvh_trainer_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
 
 
 class Server:

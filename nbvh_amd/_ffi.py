@@ -28,6 +28,8 @@ PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
 QUERY_SORT_RAYS = 1
 QUERY_WAVEFRONT = 2
 BACKGROUND_CONSTANT, BACKGROUND_SKY = 0, 1
+TRAIN_GRADIENTS_ONLY = 1
+TRAIN_TABLES, TRAIN_MLP, TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP = 0, 1, 2, 3
 QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
@@ -85,6 +87,12 @@ class PathParams(ctypes.Structure):
 
 class PathStats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint32 * 16), ("reserved", ctypes.c_uint32 * 16)]
+
+
+class TrainConfig(ctypes.Structure):
+    _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("epsilon", ctypes.c_float), ("l2_reg", ctypes.c_float), ("max_batch", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class LaunchShape(ctypes.Structure):
@@ -167,6 +175,11 @@ SIGNATURES = {
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],
     "nbvh_query_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P],
     "nbvh_query_hybrid": [P, P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
+    "nbvh_trainer_create": [P, P, ctypes.POINTER(TrainConfig), ctypes.POINTER(P)],
+    "nbvh_trainer_destroy": [P],
+    "nbvh_trainer_step": [P, P, P, U32, U32, ctypes.POINTER(F32), P],
+    "nbvh_trainer_get": [P, U32, P, U64, PU64],
+    "nbvh_trainer_commit": [P],
     "nbvh_path_trace": [P, P, P, ctypes.POINTER(PathParams), ctypes.POINTER(Camera), U32, U32, P,
                         ctypes.POINTER(PathStats), P],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],

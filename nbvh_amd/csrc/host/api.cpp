@@ -100,6 +100,8 @@ std::vector<float> fp32_kernel_layout(const nbvh_network_config& cfg, const std:
     return out;
 }
 
+}  // namespace
+
 void refresh_args(nbvh_scene* s)
 {
     NetArgs& a = s->args;
@@ -173,6 +175,8 @@ void upload_network(nbvh_scene* s)
     s->d_frag8.upload(ctx, mw.frag8.data(), mw.frag8.size());
     for (int i = 0; i < 8; i++) s->fp8_wscale[i] = mw.fp8_wscale[i];
 }
+
+namespace {
 
 // Queue counters of one query; a ring of 2048 slots, so at most 2048 queries of one context may
 // be in flight at once.

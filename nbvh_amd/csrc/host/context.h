@@ -106,6 +106,10 @@ struct nbvh_scene {
 
 namespace nbvh {
 
+// Kernel arguments of a scene from its host state; device copies of its network (all forms).
+void refresh_args(nbvh_scene* s);
+void upload_network(nbvh_scene* s);
+
 // Queue one query launch (counter reset + kernel) over the rays of `batch` on `stream`.
 // Validates the parameters against the scene; throws Error on failure.
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,

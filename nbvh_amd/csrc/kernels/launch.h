@@ -48,6 +48,14 @@ hipError_t wf_consume(const NetArgs& a, const RayBatch& b, const WfSeg* segs, co
                       const float* logits, uint32_t m, WfRayState* st, uint32_t* next, uint32_t* next_count,
                       hipStream_t stream);
 
+// Training (train.hip): per-sample activation bytes, zero fill, loss + gradients of one batch of
+// encoded inputs (9 floats) and targets (8 floats) for the supported shapes (*supported), Adam.
+size_t train_act_bytes(uint32_t n);
+hipError_t train_zero(float* p, uint64_t n, hipStream_t stream);
+hipError_t train_gradients(const TrainArgs& t, const float* in, const float* targets, uint32_t n, hipStream_t stream,
+                           bool* supported);
+hipError_t train_adam(float* w, const float* g, float* m, float* v, uint64_t n, const AdamArgs& h, hipStream_t stream);
+
 // Path-trace driver: per-path state init (pixel = index, throughput 1) and one depth's shading
 // and compaction (live paths appended at *counter).
 hipError_t path_init(PathState* st, uint32_t n, hipStream_t stream);

@@ -99,6 +99,23 @@ struct WfRayState {
     uint32_t words[16];
 };
 
+// One training step's buffers (kernels/train.hip): the network being trained (FP32 tables in
+// tcnn order, row-major matrices), per-sample activation records, gradients, loss accumulator.
+struct TrainArgs {
+    NetArgs      net;          // shape and grid levels
+    const float* tables;
+    const float* mlp;
+    float*       act;          // n x 624 floats
+    float*       grad_tables;
+    float*       grad_mlp;
+    float*       loss;
+};
+
+// Adam step constants; inv_bias1/2 = 1 / (1 - beta^t).
+struct AdamArgs {
+    float lr, beta1, beta2, epsilon, l2_reg, inv_bias1, inv_bias2;
+};
+
 // One live path of the path-trace driver (32 B): its pixel, throughput and radiance so far.
 struct PathState {
     uint32_t pixel;

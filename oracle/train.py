@@ -3,7 +3,8 @@
 Only tests/ use this module, as the checker of the GPU trainer. It restates, in numpy and double
 precision:
   - the hash-grid encoding with its corner indices and trilinear weights (the oracle's grid
-    convention, oracle/nbvh_oracle.cpp grid_point);
+    convention, oracle/nbvh_oracle.cpp grid_point), grid positions rounded to FP32 as the FP32
+    kernels form them, everything after them in FP64;
   - the MLP forward pass (no biases, ReLU hidden layers, Sigmoid on all outputs);
   - the reference's composite loss with its default configuration
     (include/cuda/neural/utils/pt_composite_loss.cuh:19-123, defaults
@@ -13,7 +14,8 @@ precision:
     visibility is < 0.5, each element's loss and gradient divided by n * 8;
   - backpropagation to the MLP matrices and the grid tables;
   - one Adam step (Kingma and Ba 2015, Algorithm 1) with L2 regularisation added to the gradient,
-    the reference's optimiser defaults (include/cuda/neural/utils/tcnn_config.h:153-166).
+    the reference's optimiser defaults (include/cuda/neural/utils/tcnn_config.h:153-166);
+    parameters with an exactly zero gradient are left alone (sparse hash-table updates).
 The gradients are pinned by central finite differences of the loss (tests/test_oracle_train.py).
 """
 import numpy as np
@@ -46,9 +48,14 @@ def encode(lv, x):
     for p in range(P):
         pts = x[:, 3 * p:3 * p + 3].astype(np.float64)
         for l in range(L):
-            pos = float(lv["scale"][l]) * pts + 0.5
-            g = np.floor(pos)
-            fr = pos - g
+            # Grid position as the FP32 kernels form it, fmaf(scale, x, 0.5f) rounded once to
+            # float (the product of two floats is exact in double): at the finest levels the
+            # fraction keeps only ~11 bits, the same for every FP32 implementation, so the
+            # restatement takes the positions in FP32 and does the rest in FP64.
+            pos32 = (np.float64(np.float32(lv["scale"][l])) * pts.astype(np.float32).astype(np.float64)
+                     + 0.5).astype(np.float32)
+            g = np.floor(pos32).astype(np.float64)
+            fr = (pos32 - np.floor(pos32)).astype(np.float64)
             idx = _corner_index(lv, l, g.astype(np.int64)) + int(lv["offset"][l])
             w = np.ones((n, 8))
             for c in range(8):
@@ -145,10 +152,13 @@ def train_gradients(lv, cfg, tables, mlp, x, targets):
 
 
 def adam_step(w, g, m, v, step, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8, l2_reg=1e-8):
-    """One Adam step (step counts from 1); returns new (w, m, v)."""
-    g = g + l2_reg * w
-    m = beta1 * m + (1.0 - beta1) * g
-    v = beta2 * v + (1.0 - beta2) * g * g
-    mh = m / (1.0 - beta1 ** step)
-    vh = v / (1.0 - beta2 ** step)
-    return w - lr * mh / (np.sqrt(vh) + eps), m, v
+    """One Adam step (step counts from 1); returns new (w, m, v). Parameters whose gradient is
+    exactly zero (hash-table entries no sample touched) keep their value and moments."""
+    live = g != 0
+    g2 = g + l2_reg * w
+    m2 = beta1 * m + (1.0 - beta1) * g2
+    v2 = beta2 * v + (1.0 - beta2) * g2 * g2
+    mh = m2 / (1.0 - beta1 ** step)
+    vh = v2 / (1.0 - beta2 ** step)
+    w2 = w - lr * mh / (np.sqrt(vh) + eps)
+    return np.where(live, w2, w), np.where(live, m2, m), np.where(live, v2, v)

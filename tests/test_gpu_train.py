@@ -1,0 +1,137 @@
+"""GPU trainer (nbvh_trainer_*) against the FP64 training restatement (oracle/train.py).
+
+Bars: the loss and the matrix gradients within 1e-4 (relative, as vector norms) of the FP64
+restatement and the table gradients within 1e-3 (the GPU computes grid cells in FP32, the oracle in
+FP64, so a point within an ulp of a cell boundary may scatter into the neighbouring cell); one Adam
+step equal to the restatement's step on the GPU's own gradients to 1e-6; and training on segments
+of the benchmark mesh's N-BVH leaves labelled from its triangles raises the network's held-out
+visibility accuracy well above the class prior.
+"""
+import numpy as np
+import pytest
+
+from conftest import bench_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(nb):
+    c = nb.Context()
+    yield c
+    c.close()
+
+
+def leaf_segments(nb, nbvh, n, rng):
+    """Training segments in the spirit of the reference's leaf sampling: a uniformly chosen N-BVH
+    leaf, a uniform point in its box and a uniform direction; the segment is the line's passage
+    through the box (entry and exit distances from an origin 3 units back along the line)."""
+    leaves = nbvh[nbvh["prim_count"] > 0]
+    b = leaves[rng.integers(0, len(leaves), n)]
+    blo, bhi = b["aabb_min"].astype(np.float64), b["aabb_max"].astype(np.float64)
+    p = rng.uniform(blo, bhi)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = p - 3.0 * d
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0, t1 = (blo - o) / d, (bhi - o) / d
+    mint, maxt = np.max(np.minimum(t0, t1), axis=1), np.min(np.maximum(t0, t1), axis=1)
+    rays = np.zeros(n, nb.RAY_DTYPE)
+    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
+    return rays, mint.astype(np.float32), maxt.astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def setup(nb, orc, ctx):
+    v, t, nodes, prims, _ = bench_scene(nb)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    scene.init_network(nb.network_config(), seed=1, table_range=1e-2)
+    cfg, tables, mlp = scene.network()
+    lo, hi = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    rays, mint, maxt = leaf_segments(nb, scene.nbvh(), 420000, np.random.default_rng(0))
+    x = net.encode(rays, mint, maxt, mode=orc.FP32)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    targets = mod.segment_labels(rays, mint, maxt)  # the scene's triangles label every segment
+    yield dict(scene=scene, cfg=cfg, tables=tables, mlp=mlp, net=net, x=x, t=targets, mod=mod)
+    scene.close()
+
+
+def test_gradients_match_oracle(nb, orc, ctx, setup):
+    # Targets of the L1 outputs (depth, normal) are placed 0.25 away from the prediction, so that
+    # no element sits at the kink of |d| where FP32 and FP64 could take different signs.
+    from oracle import train
+    x = setup["x"][:512]
+    lv = setup["net"].levels()
+    T64, M64 = setup["tables"].astype(np.float64), setup["mlp"].astype(np.float64)
+    y = train.forward(lv, setup["cfg"], T64, M64, x.astype(np.float64))[3]
+    t = setup["t"][:512].astype(np.float64).copy()
+    t[:, 1] = y[:, 1] + np.where(y[:, 1] > 0.5, -0.25, 0.25)
+    lin = ((y[:, 2:5] * 2.0 - 1.0) / 0.1) * 0.5 + 0.5
+    t[:, 2:5] = lin + np.where(lin > 0.5, -0.25, 0.25)
+    t = t.astype(np.float32)
+    tr = nb.Trainer(ctx, setup["scene"], max_batch=4096)
+    loss = tr.step(x, t, gradients_only=True)
+    gt, gm = tr.get(nb.TRAIN_GRAD_TABLES), tr.get(nb.TRAIN_GRAD_MLP)
+    l64, dt64, dm64, _ = train.train_gradients(lv, setup["cfg"], T64, M64, x.astype(np.float64),
+                                               t.astype(np.float64))
+    assert abs(loss - l64) <= 1e-4 * abs(l64), (loss, l64)
+    assert np.linalg.norm(gm - dm64) <= 1e-4 * np.linalg.norm(dm64)
+    assert np.linalg.norm(gt - dt64) <= 1e-3 * np.linalg.norm(dt64)
+    assert np.count_nonzero(gt) > 1000 and np.abs(dt64).max() > 0
+    # gradients only: the parameters are untouched
+    assert np.array_equal(tr.get(nb.TRAIN_MLP), setup["mlp"]) and np.array_equal(tr.get(nb.TRAIN_TABLES),
+                                                                                 setup["tables"])
+    tr.close()
+
+
+def test_adam_step_matches_oracle(nb, orc, ctx, setup):
+    from oracle import train
+    x, t = setup["x"][:2048], setup["t"][:2048]
+    tr = nb.Trainer(ctx, setup["scene"], max_batch=4096)
+    tr.step(x, t)
+    # The gradients the update used stay in the trainer: the restatement's Adam on them.
+    gt, gm = tr.get(nb.TRAIN_GRAD_TABLES), tr.get(nb.TRAIN_GRAD_MLP)
+    wt, wm = tr.get(nb.TRAIN_TABLES), tr.get(nb.TRAIN_MLP)
+    et, _, _ = train.adam_step(setup["tables"].astype(np.float64), gt.astype(np.float64), 0.0, 0.0, 1)
+    em, _, _ = train.adam_step(setup["mlp"].astype(np.float64), gm.astype(np.float64), 0.0, 0.0, 1)
+    assert np.allclose(wm, em, rtol=0, atol=1e-6)
+    assert np.allclose(wt, et, rtol=0, atol=1e-6)
+    untouched = gt == 0
+    assert untouched.sum() > 0 and np.array_equal(wt[untouched], setup["tables"][untouched])
+    assert np.abs(wm - setup["mlp"]).max() > 5e-3  # a first Adam step moves by about the learning rate
+    tr.close()
+
+
+def test_training_learns_visibility(nb, orc, ctx, setup):
+    from nbvh_amd import hip
+    x, t = setup["x"], setup["t"]
+    rng = np.random.default_rng(1)
+    held, pool = np.arange(20000), np.arange(20000, len(x))
+    assert 0.1 < t[:, 0].mean() < 0.9
+
+    def accuracy():
+        out, _ = setup["mod"].network_inference(x[held], precision=nb.PREC_FP32)
+        return float(np.mean((out[:, 0] > 0.5) == (t[held, 0] > 0.5)))
+
+    acc0 = accuracy()
+    tr = nb.Trainer(ctx, setup["scene"], max_batch=8192)
+    dx, dt = hip.Buffer.from_array(x[pool]), hip.Buffer.from_array(t[pool])
+    losses = []
+    for step in range(250):
+        off = int(rng.integers(0, len(pool) - 8192))
+        losses.append(tr.step_device(dx.ptr + off * 36, dt.ptr + off * 32, 8192))
+    tr.commit()
+    acc1 = accuracy()
+    prior = max(t[held, 0].mean(), 1 - t[held, 0].mean())
+    # Measured: held-out accuracy 0.62 (the class prior) -> 0.86 after 250 steps, loss / 10.
+    assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5]), (losses[:5], losses[-20:])
+    assert acc1 > prior + 0.15 and acc1 > acc0 + 0.15, (acc0, acc1, prior)
+    # The committed network is the trainer's, in every precision.
+    cfg, tables, mlp = setup["scene"].network()
+    assert np.array_equal(mlp, tr.get(nb.TRAIN_MLP)) and np.array_equal(tables, tr.get(nb.TRAIN_TABLES))
+    out16, _ = setup["mod"].network_inference(x[held], precision=nb.PREC_FP16)
+    assert np.mean((out16[:, 0] > 0.5) == (t[held, 0] > 0.5)) > acc1 - 0.01
+    tr.close()

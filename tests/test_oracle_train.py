@@ -26,7 +26,9 @@ def test_forward_matches_the_query_oracle(nb, orc, small):
     net = orc.Net(cfg, tables.astype(np.float32), mlp.astype(np.float32), (-1, -1, -1), (1, 1, 1))
     out, _ = net.network(x.astype(np.float32), mode=orc.FP64)
     _, _, _, y = train.train_gradients(lv, cfg, tables, mlp, x.astype(np.float32).astype(np.float64), t)
-    assert np.allclose(y, out, rtol=0, atol=1e-9)
+    # The restatement forms grid positions in FP32 (as the kernels do), the query oracle's FP64
+    # mode in FP64: the outputs agree to the FP32 resolution of the finest grid positions.
+    assert np.allclose(y, out, rtol=0, atol=1e-5)
 
 
 def test_gradients_match_finite_differences(small):
@@ -102,5 +104,11 @@ def test_adam_closed_form(small):
     assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
     w2, _, _ = train.adam_step(w1, g, m1, v1, 2, lr=0.01, l2_reg=0.0, eps=0.0)
     assert np.allclose(w2, w1 - 0.01 * np.sign(g))  # a constant gradient keeps m^/sqrt(v^) = sign
-    w3, _, _ = train.adam_step(w, np.zeros(100), np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=1.0, eps=0.0)
+    w3, _, _ = train.adam_step(w, np.full(100, 1e-30), np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=1.0,
+                               eps=0.0)
     assert np.allclose(w3, w - 0.01 * np.sign(w))  # L2 regularisation enters as g + l2 * w
+    g4 = g.copy()
+    g4[::2] = 0.0
+    w4, m4, v4 = train.adam_step(w, g4, np.zeros(100), np.zeros(100), 1, lr=0.01)
+    assert np.array_equal(w4[::2], w[::2]) and not m4[::2].any() and not v4[::2].any()  # sparse update
+    assert np.allclose(w4[1::2], w[1::2] - 0.01 * np.sign(g[1::2]), atol=1e-6)
