@@ -209,6 +209,60 @@ def chess_bench(nb, hip, ctx, oracle, W=1024, H=1024, steps=20):
     return out
 
 
+def train_bench(nb, hip, ctx, oracle, batch=1 << 16, steps=100, pool=1 << 20):
+    """Training step throughput (nbvh_trainer_step, FP32) of the headline network on the
+    benchmark mesh: segments sampled in its N-BVH leaves (nbvh_amd.leaf_segments), encoded and
+    labelled from the triangles, minibatches of `batch` per step; the held-out visibility accuracy
+    before and after."""
+    v, t = nb.icosphere(6, seed=7, amplitude=0.1)
+    sc = nb.Scene(ctx, v, t)
+    sc.build_nbvh(256)
+    sc.init_network(nb.network_config(), seed=1, table_range=1e-2)
+    cfg, tables, mlp = sc.network()
+    lo, hi = sc.bbox()
+    net = oracle.Net(cfg, tables, mlp, lo, hi)
+    rays, mint, maxt = nb.leaf_segments(sc.nbvh(), pool + 20000, seed=3)
+    x = net.encode(rays, mint, maxt, mode=oracle.FP32)
+    mod = nb.NeuralBVHModule(ctx, sc)
+    tg = mod.segment_labels(rays, mint, maxt)
+    held = slice(0, 20000)
+
+    def accuracy():
+        out, _ = mod.network_inference(x[held], precision=nb.PREC_FP32)
+        return float(np.mean((out[:, 0] > 0.5) == (tg[held, 0] > 0.5)))
+
+    acc0 = accuracy()
+    dx, dt = hip.Buffer.from_array(x[20000:]), hip.Buffer.from_array(tg[20000:])
+    tr = nb.Trainer(ctx, sc, max_batch=batch)
+    s = hip.Stream()
+    rng = np.random.default_rng(5)
+    offs = rng.integers(0, pool - batch, steps + 1)
+    first = tr.step_device(dx.ptr + int(offs[0]) * 36, dt.ptr + int(offs[0]) * 32, batch, stream=s.handle)
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(s)
+    for k in range(1, steps + 1):
+        tr.step_device(dx.ptr + int(offs[k]) * 36, dt.ptr + int(offs[k]) * 32, batch, stream=s.handle)
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    last = tr.step_device(dx.ptr, dt.ptr, batch, gradients_only=True, stream=s.handle)
+    tr.commit()
+    acc1 = accuracy()
+    tr.close()
+    s.destroy()
+    sc.close()
+    out = {"workload": f"headline network (hash grid 16x2 2^19, MLP 96-64x4-8) on the icosphere's 256 N-BVH leaves, "
+                       f"FP32, minibatches of {batch} leaf segments from a pool of {pool}",
+           "ms_per_step": round(ms, 3), "samples_per_s": round(batch / (ms * 1e-3) / 1e6, 2),
+           "unit": "M samples/s", "steps": steps, "loss_first": round(first, 5), "loss_last": round(last, 5),
+           "heldout_visibility_accuracy": {"before": round(acc0, 4), "after": round(acc1, 4),
+                                           "class_prior": round(float(max(tg[held, 0].mean(),
+                                                                              1 - tg[held, 0].mean())), 4)}}
+    log(f"training: {out['ms_per_step']} ms per step of {batch} ({out['samples_per_s']} M samples/s), loss "
+        f"{out['loss_first']} -> {out['loss_last']}, held-out visibility accuracy {acc0:.3f} -> {acc1:.3f}")
+    return out
+
+
 def gather_ceiling(sizes_mib=(2, 4, 8, 16, 32, 64)):
     """Random-gather line rate of the GPU (nbvh_amd/libnbvh_bench.so, csrc/bench/gather_ceiling.hip):
     8 waves per CU, every lane of a load instruction in its own random 128-byte line of a table of
@@ -630,6 +684,7 @@ def main():
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gather-ceiling", action="store_true", help="skip the random-gather line-rate measurement")
+    ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the FP32 plain path")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (continuous batching) run")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the 4M mixed-ray wavefront run")
@@ -750,6 +805,8 @@ def main():
             result["chess_nbvh"] = chess_bench(nb, hip, ctx, oracle)
         if not args.no_gather_ceiling:
             result["gather_ceiling"] = gather_ceiling()
+        if not args.no_train:
+            result["training"] = train_bench(nb, hip, ctx, oracle)
 
         if not args.no_path_traced and prec != nb.PREC_FP32:
             pt = path_traced_bench(nb, hip, ctx, scene, oracle)

@@ -20,7 +20,7 @@ from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
 __all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "NbvhError", "network_config", "icosphere", "bvh_build",
            "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
-           "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
+           "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "leaf_segments", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
 
 def hit_kind(hits):
@@ -479,6 +479,33 @@ class NeuralBVHModule:
         self.ctx.check(_lib().nbvh_network_inference(self.ctx.handle, self.scene.handle, precision,
                                                      ctypes.c_void_p(d_in_ptr), n, ctypes.c_void_p(d_out_ptr),
                                                      ctypes.c_void_p(d_logits_ptr or 0), ctypes.c_void_p(stream or 0)))
+
+
+def leaf_segments(nbvh, n, seed=0, lod=0):
+    """Training segments in the spirit of the reference's leaf sampling: a uniformly chosen N-BVH
+    leaf (at LoD `lod`), a uniform point in its box and a uniform direction; the segment is the
+    line's passage through the box, as distances from an origin 3 units back along the line.
+    Returns (rays, mint, maxt): encode with the network's segment encoding and label with
+    NeuralBVHModule.segment_labels. (The reference's granular, loss-driven node sampling and its
+    tree-cut learning are not restated.)"""
+    rng = np.random.default_rng(seed)
+    nbvh = np.asarray(nbvh, NODE_DTYPE)
+    pc = nbvh["prim_count"]
+    leaf = ((pc > 0) & (pc <= 2 + lod)) | (nbvh["left_or_prim_offset"] == NO_NEURAL_CHILD)
+    leaves = nbvh[leaf]
+    b = leaves[rng.integers(0, len(leaves), n)]
+    blo, bhi = b["aabb_min"].astype(np.float64), b["aabb_max"].astype(np.float64)
+    p = rng.uniform(blo, bhi)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = p - 3.0 * d
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0, t1 = (blo - o) / d, (bhi - o) / d
+    mint, maxt = np.max(np.minimum(t0, t1), axis=1), np.min(np.maximum(t0, t1), axis=1)
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
+    return rays, mint.astype(np.float32), maxt.astype(np.float32)
 
 
 class Trainer:

@@ -22,26 +22,6 @@ def ctx(nb):
     c.close()
 
 
-def leaf_segments(nb, nbvh, n, rng):
-    """Training segments in the spirit of the reference's leaf sampling: a uniformly chosen N-BVH
-    leaf, a uniform point in its box and a uniform direction; the segment is the line's passage
-    through the box (entry and exit distances from an origin 3 units back along the line)."""
-    leaves = nbvh[nbvh["prim_count"] > 0]
-    b = leaves[rng.integers(0, len(leaves), n)]
-    blo, bhi = b["aabb_min"].astype(np.float64), b["aabb_max"].astype(np.float64)
-    p = rng.uniform(blo, bhi)
-    d = rng.normal(size=(n, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    o = p - 3.0 * d
-    with np.errstate(divide="ignore", invalid="ignore"):
-        t0, t1 = (blo - o) / d, (bhi - o) / d
-    mint, maxt = np.max(np.minimum(t0, t1), axis=1), np.min(np.maximum(t0, t1), axis=1)
-    rays = np.zeros(n, nb.RAY_DTYPE)
-    rays["o"], rays["d"] = o.astype(np.float32), d.astype(np.float32)
-    rays["tmin"], rays["tmax"] = 1e-4, np.float32(3.4028234663852886e38)
-    return rays, mint.astype(np.float32), maxt.astype(np.float32)
-
-
 @pytest.fixture(scope="module")
 def setup(nb, orc, ctx):
     v, t, nodes, prims, _ = bench_scene(nb)
@@ -51,7 +31,7 @@ def setup(nb, orc, ctx):
     cfg, tables, mlp = scene.network()
     lo, hi = scene.bbox()
     net = orc.Net(cfg, tables, mlp, lo, hi)
-    rays, mint, maxt = leaf_segments(nb, scene.nbvh(), 420000, np.random.default_rng(0))
+    rays, mint, maxt = nb.leaf_segments(scene.nbvh(), 420000, seed=0)
     x = net.encode(rays, mint, maxt, mode=orc.FP32)
     mod = nb.NeuralBVHModule(ctx, scene)
     targets = mod.segment_labels(rays, mint, maxt)  # the scene's triangles label every segment
