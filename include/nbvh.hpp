@@ -246,6 +246,40 @@ private:
     nbvh_scene* scene_;
 };
 
+// Training of a scene's network (the reference's trainer->training_step): composite loss and Adam
+// with the reference's defaults; commit() writes the trained network into the scene.
+class Trainer {
+public:
+    explicit Trainer(Context& ctx, Scene& scene, uint32_t max_batch = 1u << 18, float learning_rate = 1e-2f)
+        : ctx_(ctx.get())
+    {
+        nbvh_train_config c{};
+        c.learning_rate = learning_rate;
+        c.beta1         = 0.9f;
+        c.beta2         = 0.999f;
+        c.epsilon       = 1e-8f;
+        c.l2_reg        = 1e-8f;
+        c.max_batch     = max_batch;
+        check(nbvh_trainer_create(ctx_, scene.get(), &c, &t_), ctx_);
+    }
+    ~Trainer() { nbvh_trainer_destroy(t_); }
+    Trainer(const Trainer&) = delete;
+    Trainer& operator=(const Trainer&) = delete;
+
+    // One step on device buffers (n x 9 encoded segments, n x 8 targets); returns the loss.
+    float training_step(const float* d_inputs, const float* d_targets, uint32_t n, void* stream = nullptr)
+    {
+        float loss = 0.f;
+        check(nbvh_trainer_step(t_, d_inputs, d_targets, n, 0u, &loss, stream), ctx_);
+        return loss;
+    }
+    void commit() { check(nbvh_trainer_commit(t_), ctx_); }
+
+private:
+    nbvh_ctx* ctx_ = nullptr;
+    nbvh_trainer* t_ = nullptr;
+};
+
 // Path-trace defaults of the reference's path module: max depth 5 (4 bounces), sky background.
 inline nbvh_path_params path_params(uint32_t precision = NBVH_PREC_FP16, uint32_t max_depth = 5,
                                     uint32_t rr_start_depth = 0, uint32_t sample = 0)
