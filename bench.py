@@ -251,16 +251,42 @@ def train_bench(nb, hip, ctx, oracle, batch=1 << 16, steps=100, pool=1 << 20):
     tr.close()
     s.destroy()
     sc.close()
+    # The table-gradient scatter is one FP32 atomic add per (point, level, corner, feature) of every
+    # sample; the GPU's random atomic-add rate bounds the step from below.
+    adds = batch * cfg.n_points * cfg.grid.n_levels * 8 * cfg.grid.n_features_per_level
+    ceiling = atomic_ceiling()
     out = {"workload": f"headline network (hash grid 16x2 2^19, MLP 96-64x4-8) on the icosphere's 256 N-BVH leaves, "
                        f"FP32, minibatches of {batch} leaf segments from a pool of {pool}",
            "ms_per_step": round(ms, 3), "samples_per_s": round(batch / (ms * 1e-3) / 1e6, 2),
            "unit": "M samples/s", "steps": steps, "loss_first": round(first, 5), "loss_last": round(last, 5),
            "heldout_visibility_accuracy": {"before": round(acc0, 4), "after": round(acc1, 4),
                                            "class_prior": round(float(max(tg[held, 0].mean(),
-                                                                              1 - tg[held, 0].mean())), 4)}}
+                                                                              1 - tg[held, 0].mean())), 4)},
+           "roofline": {"bound": "atomics", "table_gradient_adds_per_step": adds,
+                        "atomic_adds_per_s": round(ceiling, 1), "floor_ms": round(adds / ceiling * 1e3, 3),
+                        "frac": round(adds / ceiling * 1e3 / ms, 3)}}
     log(f"training: {out['ms_per_step']} ms per step of {batch} ({out['samples_per_s']} M samples/s), loss "
-        f"{out['loss_first']} -> {out['loss_last']}, held-out visibility accuracy {acc0:.3f} -> {acc1:.3f}")
+        f"{out['loss_first']} -> {out['loss_last']}, held-out visibility accuracy {acc0:.3f} -> {acc1:.3f}; "
+        f"{adds / 1e6:.1f} M table-gradient atomic adds at the GPU's {ceiling / 1e9:.1f} G adds/s take "
+        f"{adds / ceiling * 1e3:.2f} ms ({out['roofline']['frac']} of the step)")
     return out
+
+
+def atomic_ceiling(entries=6529022, waves_per_cu=8, per_lane=192):
+    """Rate of random FP32 atomic adds (pairs on the two floats of one entry, as the table-gradient
+    scatter issues them) into a table of the headline grid's size (nbvh_amd/libnbvh_bench.so,
+    csrc/bench/gather_ceiling.hip); the table's final sum is checked against the number of adds."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "nbvh_amd", "libnbvh_bench.so"))
+    fn = lib.nbvh_bench_atomic_adds
+    fn.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
+    fn.restype = ctypes.c_int
+    ms, rate, exact = ctypes.c_float(), ctypes.c_double(), ctypes.c_int32()
+    st = fn(entries, waves_per_cu, per_lane, ctypes.byref(ms), ctypes.byref(rate), ctypes.byref(exact))
+    if st != 0 or exact.value != 1:
+        raise RuntimeError(f"atomic-add ceiling failed (HIP error {st}, exact sum {exact.value})")
+    return rate.value
 
 
 def gather_ceiling(sizes_mib=(2, 4, 8, 16, 32, 64)):

@@ -38,6 +38,7 @@ TrainArgs train_args(nbvh_trainer* t)
     a.tables      = t->tables.as<float>();
     a.mlp         = t->mlp.as<float>();
     a.act         = t->act.as<float>();
+    a.act_stride  = train_act_stride(t->cfg.max_batch);
     a.grad_tables = t->g_tables.as<float>();
     a.grad_mlp    = t->g_mlp.as<float>();
     a.loss        = t->loss.as<float>();

@@ -48,9 +48,10 @@ hipError_t wf_consume(const NetArgs& a, const RayBatch& b, const WfSeg* segs, co
                       const float* logits, uint32_t m, WfRayState* st, uint32_t* next, uint32_t* next_count,
                       hipStream_t stream);
 
-// Training (train.hip): per-sample activation bytes, zero fill, loss + gradients of one batch of
+// Training (train.hip): the activation row stride and workspace bytes of a max batch, zero fill, loss + gradients of one batch of
 // encoded inputs (9 floats) and targets (8 floats) for the supported shapes (*supported), Adam.
-size_t train_act_bytes(uint32_t n);
+uint32_t train_act_stride(uint32_t max_batch);
+size_t   train_act_bytes(uint32_t max_batch);
 hipError_t train_zero(float* p, uint64_t n, hipStream_t stream);
 hipError_t train_gradients(const TrainArgs& t, const float* in, const float* targets, uint32_t n, hipStream_t stream,
                            bool* supported);

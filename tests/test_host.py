@@ -276,8 +276,8 @@ def test_tcnn_parameter_order_round_trip(nb, tmp_path):
 
 
 def test_gather_ceiling_helper_validates_arguments():
-    """The benchmark's line-rate helper library loads and rejects invalid shapes before any GPU
-    call (hipErrorInvalidValue = 1)."""
+    """The benchmark's line-rate and atomic-rate helpers load and reject invalid shapes before any
+    GPU call (hipErrorInvalidValue = 1)."""
     lib = ctypes.CDLL(os.path.join(ROOT, "nbvh_amd", "libnbvh_bench.so"))
     fn = lib.nbvh_bench_gather_lines
     fn.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -286,3 +286,10 @@ def test_gather_ceiling_helper_validates_arguments():
     ms, rate = ctypes.c_float(), ctypes.c_double()
     for args in ((64, 1, 8, 512), (1 << 20, 3, 8, 512), (1 << 20, 64, 8, 512), (1 << 20, 1, 8, 7), (1 << 20, 1, 6, 512)):
         assert fn(*args, ctypes.byref(ms), ctypes.byref(rate)) == 1, args
+    add = lib.nbvh_bench_atomic_adds
+    add.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
+    add.restype = ctypes.c_int
+    exact = ctypes.c_int32()
+    for args in ((0, 8, 1), ((1 << 28) + 1, 8, 1), (1024, 6, 1), (1024, 8, 0)):
+        assert add(*args, ctypes.byref(ms), ctypes.byref(rate), ctypes.byref(exact)) == 1, args
