@@ -2,7 +2,7 @@
 // replacement of the reference's trainer->training_step(stream, inputs[9 x n], targets[8 x n])
 // (src/cuda/neural/neural_bvh_renderer.cu:170-173) for its default configuration. FP32 throughout.
 //
-// Workspace: 624 activation rows of act_stride floats each (row r, sample i at r * stride + i,
+// Workspace: 720 activation rows of act_stride floats each (row r, sample i at r * stride + i,
 // so that a wave's 64 samples touch 256 consecutive bytes of a row), the five matrices transposed,
 // and per-chunk partial matrix gradients.
 //   train_transpose  W_l [out][in] -> W_l^T [in][out], the backward pass's row order;
@@ -14,8 +14,10 @@
 //   train_backward   one thread per sample: the composite loss of the reference's defaults
 //                    (pt_composite_loss.cuh:19-123, neural_bvh_module.h:250-266), its gradient
 //                    through the Sigmoid outputs and the ReLU layers (rows of W^T), stored per
-//                    layer, and the feature gradients scattered into the table gradient with
-//                    atomics;
+//                    layer, down to the feature gradients;
+//   train_scatter    the table gradient, one workgroup per 128 KB slice of a level: the slice is
+//                    summed in LDS from every (sample, point) whose corners fall in it and
+//                    stored whole, so no global atomics are needed;
 //   train_wgrad      the matrix gradients dW[o][i] = sum over samples of delta[o] * act[i]: each
 //                    workgroup reduces one chunk of samples for one matrix through LDS tiles of 32
 //                    samples and writes its partial; train_wgrad_reduce adds the partials in
@@ -24,11 +26,12 @@
 //                    gradient, on the tables and the matrices; parameters whose gradient is
 //                    exactly zero (table entries no sample touched) keep their value and moments.
 // Gradients are sums over the batch of the per-element gradients divided by n * 8, as the
-// reference's loss kernel writes them; the table gradients and the loss are summed with atomics,
-// so their summation order is not fixed.
+// reference's loss kernel writes them; the table gradients (LDS atomics) and the loss are summed
+// in an order that is not fixed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "../device/common.h"
 #include "launch.h"
@@ -37,9 +40,12 @@ namespace nbvh {
 namespace dev {
 
 constexpr int kK0 = 96, kH = 64, kO = 8, kLayers = 5;
-// Activation rows: a0 (features) | h1..h4 | y | d1..d4 (deltas of h1..h4) | d5 (delta of y).
-constexpr int kA0 = 0, kHid = kA0 + kK0, kY = kHid + 4 * kH, kD = kY + kO, kD5 = kD + 4 * kH;
-constexpr int kRows   = kD5 + kO;                           // 624 rows
+// Activation rows: a0 (features) | h1..h4 | y | d1..d4 (deltas of h1..h4) | d5 (delta of y) |
+// dx (feature gradients).
+constexpr int kA0 = 0, kHid = kA0 + kK0, kY = kHid + 4 * kH, kD = kY + kO, kD5 = kD + 4 * kH, kDX = kD5 + kO;
+constexpr int kRows   = kDX + kK0;                          // 720 rows
+constexpr int kSliceFloats = 32768;                         // table-gradient slice in LDS (128 KB)
+constexpr uint32_t kScatterThreads = 1024;                  // threads sharing one slice
 constexpr int kMlp    = kH * kK0 + 3 * kH * kH + kO * kH;  // 18,944 matrix entries
 constexpr int kChunks = 256;                                // most matrix-gradient partials
 
@@ -151,8 +157,7 @@ template <int P, int L, int F>
 __global__ __launch_bounds__(64) void train_backward_kernel(TrainArgs t, const float* __restrict__ wt,
                                                             const float* __restrict__ in,
                                                             const float* __restrict__ targets,
-                                                            float* __restrict__ act, float* __restrict__ gtab,
-                                                            uint32_t n)
+                                                            float* __restrict__ act, uint32_t n)
 {
     __shared__ float col[kH * 64];
     const uint32_t lane = threadIdx.x, i = blockIdx.x * 64 + lane;
@@ -204,39 +209,102 @@ __global__ __launch_bounds__(64) void train_backward_kernel(TrainArgs t, const f
     }
 #pragma unroll
     for (int k = 0; k < kH; k++) d[k] = col[k * 64 + lane];  // delta of h1
-    // Feature gradients (rows of W0^T) and their scatter into the tables' gradient.
-    float pts[3 * P];
+    // Feature gradients (rows of W0^T); train_scatter adds them into the tables' gradient.
+#pragma unroll 4
+    for (int k = 0; k < kK0; k++) {
+        const float* T = wt + k * kH;
+        float s = 0.f;
 #pragma unroll
-    for (int c = 0; c < 3 * P; c++) pts[c] = in[(size_t)i * (3 * P) + c];
-    const bool exact = !in_unit_cube<3 * P>(pts);
+        for (int o = 0; o < kH; o++) s = fmaf(T[o], d[o], s);
+        row(act, S, kDX + k, i) = s;
+    }
+}
+
+// Work items of the table gradient. Slice j of level l holds the level's entries
+// [j * E, min((j + 1) * E, size)), E = kSliceFloats / F; each slice is split into parts[l] parts
+// of the batch's (sample, point) pairs; level l's items are [first[l], first[l + 1]), item
+// first[l] + j * parts[l] + k being part k of slice j.
+struct SliceMap {
+    uint32_t first[kMaxLevels + 1];
+    uint32_t parts[kMaxLevels];
+};
+
+// One workgroup per item: the slice's gradient accumulates in LDS while the workgroup visits its
+// part of the batch's (sample, point) pairs, recomputes the cell at the slice's level and adds
+// w_c * dx[f] for each corner c whose entry lies in the slice (LDS atomics). A slice of a single
+// part is then stored whole, zeros included; the parts of a split slice add their non-zero
+// entries to the (zeroed) gradient with global atomics. Every entry of every level belongs to
+// exactly one slice, so the table gradient is written completely. The cost is a scan of the
+// batch's points per item instead of one global atomic add per (point, level, corner, feature);
+// slices of the coarse dense levels, which receive many corners, are split so that no workgroup
+// adds many more corners than the others.
+template <int P, int L, int F>
+__global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArgs t, SliceMap map, const float* __restrict__ in,
+                                                            const float* __restrict__ act, float* __restrict__ gtab,
+                                                            uint32_t n)
+{
+    constexpr uint32_t E = kSliceFloats / F;
+    __shared__ float acc[kSliceFloats];
+    int l = 0;
+    while (l + 1 < L && blockIdx.x >= map.first[l + 1]) l++;
+    const GridLevel& lv = t.net.level[l];
+    const uint32_t parts = map.parts[l], item = blockIdx.x - map.first[l];
+    const uint32_t slice = item / parts, part = item - slice * parts;
+    const uint32_t e0 = slice * E;
+    const uint32_t len = lv.size - e0 < E ? lv.size - e0 : E;
+    for (uint32_t j = threadIdx.x; j < len * F; j += kScatterThreads) acc[j] = 0.f;
+    __syncthreads();
+    const size_t S = t.act_stride;
+    const uint32_t total = n * (uint32_t)P;
+    const uint32_t j0 = (uint32_t)((uint64_t)total * part / parts), j1 = (uint32_t)((uint64_t)total * (part + 1) / parts);
+    // Two (sample, point) pairs per thread and iteration, their point loads issued together.
+    constexpr uint32_t B = kScatterThreads;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 2 * B) {
+        float pts[2][3];
 #pragma unroll
-    for (int p = 0; p < P; p++) {
-        for (int l = 0; l < L; l++) {
-            float dx[F];
+        for (int h = 0; h < 2; h++) {
+            const uint32_t jj = j + h * B < j1 ? j + h * B : j;
+            const uint32_t i = jj / P, p = jj - i * P;
 #pragma unroll
-            for (int f = 0; f < F; f++) {
-                const float* T = wt + (p * L * F + l * F + f) * kH;
-                float s = 0.f;
+            for (int c = 0; c < 3; c++) pts[h][c] = in[(size_t)i * (3 * P) + 3 * p + c];
+        }
 #pragma unroll
-                for (int o = 0; o < kH; o++) s = fmaf(T[o], d[o], s);
-                dx[f] = s;
-            }
-            const GridLevel& lv = t.net.level[l];
+        for (int h = 0; h < 2; h++) {
+            const uint32_t jj = j + h * B;
+            if (jj >= j1) break;
+            const uint32_t i = jj / P, p = jj - i * P;
             uint32_t g[3], idx[8];
             float fr[3];
-            grid_cell(lv, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], g, fr);
-            if (exact)
-                grid_corners<true>(lv, g, idx);
-            else
+            grid_cell(lv, pts[h][0], pts[h][1], pts[h][2], g, fr);
+            if (in_unit_cube<3>(pts[h]))
                 grid_corners<false>(lv, g, idx);
-            float* gt = gtab + (size_t)lv.offset * F;
+            else
+                grid_corners<true>(lv, g, idx);
+            bool any = false;
+#pragma unroll
+            for (int c = 0; c < 8; c++) any = any || (idx[c] - e0 < len);
+            if (!any) continue;
+            float dx[F];
+#pragma unroll
+            for (int f = 0; f < F; f++) dx[f] = act[(size_t)(kDX + (p * L + l) * F + f) * S + i];
 #pragma unroll
             for (uint32_t c = 0; c < 8; c++) {
-                const float w = corner_weight(c, fr);
+                const uint32_t r = idx[c] - e0;
+                if (r < len) {
+                    const float w = corner_weight(c, fr);
 #pragma unroll
-                for (int f = 0; f < F; f++) atomicAdd(gt + (size_t)idx[c] * F + f, w * dx[f]);
+                    for (int f = 0; f < F; f++) atomicAdd(&acc[r * F + f], w * dx[f]);
+                }
             }
         }
+    }
+    __syncthreads();
+    float* out = gtab + ((size_t)lv.offset + e0) * F;
+    if (parts == 1) {
+        for (uint32_t j = threadIdx.x; j < len * F; j += B) out[j] = acc[j];
+    } else {
+        for (uint32_t j = threadIdx.x; j < len * F; j += B)
+            if (acc[j] != 0.f) atomicAdd(out + j, acc[j]);
     }
 }
 
@@ -365,6 +433,17 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
     float* const   wt  = act + (size_t)dev::kRows * S;
     float* const   part = wt + dev::kMlp;
     const dim3 grid((n + 63) / 64), block(64), grid_mlp((dev::kMlp + 255) / 256);
+    if (a.n_levels > (uint32_t)kMaxLevels || a.n_features == 0) return hipErrorInvalidValue;
+    // Parts per slice: about 32K corners per workgroup if the batch's corners spread evenly over
+    // the level (a skewed batch only makes some workgroups slower).
+    dev::SliceMap map{};
+    for (uint32_t l = 0; l < a.n_levels; l++) {
+        const uint32_t E = dev::kSliceFloats / a.n_features, size = a.level[l].size;
+        const uint32_t slices = (size + E - 1) / E;
+        const double corners = (double)n * a.n_points * 8.0 * std::min(E, size) / size;
+        map.parts[l]     = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(corners / 32768.0)));
+        map.first[l + 1] = map.first[l] + slices * map.parts[l];
+    }
     bool shape = false;
 #define X(P_, L_, F_)                                                                                                \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                                                \
@@ -372,7 +451,10 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
         hipLaunchKernelGGL(dev::train_transpose_kernel, grid_mlp, dim3(256), 0, stream, t.mlp, wt);                  \
         hipLaunchKernelGGL((dev::train_forward_kernel<P_, L_, F_>), grid, block, 0, stream, t, t.mlp, in, act, n);    \
         hipLaunchKernelGGL((dev::train_backward_kernel<P_, L_, F_>), grid, block, 0, stream, t, (const float*)wt, in, \
-                           targets, act, t.grad_tables, n);                                                          \
+                           targets, act, n);                                                                         \
+        hipLaunchKernelGGL((dev::train_scatter_kernel<P_, L_, F_>), dim3(map.first[L_]), dim3(dev::kScatterThreads), 0, \
+                           stream, t, map,                                                                           \
+                           in, (const float*)act, t.grad_tables, n);                                                 \
     }
     X(3, 16, 2) X(3, 8, 4)
 #undef X
