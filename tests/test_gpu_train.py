@@ -39,32 +39,68 @@ def setup(nb, orc, ctx):
     scene.close()
 
 
-def test_gradients_match_oracle(nb, orc, ctx, setup):
-    # Targets of the L1 outputs (depth, normal) are placed 0.25 away from the prediction, so that
-    # no element sits at the kink of |d| where FP32 and FP64 could take different signs.
+def _check_gradients(nb, ctx, scene, cfg, tables, mlp, lv, x, t):
+    """One gradients-only step against the FP64 restatement. Targets of the L1 outputs (depth,
+    normal) are placed 0.25 away from the prediction, so that no element sits at the kink of |d|
+    where FP32 and FP64 could take different signs."""
     from oracle import train
-    x = setup["x"][:512]
-    lv = setup["net"].levels()
-    T64, M64 = setup["tables"].astype(np.float64), setup["mlp"].astype(np.float64)
-    y = train.forward(lv, setup["cfg"], T64, M64, x.astype(np.float64))[3]
-    t = setup["t"][:512].astype(np.float64).copy()
+    T64, M64 = tables.astype(np.float64), mlp.astype(np.float64)
+    y = train.forward(lv, cfg, T64, M64, x.astype(np.float64))[3]
+    t = t.astype(np.float64).copy()
     t[:, 1] = y[:, 1] + np.where(y[:, 1] > 0.5, -0.25, 0.25)
     lin = ((y[:, 2:5] * 2.0 - 1.0) / 0.1) * 0.5 + 0.5
     t[:, 2:5] = lin + np.where(lin > 0.5, -0.25, 0.25)
     t = t.astype(np.float32)
-    tr = nb.Trainer(ctx, setup["scene"], max_batch=4096)
+    tr = nb.Trainer(ctx, scene, max_batch=4096)
     loss = tr.step(x, t, gradients_only=True)
     gt, gm = tr.get(nb.TRAIN_GRAD_TABLES), tr.get(nb.TRAIN_GRAD_MLP)
-    l64, dt64, dm64, _ = train.train_gradients(lv, setup["cfg"], T64, M64, x.astype(np.float64),
-                                               t.astype(np.float64))
+    l64, dt64, dm64, _ = train.train_gradients(lv, cfg, T64, M64, x.astype(np.float64), t.astype(np.float64))
     assert abs(loss - l64) <= 1e-4 * abs(l64), (loss, l64)
     assert np.linalg.norm(gm - dm64) <= 1e-4 * np.linalg.norm(dm64)
     assert np.linalg.norm(gt - dt64) <= 1e-3 * np.linalg.norm(dt64)
     assert np.count_nonzero(gt) > 1000 and np.abs(dt64).max() > 0
+    # Entries no sample touched have an exactly zero gradient (the Adam step skips them).
+    assert np.array_equal(gt == 0, dt64 == 0)
     # gradients only: the parameters are untouched
-    assert np.array_equal(tr.get(nb.TRAIN_MLP), setup["mlp"]) and np.array_equal(tr.get(nb.TRAIN_TABLES),
-                                                                                 setup["tables"])
+    assert np.array_equal(tr.get(nb.TRAIN_MLP), mlp) and np.array_equal(tr.get(nb.TRAIN_TABLES), tables)
     tr.close()
+
+
+def test_gradients_match_oracle(nb, orc, ctx, setup):
+    _check_gradients(nb, ctx, setup["scene"], setup["cfg"], setup["tables"], setup["mlp"], setup["net"].levels(),
+                     setup["x"][:512], setup["t"][:512])
+
+
+@pytest.mark.parametrize("n", [1, 2000])
+def test_chess_shape_gradients_match_oracle(nb, orc, ctx, setup, n):
+    """The (3, 8, 4) shape of the reference's chess configuration (8 levels x 4 features, 2^18
+    entries, base 8, scale 2) on the same segments; n = 1 is the smallest batch."""
+    cfg = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=18, base_resolution=8,
+                            per_level_scale=2.0)
+    v, t, _, _, _ = bench_scene(nb)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    scene.init_network(cfg, seed=4, table_range=1e-2)
+    _, tables, mlp = scene.network()
+    lo, hi = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    rays, mint, maxt = nb.leaf_segments(scene.nbvh(), n, seed=2)
+    x = net.encode(rays, mint, maxt, mode=orc.FP32)
+    targets = nb.NeuralBVHModule(ctx, scene).segment_labels(rays, mint, maxt)
+    if n == 1:
+        # a single sample touches at most 3 x 8 x 8 entries
+        from oracle import train
+        tr = nb.Trainer(ctx, scene, max_batch=64)
+        tr.step(x, targets, gradients_only=True)
+        gt = tr.get(nb.TRAIN_GRAD_TABLES)
+        _, dt64, _, _ = train.train_gradients(net.levels(), cfg, tables.astype(np.float64), mlp.astype(np.float64),
+                                              x.astype(np.float64), targets.astype(np.float64))
+        assert np.array_equal(gt == 0, dt64 == 0) and 0 < np.count_nonzero(gt) <= 3 * 8 * 8 * 4
+        assert np.linalg.norm(gt - dt64) <= 1e-3 * np.linalg.norm(dt64)
+        tr.close()
+    else:
+        _check_gradients(nb, ctx, scene, cfg, tables, mlp, net.levels(), x, targets)
+    scene.close()
 
 
 def test_adam_step_matches_oracle(nb, orc, ctx, setup):
