@@ -457,7 +457,8 @@ typedef struct nbvh_train_config {
     float    beta2;         /* 0.999 */
     float    epsilon;       /* 1e-8 */
     float    l2_reg;        /* 1e-8 */
-    uint32_t max_batch;     /* samples per step, at most 2^24 (624 floats of device scratch each) */
+    uint32_t max_batch;     /* samples per step, at most 2^24 (720 floats of device scratch each,
+                               plus about 19 MB per trainer) */
     uint32_t reserved[4];
 } nbvh_train_config;
 
