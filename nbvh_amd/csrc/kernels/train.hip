@@ -44,8 +44,14 @@ constexpr int kK0 = 96, kH = 64, kO = 8, kLayers = 5;
 // dx (feature gradients).
 constexpr int kA0 = 0, kHid = kA0 + kK0, kY = kHid + 4 * kH, kD = kY + kO, kD5 = kD + 4 * kH, kDX = kD5 + kO;
 constexpr int kRows   = kDX + kK0;                          // 720 rows
-constexpr int kSliceFloats = 32768;                         // table-gradient slice in LDS (128 KB)
+#ifndef NBVH_SCATTER_SLICE_FLOATS
+#define NBVH_SCATTER_SLICE_FLOATS 32768
+#endif
+constexpr int kSliceFloats = NBVH_SCATTER_SLICE_FLOATS;     // table-gradient slice in LDS (128 KB)
 constexpr uint32_t kScatterThreads = 1024;                  // threads sharing one slice
+#ifndef NBVH_SCATTER_UNROLL
+#define NBVH_SCATTER_UNROLL 2                               // (sample, point) pairs per thread and iteration
+#endif
 constexpr int kMlp    = kH * kK0 + 3 * kH * kH + kO * kH;  // 18,944 matrix entries
 constexpr int kChunks = 256;                                // most matrix-gradient partials
 
@@ -220,6 +226,16 @@ __global__ __launch_bounds__(64) void train_backward_kernel(TrainArgs t, const f
     }
 }
 
+// Corner entries of a hashed level: grid_corners' coherent-prime hash alone.
+__device__ __forceinline__ void hash_corners(const GridLevel& L, const uint32_t g[3], uint32_t idx[8])
+{
+    const uint32_t hy0 = g[1] * 2654435761u, hy1 = (g[1] + 1u) * 2654435761u;
+    const uint32_t hz0 = g[2] * 805459861u, hz1 = (g[2] + 1u) * 805459861u;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; c++)
+        idx[c] = ((g[0] + (c & 1u)) ^ ((c & 2u) ? hy1 : hy0) ^ ((c & 4u) ? hz1 : hz0)) & (L.size - 1u);
+}
+
 // Work items of the table gradient. Slice j of level l holds the level's entries
 // [j * E, min((j + 1) * E, size)), E = kSliceFloats / F; each slice is split into parts[l] parts
 // of the batch's (sample, point) pairs; level l's items are [first[l], first[l + 1]), item
@@ -257,26 +273,29 @@ __global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArg
     const size_t S = t.act_stride;
     const uint32_t total = n * (uint32_t)P;
     const uint32_t j0 = (uint32_t)((uint64_t)total * part / parts), j1 = (uint32_t)((uint64_t)total * (part + 1) / parts);
-    // Two (sample, point) pairs per thread and iteration, their point loads issued together.
+    // U (sample, point) pairs per thread and iteration, their point loads issued together.
     constexpr uint32_t B = kScatterThreads;
-    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 2 * B) {
-        float pts[2][3];
+    constexpr int U = NBVH_SCATTER_UNROLL;
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += U * B) {
+        float pts[U][3];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < U; h++) {
             const uint32_t jj = j + h * B < j1 ? j + h * B : j;
             const uint32_t i = jj / P, p = jj - i * P;
 #pragma unroll
             for (int c = 0; c < 3; c++) pts[h][c] = in[(size_t)i * (3 * P) + 3 * p + c];
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < U; h++) {
             const uint32_t jj = j + h * B;
             if (jj >= j1) break;
             const uint32_t i = jj / P, p = jj - i * P;
             uint32_t g[3], idx[8];
             float fr[3];
             grid_cell(lv, pts[h][0], pts[h][1], pts[h][2], g, fr);
-            if (in_unit_cube<3>(pts[h]))
+            if (!lv.dense)  // uniform per workgroup: only the level's own index formula
+                hash_corners(lv, g, idx);
+            else if (in_unit_cube<3>(pts[h]))
                 grid_corners<false>(lv, g, idx);
             else
                 grid_corners<true>(lv, g, idx);
