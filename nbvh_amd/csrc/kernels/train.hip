@@ -255,9 +255,10 @@ struct SliceMap {
 // slices of the coarse dense levels, which receive many corners, are split so that no workgroup
 // adds many more corners than the others.
 template <int P, int L, int F>
-__global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArgs t, SliceMap map, const float* __restrict__ in,
-                                                            const float* __restrict__ act, float* __restrict__ gtab,
-                                                            uint32_t n)
+__global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArgs t, SliceMap map,
+                                                                        const float* __restrict__ in,
+                                                                        const float* __restrict__ act,
+                                                                        float* __restrict__ gtab, uint32_t n)
 {
     constexpr uint32_t E = kSliceFloats / F;
     __shared__ float acc[kSliceFloats];
