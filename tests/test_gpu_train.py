@@ -66,9 +66,13 @@ def _check_gradients(nb, ctx, scene, cfg, tables, mlp, lv, x, t):
     tr.close()
 
 
-def test_gradients_match_oracle(nb, orc, ctx, setup):
+@pytest.mark.parametrize("n", [512, 4000])
+def test_gradients_match_oracle(nb, orc, ctx, setup, n):
+    """n = 4000 puts 96,000 corners into the coarsest level's single slice, which the trainer then
+    splits across workgroups (their partial slices combined with global atomics); at n = 512 every
+    slice is summed by one workgroup."""
     _check_gradients(nb, ctx, setup["scene"], setup["cfg"], setup["tables"], setup["mlp"], setup["net"].levels(),
-                     setup["x"][:512], setup["t"][:512])
+                     setup["x"][:n], setup["t"][:n])
 
 
 @pytest.mark.parametrize("n", [1, 2000])
