@@ -833,6 +833,18 @@ def main():
             result["chess_nbvh"] = chess_bench(nb, hip, ctx, oracle)
         if not args.no_gather_ceiling:
             result["gather_ceiling"] = gather_ceiling()
+            # The query's memory-side lines (its profiled FETCH_SIZE + WRITE_SIZE traffic / 128 B;
+            # the hit writes are ~1 % of it) against the random-line rate measured for a table of
+            # the FP16 tables' size (26 MB -> 32 MiB).
+            traffic = result["roofline"]["traffic"]
+            if traffic and prec == nb.PREC_FP16 and (W, H) == (1024, 1024):
+                achieved = traffic / 128 / (ms * 1e-3) / 1e9
+                ceiling = result["gather_ceiling"]["32MiB_glines_per_s"]
+                result["roofline"]["line_rate"] = {"achieved_glines_per_s": round(achieved, 2),
+                                                   "ceiling_glines_per_s": ceiling,
+                                                   "frac": round(achieved / ceiling, 3)}
+                log(f"query memory-side line rate {achieved:.1f} G lines/s = {achieved / ceiling:.3f} of the "
+                    f"{ceiling} G lines/s random-line ceiling of a 32 MiB table")
         if not args.no_train:
             result["training"] = train_bench(nb, hip, ctx, oracle)
 
