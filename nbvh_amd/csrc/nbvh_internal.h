@@ -105,7 +105,7 @@ struct TrainArgs {
     NetArgs      net;          // shape and grid levels
     const float* tables;
     const float* mlp;
-    float*       act;          // workspace: activation rows (624 x act_stride floats), then
+    float*       act;          // workspace: activation rows (720 x act_stride floats), then
                                // transposed matrices and per-chunk matrix-gradient partials
     uint32_t     act_stride;   // samples per activation row (max batch rounded up to 64)
     float*       grad_tables;
