@@ -440,9 +440,13 @@ int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scen
  *          held constant, include/cuda/neural/utils/loss_gradient.cuh), these three masked where
  *          the target visibility is < 0.5; each element divided by n * 8;
  *   update Adam with L2 regularisation added to the gradient (defaults of
- *          include/cuda/neural/utils/tcnn_config.h:153-166; the learning-rate decay schedule is
- * the caller.s: change learning_rate between steps with a new trainer config). Parameters
- *          whose gradient is exactly zero (table entries no sample touched) are left unchanged.
+ *          include/cuda/neural/utils/tcnn_config.h:153-166), under the reference's exponential
+ *          learning-rate decay (src/cuda/neural/neural_bvh_module.cu:146-150: decay_start 10000,
+ *          decay_interval 2000, decay_base 0.33): after each update number s >= decay_start with
+ *          (s - decay_start) % decay_interval == 0 the rate is multiplied by decay_base, so update
+ *          t uses learning_rate * decay_base^k, k = the number of such s < t. decay_interval 0
+ *          turns the decay off. Parameters whose gradient is exactly zero (table entries no
+ *          sample touched) are left unchanged.
  * The segment sampling and tree-cut learning of the reference's training loop are not part of
  * this: inputs are encoded segments (as for nbvh_network_inference) and targets are the 8 outputs
  * (nbvh_segment_labels gives them from the scene's triangles). The trainer holds its own FP32 copy
@@ -459,7 +463,10 @@ typedef struct nbvh_train_config {
     float    l2_reg;        /* 1e-8 */
     uint32_t max_batch;     /* samples per step, at most 2^24 (720 floats of device scratch each,
                                plus about 19 MB per trainer) */
-    uint32_t reserved[4];
+    uint32_t decay_start;   /* 10000 in the reference; 0 with decay_interval 0: no decay */
+    uint32_t decay_interval;/* 2000 in the reference; 0: no decay */
+    float    decay_base;    /* 0.33 in the reference */
+    uint32_t reserved;
 } nbvh_train_config;
 
 enum nbvh_train_flags {

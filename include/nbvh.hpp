@@ -260,6 +260,9 @@ public:
         c.epsilon       = 1e-8f;
         c.l2_reg        = 1e-8f;
         c.max_batch     = max_batch;
+        c.decay_start    = 10000;  // the reference's exponential learning-rate decay
+        c.decay_interval = 2000;
+        c.decay_base     = 0.33f;
         check(nbvh_trainer_create(ctx_, scene.get(), &c, &t_), ctx_);
     }
     ~Trainer() { nbvh_trainer_destroy(t_); }

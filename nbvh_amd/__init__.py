@@ -511,12 +511,15 @@ def leaf_segments(nbvh, n, seed=0, lod=0):
 class Trainer:
     """Training of a scene's network (nbvh_trainer_*): the replacement of the reference's
     trainer->training_step on encoded segments and their 8 targets, composite loss and Adam with
-    the reference's defaults. commit() writes the trained network into the scene."""
+    the reference's defaults, including its exponential learning-rate decay (x decay_base after
+    update decay_start and every decay_interval updates after it; decay_interval=0 turns it off).
+    commit() writes the trained network into the scene."""
 
     def __init__(self, ctx, scene, learning_rate=1e-2, beta1=0.9, beta2=0.999, epsilon=1e-8, l2_reg=1e-8,
-                 max_batch=1 << 18):
+                 max_batch=1 << 18, decay_start=10000, decay_interval=2000, decay_base=0.33):
         self.ctx, self.scene = ctx, scene
-        cfg = _ffi.TrainConfig(learning_rate, beta1, beta2, epsilon, l2_reg, max_batch)
+        cfg = _ffi.TrainConfig(learning_rate, beta1, beta2, epsilon, l2_reg, max_batch, decay_start,
+                               decay_interval, decay_base)
         self.handle = ctypes.c_void_p()
         self.ctx.check(_lib().nbvh_trainer_create(ctx.handle, scene.handle, ctypes.byref(cfg),
                                                   ctypes.byref(self.handle)))

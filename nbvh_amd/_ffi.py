@@ -92,7 +92,8 @@ class PathStats(ctypes.Structure):
 class TrainConfig(ctypes.Structure):
     _fields_ = [("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
                 ("epsilon", ctypes.c_float), ("l2_reg", ctypes.c_float), ("max_batch", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("decay_start", ctypes.c_uint32), ("decay_interval", ctypes.c_uint32), ("decay_base", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
 
 
 class LaunchShape(ctypes.Structure):

@@ -31,6 +31,16 @@ bool trainable(const nbvh_network_config& c)
     return shape && c.n_neurons == 64 && c.n_hidden_layers == 4 && c.n_outputs == 8;
 }
 
+// Learning-rate factor of update number `step` (from 1) under the exponential decay: decay_base to
+// the number of earlier updates s >= decay_start with (s - decay_start) % decay_interval == 0.
+float decay_factor(const nbvh_train_config& c, uint64_t step)
+{
+    if (c.decay_interval == 0 || step <= 1 || step - 1 < c.decay_start) return 1.f;
+    // s = decay_start, decay_start + interval, ... <= step - 1; s = 0 is not an update.
+    const uint64_t k = (step - 1 - c.decay_start) / c.decay_interval + (c.decay_start > 0 ? 1 : 0);
+    return (float)std::pow((double)c.decay_base, (double)k);
+}
+
 TrainArgs train_args(nbvh_trainer* t)
 {
     TrainArgs a{};
@@ -57,6 +67,8 @@ NBVH_API int32_t nbvh_trainer_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbv
         if (!scene->has_net) fail(NBVH_ERR_STATE, "training a scene without network weights");
         if (!trainable(scene->cfg)) fail(NBVH_ERR_UNSUPPORTED, "no training kernels for this network shape");
         require(cfg->max_batch >= 1 && cfg->max_batch <= (1u << 24), "max_batch must be 1..2^24");
+        require(cfg->decay_interval == 0 || (std::isfinite(cfg->decay_base) && cfg->decay_base > 0.f),
+                "decay_base must be finite and > 0");
         require(cfg->learning_rate > 0.f && cfg->beta1 >= 0.f && cfg->beta1 < 1.f && cfg->beta2 >= 0.f &&
                     cfg->beta2 < 1.f && cfg->epsilon >= 0.f && cfg->l2_reg >= 0.f,
                 "optimiser parameters out of range");
@@ -110,7 +122,7 @@ NBVH_API int32_t nbvh_trainer_step(nbvh_trainer* t, const float* d_inputs, const
         if ((flags & NBVH_TRAIN_GRADIENTS_ONLY) == 0u && n > 0) {
             t->step++;
             AdamArgs h{};
-            h.lr        = t->cfg.learning_rate;
+            h.lr        = t->cfg.learning_rate * decay_factor(t->cfg, t->step);
             h.beta1     = t->cfg.beta1;
             h.beta2     = t->cfg.beta2;
             h.epsilon   = t->cfg.epsilon;

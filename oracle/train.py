@@ -14,8 +14,9 @@ precision:
     visibility is < 0.5, each element's loss and gradient divided by n * 8;
   - backpropagation to the MLP matrices and the grid tables;
   - one Adam step (Kingma and Ba 2015, Algorithm 1) with L2 regularisation added to the gradient,
-    the reference's optimiser defaults (include/cuda/neural/utils/tcnn_config.h:153-166);
-    parameters with an exactly zero gradient are left alone (sparse hash-table updates).
+    the reference's optimiser defaults (include/cuda/neural/utils/tcnn_config.h:153-166) and
+    its exponential learning-rate decay (lr_factor); parameters with an exactly zero gradient
+    are left alone (sparse hash-table updates).
 The gradients are pinned by central finite differences of the loss (tests/test_oracle_train.py).
 """
 import numpy as np
@@ -149,6 +150,18 @@ def train_gradients(lv, cfg, tables, mlp, x, targets):
         np.add.at(dt, idx.reshape(-1), (w[:, :, None] * dx[:, None, k:k + F]).reshape(-1, F))
     dmlp = np.concatenate([g.reshape(-1) for g in dW])
     return float(values.sum()), dtab, dmlp, y
+
+
+def lr_factor(step, decay_start, decay_interval, decay_base):
+    """Learning-rate factor of update number `step` (from 1) under the reference's exponential
+    decay (src/cuda/neural/neural_bvh_module.cu:146-150, tcnn's ExponentialDecay wrapper around
+    Adam): decay_base to the number of earlier updates s >= decay_start with
+    (s - decay_start) % decay_interval == 0, counted one by one. Which update first uses a decayed
+    rate is tcnn's convention, absent here: parity unpinned for that off-by-one."""
+    if decay_interval == 0:
+        return 1.0
+    k = sum(1 for s in range(1, step) if s >= decay_start and (s - decay_start) % decay_interval == 0)
+    return decay_base ** k
 
 
 def adam_step(w, g, m, v, step, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8, l2_reg=1e-8):

@@ -125,6 +125,28 @@ def test_adam_step_matches_oracle(nb, orc, ctx, setup):
     tr.close()
 
 
+def test_adam_with_lr_decay_matches_oracle(nb, orc, ctx, setup):
+    """Five updates under a short exponential decay (x 0.5 after updates 2 and 4): every update
+    equals the restatement's Adam step, at the restatement's learning-rate factor, on the
+    trainer's previous parameters and its own gradients (moments carried in FP64). A first Adam
+    step moves each parameter by about the learning rate, so a wrong factor shows as ~1e-3."""
+    from oracle import train
+    x, t = setup["x"], setup["t"]
+    tr = nb.Trainer(ctx, setup["scene"], max_batch=4096, decay_start=2, decay_interval=2, decay_base=0.5)
+    mom = {k: (np.zeros(n), np.zeros(n)) for k, n in (("t", setup["tables"].size), ("m", setup["mlp"].size))}
+    for step in range(1, 6):
+        prev = {"t": tr.get(nb.TRAIN_TABLES).astype(np.float64), "m": tr.get(nb.TRAIN_MLP).astype(np.float64)}
+        tr.step(x[step * 2000:step * 2000 + 2000], t[step * 2000:step * 2000 + 2000])
+        grads = {"t": tr.get(nb.TRAIN_GRAD_TABLES).astype(np.float64), "m": tr.get(nb.TRAIN_GRAD_MLP).astype(np.float64)}
+        got = {"t": tr.get(nb.TRAIN_TABLES), "m": tr.get(nb.TRAIN_MLP)}
+        lr = 1e-2 * train.lr_factor(step, 2, 2, 0.5)
+        for k in ("t", "m"):
+            w, m, v = train.adam_step(prev[k], grads[k], mom[k][0], mom[k][1], step, lr=lr)
+            mom[k] = (m, v)
+            assert np.allclose(got[k], w, rtol=0, atol=1e-6), (step, k, np.abs(got[k] - w).max())
+    tr.close()
+
+
 def test_training_learns_visibility(nb, orc, ctx, setup):
     from nbvh_amd import hip
     x, t = setup["x"], setup["t"]

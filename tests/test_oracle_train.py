@@ -112,3 +112,15 @@ def test_adam_closed_form(small):
     w4, m4, v4 = train.adam_step(w, g4, np.zeros(100), np.zeros(100), 1, lr=0.01)
     assert np.array_equal(w4[::2], w[::2]) and not m4[::2].any() and not v4[::2].any()  # sparse update
     assert np.allclose(w4[1::2], w[1::2] - 0.01 * np.sign(g[1::2]), atol=1e-6)
+
+
+def test_lr_decay_schedule():
+    """The exponential decay's factor per update (from 1): x base after update decay_start and
+    every decay_interval updates after it, worked by hand for small schedules and the
+    reference's defaults (10000, 2000, 0.33)."""
+    from oracle import train
+    assert [train.lr_factor(t, 2, 2, 0.5) for t in range(1, 8)] == [1, 1, 0.5, 0.5, 0.25, 0.25, 0.125]
+    assert [train.lr_factor(t, 0, 2, 0.5) for t in range(1, 7)] == [1, 1, 0.5, 0.5, 0.25, 0.25]
+    assert [train.lr_factor(t, 10000, 2000, 0.33) for t in (1, 10000, 10001, 12000, 12001)] == \
+        [1, 1, 0.33, 0.33, 0.33 ** 2]
+    assert train.lr_factor(50, 1, 0, 0.5) == 1.0
