@@ -525,14 +525,17 @@ class Trainer:
                                                   ctypes.byref(self.handle)))
         self.max_batch = max_batch
 
-    def step_device(self, d_inputs_ptr, d_targets_ptr, n, gradients_only=False, stream=None):
-        """One step on device buffers (n x 9 inputs, n x 8 targets); returns the loss."""
+    def step_device(self, d_inputs_ptr, d_targets_ptr, n, gradients_only=False, stream=None, want_loss=True):
+        """One step on device buffers (n x 9 inputs, n x 8 targets). want_loss: wait for the step
+        and return its loss; otherwise the step is only queued on the stream (returns None), so
+        steps can be issued back to back."""
         loss = ctypes.c_float()
         self.ctx.check(_lib().nbvh_trainer_step(self.handle, ctypes.c_void_p(d_inputs_ptr),
                                                 ctypes.c_void_p(d_targets_ptr), n,
                                                 _ffi.TRAIN_GRADIENTS_ONLY if gradients_only else 0,
-                                                ctypes.byref(loss), ctypes.c_void_p(stream or 0)))
-        return loss.value
+                                                ctypes.byref(loss) if want_loss else None,
+                                                ctypes.c_void_p(stream or 0)))
+        return loss.value if want_loss else None
 
     def step(self, inputs, targets, gradients_only=False):
         from . import hip

@@ -17,7 +17,9 @@ struct nbvh_trainer {
     nbvh_train_config cfg{};
     uint64_t n_table = 0, n_mlp = 0;
     uint32_t step    = 0;
-    DevBuf tables, mlp, g_tables, g_mlp, m_tables, v_tables, m_mlp, v_mlp, act, loss;
+    uint64_t net_generation = 0;  // the scene's network generation the buffers were sized for
+    NetArgs net{};                // the scene's network arguments at creation (shape and levels)
+    DevBuf tables, mlp, g_tables, g_mlp, m_tables, v_tables, m_mlp, v_mlp, n_tables, n_mlp_steps, act, loss;
 };
 
 namespace {
@@ -41,10 +43,19 @@ float decay_factor(const nbvh_train_config& c, uint64_t step)
     return (float)std::pow((double)c.decay_base, (double)k);
 }
 
+// The trainer's buffers are sized for the network the scene held at nbvh_trainer_create; a later
+// nbvh_scene_load_network / nbvh_scene_init_network replaces it (possibly with other table sizes),
+// after which the trainer refuses to run or commit.
+void check_generation(const nbvh_trainer* t)
+{
+    if (t->scene->net_generation != t->net_generation)
+        fail(NBVH_ERR_STATE, "the scene's network was replaced after the trainer was created");
+}
+
 TrainArgs train_args(nbvh_trainer* t)
 {
     TrainArgs a{};
-    a.net         = t->scene->args;
+    a.net         = t->net;
     a.tables      = t->tables.as<float>();
     a.mlp         = t->mlp.as<float>();
     a.act         = t->act.as<float>();
@@ -79,16 +90,22 @@ NBVH_API int32_t nbvh_trainer_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbv
         t->cfg = *cfg;
         t->n_table = scene->tables.size();
         t->n_mlp = scene->mlp.size();
+        t->net_generation = scene->net_generation;
+        t->net = scene->args;
         t->tables.upload(ctx, scene->tables.data(), t->n_table);
         t->mlp.upload(ctx, scene->mlp.data(), t->n_mlp);
         for (DevBuf* b : {&t->g_tables, &t->m_tables, &t->v_tables}) b->alloc(ctx, t->n_table * sizeof(float));
         for (DevBuf* b : {&t->g_mlp, &t->m_mlp, &t->v_mlp}) b->alloc(ctx, t->n_mlp * sizeof(float));
+        t->n_tables.alloc(ctx, t->n_table * sizeof(uint32_t));
+        t->n_mlp_steps.alloc(ctx, t->n_mlp * sizeof(uint32_t));
         t->act.alloc(ctx, train_act_bytes(cfg->max_batch));
         t->loss.alloc(ctx, 256);
         hip_check(train_zero(t->m_tables.as<float>(), t->n_table, nullptr), "zero");
         hip_check(train_zero(t->v_tables.as<float>(), t->n_table, nullptr), "zero");
         hip_check(train_zero(t->m_mlp.as<float>(), t->n_mlp, nullptr), "zero");
         hip_check(train_zero(t->v_mlp.as<float>(), t->n_mlp, nullptr), "zero");
+        hip_check(train_zero(t->n_tables.as<float>(), t->n_table, nullptr), "zero");  // update counts
+        hip_check(train_zero(t->n_mlp_steps.as<float>(), t->n_mlp, nullptr), "zero");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         *out = t.release();
     });
@@ -111,6 +128,7 @@ NBVH_API int32_t nbvh_trainer_step(nbvh_trainer* t, const float* d_inputs, const
         require((flags & ~(uint32_t)NBVH_TRAIN_GRADIENTS_ONLY) == 0u, "unknown training flags");
         require(n <= t->cfg.max_batch, "batch larger than the trainer's max_batch");
         if (n > 0) require(d_inputs && d_targets, "NULL buffer");
+        check_generation(t);
         hipStream_t st = (hipStream_t)stream;
         hip_check(hipSetDevice(t->ctx->device), "hipSetDevice");
         hip_check(train_zero(t->g_tables.as<float>(), t->n_table, st), "zero");
@@ -127,13 +145,11 @@ NBVH_API int32_t nbvh_trainer_step(nbvh_trainer* t, const float* d_inputs, const
             h.beta2     = t->cfg.beta2;
             h.epsilon   = t->cfg.epsilon;
             h.l2_reg    = t->cfg.l2_reg;
-            h.inv_bias1 = (float)(1.0 / (1.0 - std::pow((double)t->cfg.beta1, (double)t->step)));
-            h.inv_bias2 = (float)(1.0 / (1.0 - std::pow((double)t->cfg.beta2, (double)t->step)));
             hip_check(train_adam(t->tables.as<float>(), t->g_tables.as<float>(), t->m_tables.as<float>(),
-                                 t->v_tables.as<float>(), t->n_table, h, st),
+                                 t->v_tables.as<float>(), t->n_tables.as<uint32_t>(), t->n_table, false, h, st),
                       "adam tables");
             hip_check(train_adam(t->mlp.as<float>(), t->g_mlp.as<float>(), t->m_mlp.as<float>(), t->v_mlp.as<float>(),
-                                 t->n_mlp, h, st),
+                                 t->n_mlp_steps.as<uint32_t>(), t->n_mlp, true, h, st),
                       "adam mlp");
         }
         if (loss) {
@@ -169,6 +185,7 @@ NBVH_API int32_t nbvh_trainer_commit(nbvh_trainer* t)
 {
     return guarded(t ? t->ctx : nullptr, [&] {
         require(t != nullptr, "NULL trainer");
+        check_generation(t);
         nbvh_scene* s = t->scene;
         hip_check(hipSetDevice(t->ctx->device), "hipSetDevice");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");

@@ -55,7 +55,8 @@ size_t   train_act_bytes(uint32_t max_batch);
 hipError_t train_zero(float* p, uint64_t n, hipStream_t stream);
 hipError_t train_gradients(const TrainArgs& t, const float* in, const float* targets, uint32_t n, hipStream_t stream,
                            bool* supported);
-hipError_t train_adam(float* w, const float* g, float* m, float* v, uint64_t n, const AdamArgs& h, hipStream_t stream);
+hipError_t train_adam(float* w, const float* g, float* m, float* v, uint32_t* steps, uint64_t n, bool matrix,
+                      const AdamArgs& h, hipStream_t stream);
 
 // Path-trace driver: per-path state init (pixel = index, throughput 1) and one depth's shading
 // and compaction (live paths appended at *counter).

@@ -22,9 +22,9 @@
 //                    workgroup reduces one chunk of samples for one matrix through LDS tiles of 32
 //                    samples and writes its partial; train_wgrad_reduce adds the partials in
 //                    chunk order (deterministic);
-//   train_adam       Adam (Kingma and Ba 2015, Algorithm 1) with L2 regularisation added to the
-//                    gradient, on the tables and the matrices; parameters whose gradient is
-//                    exactly zero (table entries no sample touched) keep their value and moments.
+//   train_adam       tiny-cuda-nn's Adam: per-parameter update counts for the bias correction,
+//                    L2 regularisation and dense updates for the matrices, sparse updates
+//                    without L2 for the tables (entries no sample touched keep their value).
 // Gradients are sums over the batch of the per-element gradients divided by n * 8, as the
 // reference's loss kernel writes them; the table gradients (LDS atomics) and the loss are summed
 // in an order that is not fixed.
@@ -404,18 +404,32 @@ __global__ __launch_bounds__(64) void train_wgrad_reduce_kernel(const float* __r
     grad[j] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
+// tiny-cuda-nn's AdamOptimizer step (public optimizers/adam.h, the reference's optimiser):
+// MATRIX parameters (the MLP) get L2 regularisation added to the gradient and are updated every
+// step, also with a zero gradient (their moments decay); the other parameters (the grid tables)
+// are skipped where the gradient is exactly zero and get no L2 term. Each parameter counts its
+// own updates t, and the bias correction is folded into the rate:
+// w -= lr * sqrt(1 - beta2^t) / (1 - beta1^t) * m / (sqrt(v) + epsilon).
+template <bool MATRIX>
 __global__ __launch_bounds__(256) void train_adam_kernel(float* __restrict__ w, const float* __restrict__ g,
-                                                         float* __restrict__ m, float* __restrict__ v, uint64_t n,
-                                                         AdamArgs h)
+                                                         float* __restrict__ m, float* __restrict__ v,
+                                                         uint32_t* __restrict__ steps, uint64_t n, AdamArgs h)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || g[i] == 0.f) return;  // sparse update: entries no sample touched stay as they are
-    const float gi = g[i] + h.l2_reg * w[i];
+    if (i >= n) return;
+    float gi = g[i];
+    if (MATRIX)
+        gi += h.l2_reg * w[i];
+    else if (gi == 0.f)
+        return;  // sparse update: entries no sample touched keep their value, moments and count
     const float mi = h.beta1 * m[i] + (1.f - h.beta1) * gi;
     const float vi = h.beta2 * v[i] + (1.f - h.beta2) * gi * gi;
+    const uint32_t t = steps[i] + 1u;
+    const float lr = h.lr * sqrtf(1.f - powf(h.beta2, (float)t)) / (1.f - powf(h.beta1, (float)t));
     m[i] = mi;
     v[i] = vi;
-    w[i] = w[i] - h.lr * (mi * h.inv_bias1) / (sqrtf(vi * h.inv_bias2) + h.epsilon);
+    steps[i] = t;
+    w[i] = w[i] - lr * mi / (sqrtf(vi) + h.epsilon);
 }
 
 __global__ __launch_bounds__(256) void train_zero_kernel(float* __restrict__ p, uint64_t n)
@@ -494,10 +508,15 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
     return hipGetLastError();
 }
 
-hipError_t train_adam(float* w, const float* g, float* m, float* v, uint64_t n, const AdamArgs& h, hipStream_t stream)
+hipError_t train_adam(float* w, const float* g, float* m, float* v, uint32_t* steps, uint64_t n, bool matrix,
+                      const AdamArgs& h, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(dev::train_adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, w, g, m, v, n, h);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (matrix)
+        hipLaunchKernelGGL(dev::train_adam_kernel<true>, grid, dim3(256), 0, stream, w, g, m, v, steps, n, h);
+    else
+        hipLaunchKernelGGL(dev::train_adam_kernel<false>, grid, dim3(256), 0, stream, w, g, m, v, steps, n, h);
     return hipGetLastError();
 }
 

@@ -164,14 +164,25 @@ def lr_factor(step, decay_start, decay_interval, decay_base):
     return decay_base ** k
 
 
-def adam_step(w, g, m, v, step, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8, l2_reg=1e-8):
-    """One Adam step (step counts from 1); returns new (w, m, v). Parameters whose gradient is
-    exactly zero (hash-table entries no sample touched) keep their value and moments."""
-    live = g != 0
-    g2 = g + l2_reg * w
+def adam_step(w, g, m, v, steps, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8, l2_reg=1e-8, matrix=False):
+    """One step of tiny-cuda-nn's AdamOptimizer, the reference's optimiser (public
+    tiny-cuda-nn optimizers/adam.h; tcnn is an empty submodule in the reference, so this restates
+    the published code): `steps` holds each parameter's count of earlier updates (array or
+    scalar). matrix=True (the MLP's matrices): L2 regularisation is added to the gradient and the
+    parameter is updated every step, also with a zero gradient. matrix=False (the grid tables):
+    no L2, and parameters whose gradient is exactly zero keep their value, moments and count.
+    The update is w -= lr * sqrt(1 - beta2^t) / (1 - beta1^t) * m / (sqrt(v) + eps) with t the
+    parameter's own update number. Returns new (w, m, v, steps)."""
+    steps = np.broadcast_to(np.asarray(steps, dtype=np.int64), np.shape(w))
+    if matrix:
+        live = np.ones(np.shape(w), dtype=bool)
+        g2 = g + l2_reg * w
+    else:
+        live = g != 0
+        g2 = g
     m2 = beta1 * m + (1.0 - beta1) * g2
     v2 = beta2 * v + (1.0 - beta2) * g2 * g2
-    mh = m2 / (1.0 - beta1 ** step)
-    vh = v2 / (1.0 - beta2 ** step)
-    w2 = w - lr * mh / (np.sqrt(vh) + eps)
-    return np.where(live, w2, w), np.where(live, m2, m), np.where(live, v2, v)
+    t = steps + 1
+    rate = lr * np.sqrt(1.0 - beta2 ** t) / (1.0 - beta1 ** t)
+    w2 = w - rate * m2 / (np.sqrt(v2) + eps)
+    return np.where(live, w2, w), np.where(live, m2, m), np.where(live, v2, v), np.where(live, t, steps)

@@ -115,8 +115,9 @@ def test_adam_step_matches_oracle(nb, orc, ctx, setup):
     # The gradients the update used stay in the trainer: the restatement's Adam on them.
     gt, gm = tr.get(nb.TRAIN_GRAD_TABLES), tr.get(nb.TRAIN_GRAD_MLP)
     wt, wm = tr.get(nb.TRAIN_TABLES), tr.get(nb.TRAIN_MLP)
-    et, _, _ = train.adam_step(setup["tables"].astype(np.float64), gt.astype(np.float64), 0.0, 0.0, 1)
-    em, _, _ = train.adam_step(setup["mlp"].astype(np.float64), gm.astype(np.float64), 0.0, 0.0, 1)
+    et, _, _, _ = train.adam_step(setup["tables"].astype(np.float64), gt.astype(np.float64), 0.0, 0.0, 0)
+    em, _, _, _ = train.adam_step(setup["mlp"].astype(np.float64), gm.astype(np.float64), 0.0, 0.0, 0,
+                                  matrix=True)
     assert np.allclose(wm, em, rtol=0, atol=1e-6)
     assert np.allclose(wt, et, rtol=0, atol=1e-6)
     untouched = gt == 0
@@ -133,7 +134,8 @@ def test_adam_with_lr_decay_matches_oracle(nb, orc, ctx, setup):
     from oracle import train
     x, t = setup["x"], setup["t"]
     tr = nb.Trainer(ctx, setup["scene"], max_batch=4096, decay_start=2, decay_interval=2, decay_base=0.5)
-    mom = {k: (np.zeros(n), np.zeros(n)) for k, n in (("t", setup["tables"].size), ("m", setup["mlp"].size))}
+    mom = {k: (np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64))
+           for k, n in (("t", setup["tables"].size), ("m", setup["mlp"].size))}
     for step in range(1, 6):
         prev = {"t": tr.get(nb.TRAIN_TABLES).astype(np.float64), "m": tr.get(nb.TRAIN_MLP).astype(np.float64)}
         tr.step(x[step * 2000:step * 2000 + 2000], t[step * 2000:step * 2000 + 2000])
@@ -141,8 +143,9 @@ def test_adam_with_lr_decay_matches_oracle(nb, orc, ctx, setup):
         got = {"t": tr.get(nb.TRAIN_TABLES), "m": tr.get(nb.TRAIN_MLP)}
         lr = 1e-2 * train.lr_factor(step, 2, 2, 0.5)
         for k in ("t", "m"):
-            w, m, v = train.adam_step(prev[k], grads[k], mom[k][0], mom[k][1], step, lr=lr)
-            mom[k] = (m, v)
+            w, m, v, s = train.adam_step(prev[k], grads[k], mom[k][0], mom[k][1], mom[k][2], lr=lr,
+                                         matrix=k == "m")
+            mom[k] = (m, v, s)
             assert np.allclose(got[k], w, rtol=0, atol=1e-6), (step, k, np.abs(got[k] - w).max())
     tr.close()
 
@@ -159,7 +162,8 @@ def test_training_learns_visibility(nb, orc, ctx, setup):
         return float(np.mean((out[:, 0] > 0.5) == (t[held, 0] > 0.5)))
 
     acc0 = accuracy()
-    tr = nb.Trainer(ctx, setup["scene"], max_batch=8192)
+    scene = setup["scene"]
+    tr = nb.Trainer(ctx, scene, max_batch=8192)
     dx, dt = hip.Buffer.from_array(x[pool]), hip.Buffer.from_array(t[pool])
     losses = []
     for step in range(250):
@@ -172,8 +176,34 @@ def test_training_learns_visibility(nb, orc, ctx, setup):
     assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5]), (losses[:5], losses[-20:])
     assert acc1 > prior + 0.15 and acc1 > acc0 + 0.15, (acc0, acc1, prior)
     # The committed network is the trainer's, in every precision.
-    cfg, tables, mlp = setup["scene"].network()
+    cfg, tables, mlp = scene.network()
     assert np.array_equal(mlp, tr.get(nb.TRAIN_MLP)) and np.array_equal(tables, tr.get(nb.TRAIN_TABLES))
     out16, _ = setup["mod"].network_inference(x[held], precision=nb.PREC_FP16)
     assert np.mean((out16[:, 0] > 0.5) == (t[held, 0] > 0.5)) > acc1 - 0.01
     tr.close()
+    # Put the module's seeded network back for the tests that compare against setup's copies.
+    scene.load_network_model(setup["cfg"], setup["tables"], setup["mlp"])
+
+
+def test_trainer_refuses_a_replaced_network(nb, orc, ctx, setup):
+    """A trainer's buffers are sized for the network of its creation: after the scene's network
+    is replaced (here by a smaller table size), step and commit fail with NBVH_ERR_STATE before
+    any launch or copy, and a new trainer works on the new network."""
+    v, tri, _, _, _ = bench_scene(nb)
+    scene = nb.Scene(ctx, v, tri)
+    scene.build_nbvh(64)
+    scene.init_network(nb.network_config(), seed=2, table_range=1e-2)
+    x, t = setup["x"][:512], setup["t"][:512]
+    tr = nb.Trainer(ctx, scene, max_batch=1024)
+    tr.step(x, t)
+    scene.init_network(nb.network_config(log2_hashmap_size=14), seed=3, table_range=1e-2)
+    for call in (lambda: tr.step(x, t), tr.commit):
+        with pytest.raises(nb.NbvhError) as e:
+            call()
+        assert e.value.status == nb._ffi.ERR_STATE
+    tr.close()
+    tr2 = nb.Trainer(ctx, scene, max_batch=1024)
+    assert np.isfinite(tr2.step(x, t))
+    tr2.commit()
+    tr2.close()
+    scene.close()

@@ -96,22 +96,49 @@ def test_composite_loss_cases(small):
 
 
 def test_adam_closed_form(small):
+    """tiny-cuda-nn's Adam in closed form: the first update of a parameter moves it by the rate
+    against its gradient's sign; a constant gradient keeps doing so; L2 enters the matrices only;
+    table entries with a zero gradient are skipped (value, moments and count), matrix entries are
+    not; each parameter's bias correction uses its own count; epsilon sits outside the
+    bias-corrected square root."""
     train = small[0]
     rng = np.random.default_rng(1)
     w, g = rng.normal(size=100), rng.normal(size=100)
-    w1, m1, v1 = train.adam_step(w, g, np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=0.0, eps=0.0)
-    assert np.allclose(w1, w - 0.01 * np.sign(g))  # first step: m^ = g, v^ = g^2
-    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
-    w2, _, _ = train.adam_step(w1, g, m1, v1, 2, lr=0.01, l2_reg=0.0, eps=0.0)
-    assert np.allclose(w2, w1 - 0.01 * np.sign(g))  # a constant gradient keeps m^/sqrt(v^) = sign
-    w3, _, _ = train.adam_step(w, np.full(100, 1e-30), np.zeros(100), np.zeros(100), 1, lr=0.01, l2_reg=1.0,
-                               eps=0.0)
-    assert np.allclose(w3, w - 0.01 * np.sign(w))  # L2 regularisation enters as g + l2 * w
+    z = np.zeros(100)
+    for matrix in (False, True):
+        w1, m1, v1, s1 = train.adam_step(w, g, z, z, 0, lr=0.01, l2_reg=0.0, eps=0.0, matrix=matrix)
+        assert np.allclose(w1, w - 0.01 * np.sign(g))  # first step: m^ = g, v^ = g^2
+        assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g) and np.all(s1 == 1)
+        w2, _, _, s2 = train.adam_step(w1, g, m1, v1, s1, lr=0.01, l2_reg=0.0, eps=0.0, matrix=matrix)
+        assert np.allclose(w2, w1 - 0.01 * np.sign(g)) and np.all(s2 == 2)  # constant gradient
+    # L2 regularisation enters the matrices as g + l2 * w, and not the tables.
+    tiny = np.full(100, 1e-30)
+    w3, _, _, _ = train.adam_step(w, tiny, z, z, 0, lr=0.01, l2_reg=1.0, eps=0.0, matrix=True)
+    assert np.allclose(w3, w - 0.01 * np.sign(w))
+    w3t, _, _, _ = train.adam_step(w, tiny, z, z, 0, lr=0.01, l2_reg=1.0, eps=0.0, matrix=False)
+    assert np.allclose(w3t, w - 0.01)  # sign of the tiny positive gradient only
+    # Zero gradients: tables keep everything; matrices update from their decaying moments.
     g4 = g.copy()
     g4[::2] = 0.0
-    w4, m4, v4 = train.adam_step(w, g4, np.zeros(100), np.zeros(100), 1, lr=0.01)
-    assert np.array_equal(w4[::2], w[::2]) and not m4[::2].any() and not v4[::2].any()  # sparse update
-    assert np.allclose(w4[1::2], w[1::2] - 0.01 * np.sign(g[1::2]), atol=1e-6)
+    w4, m4, v4, s4 = train.adam_step(w, g4, z, z, 0, lr=0.01)
+    assert np.array_equal(w4[::2], w[::2]) and not m4[::2].any() and not v4[::2].any() and not s4[::2].any()
+    assert np.allclose(w4[1::2], w[1::2] - 0.01 * np.sign(g[1::2]), atol=1e-6) and np.all(s4[1::2] == 1)
+    m0, v0 = np.full(100, 0.5), np.full(100, 0.25)
+    w5, m5, v5, s5 = train.adam_step(w, z, m0, v0, 3, lr=0.01, l2_reg=0.0, eps=0.0, matrix=True)
+    rate = 0.01 * np.sqrt(1 - 0.999 ** 4) / (1 - 0.9 ** 4)
+    assert np.allclose(m5, 0.45) and np.allclose(v5, 0.999 * 0.25) and np.all(s5 == 4)
+    assert np.allclose(w5, w - rate * 0.45 / np.sqrt(0.999 * 0.25))
+    # Per-parameter counts: an entry first touched at its own update 1 moves by the full rate,
+    # whatever the optimiser's global step (a global count of 5 would move it by 0.01 *
+    # sqrt(1 - 0.999^5) / (1 - 0.9^5) * 0.1 / sqrt(0.001), about 0.0049).
+    w6, _, _, _ = train.adam_step(w, g, z, z, 0, lr=0.01, l2_reg=0.0, eps=0.0)
+    assert np.allclose(np.abs(w6 - w), 0.01)
+    # Epsilon: lr * sqrt(1 - b2) / (1 - b1) * m / (sqrt(v) + eps) at the first update, here with
+    # eps = 1 and g = 1: 0.01 * sqrt(0.001) / (sqrt(0.001) + 1); the textbook form
+    # m^ / (sqrt(v^) + eps) would give 0.01 * 1 / (1 + 1).
+    w7, _, _, _ = train.adam_step(np.zeros(1), np.ones(1), np.zeros(1), np.zeros(1), 0, lr=0.01, l2_reg=0.0,
+                                  eps=1.0)
+    assert np.isclose(-w7[0], 0.01 * np.sqrt(0.001) / (np.sqrt(0.001) + 1.0))
 
 
 def test_lr_decay_schedule():
