@@ -136,6 +136,10 @@ void refresh_args(nbvh_scene* s)
     for (uint32_t k = 0; k < 4; k++) a.seg_t[k] = k < P ? delta * ((float)k + offset) : 0.f;
     const auto lv = grid_levels(cfg.grid);
     for (size_t l = 0; l < lv.size(); l++) a.level[l] = lv[l];
+    a.n_dense = 0;
+    while (a.n_dense < lv.size() && lv[a.n_dense].dense) a.n_dense++;
+    for (size_t l = a.n_dense; l < lv.size(); l++)
+        if (lv[l].dense) a.n_dense = (uint32_t)lv.size();  // not a dense prefix: every level takes the general path
     a.tables_f32      = s->d_tables32.as<float>();
     a.tables_f16      = s->d_tables16.as<uint16_t>();
     // The MFMA kernels address the FP16 tables through a buffer resource (31-bit byte range);
@@ -176,20 +180,54 @@ void upload_network(nbvh_scene* s)
     for (int i = 0; i < 8; i++) s->fp8_wscale[i] = mw.fp8_wscale[i];
 }
 
-namespace {
-
-// Queue counters of one query; a ring of 2048 slots, so at most 2048 queries of one context may
-// be in flight at once.
-uint32_t counter_slot(nbvh_ctx* ctx)
+SlotLease::SlotLease(nbvh_ctx* ctx, hipStream_t stream) : ctx_(ctx)
 {
-    constexpr uint32_t slots = nbvh_ctx::kCounters / nbvh_ctx::kCounterWords;
-    return (ctx->counter_next.fetch_add(1) % slots) * nbvh_ctx::kCounterWords;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(stream, &cs), "hipStreamIsCapturing");
+    if (cs == hipStreamCaptureStatusActive) {
+        const uint32_t c = ctx->captured_next.fetch_add(1);
+        if (c >= nbvh_ctx::kCapturedSlots) {
+            ctx->captured_next.store(nbvh_ctx::kCapturedSlots);
+            fail(NBVH_ERR_CAPACITY, "more than 1024 queries recorded into HIP graphs on this context");
+        }
+        k_        = nbvh_ctx::kEagerSlots + c;
+        captured_ = true;
+        return;
+    }
+    if (cs != hipStreamCaptureStatusNone) fail(NBVH_ERR_STATE, "the stream's graph capture was invalidated");
+    k_    = ctx->eager_next.fetch_add(1) % nbvh_ctx::kEagerSlots;
+    lock_ = std::unique_lock<std::mutex>(ctx->slot_mu[k_]);
+    hipEvent_t& ev = ctx->slot_event[k_];
+    if (ev == nullptr)
+        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreateWithFlags");
+    else
+        hip_check(hipEventSynchronize(ev), "hipEventSynchronize");  // the slot's previous query is done
 }
 
-}  // namespace
+uint32_t* SlotLease::counters() const { return ctx_->d_counters + (size_t)k_ * nbvh_ctx::kCounterWords; }
+BatchPart* SlotLease::host_parts() const { return ctx_->h_parts + (size_t)k_ * kMaxBatchParts; }
+BatchPart* SlotLease::device_parts() const { return ctx_->d_parts + (size_t)k_ * kMaxBatchParts; }
+
+void SlotLease::finish(hipStream_t stream)
+{
+    if (!captured_) hip_check(hipEventRecord(ctx_->slot_event[k_], stream), "hipEventRecord");
+}
+
+void ensure_part_tables(nbvh_ctx* ctx)
+{
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->d_parts) return;
+    const size_t bytes = (size_t)nbvh_ctx::kSlots * kMaxBatchParts * sizeof(BatchPart);
+    hip_check(hipMalloc((void**)&ctx->d_parts, bytes), "hipMalloc part tables");
+    if (hipHostMalloc((void**)&ctx->h_parts, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(ctx->d_parts);
+        ctx->d_parts = nullptr;
+        fail(NBVH_ERR_OUT_OF_MEMORY, "hipHostMalloc part tables failed");
+    }
+}
 
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-                       hipStream_t stream);
+                       hipStream_t stream, SlotLease* lease);
 void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                          hipStream_t stream);
 
@@ -212,7 +250,7 @@ NetArgs neural_args(const nbvh_scene* s, const nbvh_query_params* p)
 }
 
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-               hipStream_t stream)
+               hipStream_t stream, SlotLease* lease)
 {
     require(p != nullptr, "query params are NULL");
     require((p->flags & ~(uint32_t)(NBVH_QUERY_SORT_RAYS | NBVH_QUERY_WAVEFRONT)) == 0u, "unknown query flags");
@@ -222,7 +260,7 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
         return;
     }
     if ((p->flags & NBVH_QUERY_SORT_RAYS) == 0u || batch.n < 2 || batch.order != nullptr) {
-        run_query_ordered(ctx, s, p, batch, stream);
+        run_query_ordered(ctx, s, p, batch, stream, lease);
         return;
     }
     // Coherence order: keys + radix sort into transient stream-ordered memory, freed after the launch.
@@ -238,7 +276,7 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
         hip_check(ray_order(batch, bmin, ext, order, mem, scratch, stream), "ray order");
         RayBatch sorted = batch;
         sorted.order    = order;
-        run_query_ordered(ctx, s, p, sorted, stream);
+        run_query_ordered(ctx, s, p, sorted, stream, lease);
     } catch (...) {
         (void)hipFreeAsync(mem, stream);
         throw;
@@ -323,7 +361,7 @@ void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_pa
 }
 
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-                       hipStream_t stream)
+                       hipStream_t stream, SlotLease* lease)
 {
     require(p != nullptr, "query params are NULL");
     const uint32_t n = batch.n;
@@ -348,7 +386,12 @@ void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_para
                         : 0u;
     a.blocks_per_cu = s->shape[p->precision].blocks_per_cu;
     a.queue_bands   = s->shape[p->precision].queue_bands;
-    uint32_t* counter = ctx->d_counters + counter_slot(ctx);
+    std::unique_ptr<SlotLease> own;
+    if (lease == nullptr) {
+        own.reset(new SlotLease(ctx, stream));
+        lease = own.get();
+    }
+    uint32_t* counter = lease->counters();
     hip_check(zero_u32(counter, nbvh_ctx::kCounterWords, stream), "counter reset");
     bool supported = false;
     if (p->precision == NBVH_PREC_FP32)
@@ -357,6 +400,7 @@ void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_para
         hip_check(query_mfma(a, p->precision, batch, counter, (uint32_t)ctx->n_cus, stream, &supported),
                   "mfma query");
     if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no query kernel for this network shape / precision");
+    if (own) own->finish(stream);
 }
 
 namespace {
@@ -437,9 +481,10 @@ NBVH_API int32_t nbvh_create(const nbvh_create_info* info, nbvh_ctx** out)
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         ctx->budget = (info && info->hbm_budget_bytes) ? info->hbm_budget_bytes : (uint64_t)(0.9 * (double)free_b);
         void* p = nullptr;
-        hip_check(hipMalloc(&p, nbvh_ctx::kCounters * sizeof(uint32_t)), "hipMalloc counters");
+        const size_t counter_bytes = (size_t)nbvh_ctx::kSlots * nbvh_ctx::kCounterWords * sizeof(uint32_t);
+        hip_check(hipMalloc(&p, counter_bytes), "hipMalloc counters");
         ctx->d_counters = (uint32_t*)p;
-        hip_check(hipMemset(p, 0, nbvh_ctx::kCounters * sizeof(uint32_t)), "hipMemset counters");
+        hip_check(hipMemset(p, 0, counter_bytes), "hipMemset counters");
         *out = ctx.release();
     });
 }
@@ -449,6 +494,8 @@ NBVH_API int32_t nbvh_destroy(nbvh_ctx* ctx)
     if (!ctx) return NBVH_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    for (hipEvent_t ev : ctx->slot_event)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_parts) (void)hipFree(ctx->d_parts);
     if (ctx->h_parts) (void)hipHostFree(ctx->h_parts);
@@ -786,6 +833,7 @@ NBVH_API int32_t nbvh_scene_load_network(nbvh_scene* s, const nbvh_network_confi
         s->mlp.assign(mlp, mlp + n_mlp);
         upload_network(s);
         s->has_net = true;
+        s->net_generation++;
         refresh_args(s);
     });
 }
@@ -803,6 +851,7 @@ NBVH_API int32_t nbvh_scene_init_network(nbvh_scene* s, const nbvh_network_confi
         s->mlp.swap(m);
         upload_network(s);
         s->has_net = true;
+        s->net_generation++;
         refresh_args(s);
     });
 }
@@ -889,21 +938,10 @@ NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh
         require(s->ctx == ctx, "scene belongs to another context");
         require(n_parts >= 1 && n_parts <= kMaxBatchParts, "n_parts must be in 1..64");
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        {
-            std::lock_guard<std::mutex> g(ctx->mu);
-            if (!ctx->d_parts) {
-                const size_t bytes = (size_t)nbvh_ctx::kPartSlots * kMaxBatchParts * sizeof(BatchPart);
-                hip_check(hipMalloc((void**)&ctx->d_parts, bytes), "hipMalloc part tables");
-                if (hipHostMalloc((void**)&ctx->h_parts, bytes, hipHostMallocDefault) != hipSuccess) {
-                    (void)hipFree(ctx->d_parts);
-                    ctx->d_parts = nullptr;
-                    fail(NBVH_ERR_OUT_OF_MEMORY, "hipHostMalloc part tables failed");
-                }
-            }
-        }
-        const uint32_t k = ctx->parts_next.fetch_add(1) % nbvh_ctx::kPartSlots;
-        BatchPart* h = ctx->h_parts + (size_t)k * kMaxBatchParts;
-        BatchPart* d = ctx->d_parts + (size_t)k * kMaxBatchParts;
+        ensure_part_tables(ctx);
+        SlotLease lease(ctx, (hipStream_t)stream);
+        BatchPart* h = lease.host_parts();
+        BatchPart* d = lease.device_parts();
         uint64_t total = 0;
         for (uint32_t i = 0; i < n_parts; i++) {
             require(parts[i].n_rays == 0 || (parts[i].rays && parts[i].hits), "NULL ray or hit buffer in a part");
@@ -915,7 +953,8 @@ NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh
         if (total == 0) return;
         hip_check(hipMemcpyAsync(d, h, n_parts * sizeof(BatchPart), hipMemcpyHostToDevice, (hipStream_t)stream),
                   "part table H2D");
-        run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, (hipStream_t)stream);
+        run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, (hipStream_t)stream, &lease);
+        lease.finish((hipStream_t)stream);
     });
 }
 

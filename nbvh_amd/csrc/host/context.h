@@ -21,16 +21,22 @@ struct nbvh_ctx {
     std::atomic<uint64_t> used{0};
     mutable std::mutex mu;
     std::string last_error;
-    uint32_t* d_counters = nullptr;  // ring of queue counters for concurrent launches
-    std::atomic<uint32_t> counter_next{0};
-    static constexpr uint32_t kCounterWords = 32;            // per query: 8 band heads, 16 B apart
-    static constexpr uint32_t kCounters     = 2048 * kCounterWords;
-    // Part tables of multi-part queries (nbvh_query_parts): a ring of kPartSlots tables of
-    // kMaxBatchParts entries, device copy + pinned host staging, allocated on first use.
-    static constexpr uint32_t kPartSlots = 2048;
+    // Per-query scratch slots: queue counters (kCounterWords words) and a part table
+    // (kMaxBatchParts entries: device copy + pinned host staging, allocated on first use).
+    // Eager queries take slots [0, kEagerSlots) round robin; a slot is handed out again only
+    // after the query that last used it has completed (slot_event, waited on by the host).
+    // Queries recorded into a HIP graph take a slot of [kEagerSlots, kSlots) each, never reused,
+    // so that every replay of the graph finds its counters and part table as captured.
+    static constexpr uint32_t kCounterWords  = 32;  // per query: 8 band heads, 16 B apart
+    static constexpr uint32_t kEagerSlots    = 2048;
+    static constexpr uint32_t kCapturedSlots = 1024;
+    static constexpr uint32_t kSlots         = kEagerSlots + kCapturedSlots;
+    uint32_t* d_counters = nullptr;  // kSlots x kCounterWords
     nbvh::BatchPart* d_parts = nullptr;
     nbvh::BatchPart* h_parts = nullptr;
-    std::atomic<uint32_t> parts_next{0};
+    std::atomic<uint32_t> eager_next{0}, captured_next{0};
+    hipEvent_t slot_event[kEagerSlots] = {};
+    std::mutex slot_mu[kEagerSlots];
 };
 
 namespace nbvh {
@@ -94,6 +100,7 @@ struct nbvh_scene {
     std::vector<nbvh_node> nbvh_nodes;
     float bbox_min[3] = {0, 0, 0}, bbox_max[3] = {0, 0, 0};
     bool has_bbox = false, has_net = false;
+    uint64_t net_generation = 0;  // bumped whenever the network is replaced (load / init)
     nbvh_network_config cfg{};
     std::vector<float> tables, mlp;
     // device
@@ -110,9 +117,34 @@ namespace nbvh {
 void refresh_args(nbvh_scene* s);
 void upload_network(nbvh_scene* s);
 
-// Queue one query launch (counter reset + kernel) over the rays of `batch` on `stream`.
-// Validates the parameters against the scene; throws Error on failure.
+// One query's scratch slot (nbvh_ctx): held from before its first use on the stream until
+// finish() has recorded, on that stream, the point after its last use.
+class SlotLease {
+public:
+    SlotLease(nbvh_ctx* ctx, hipStream_t stream);
+    SlotLease(const SlotLease&) = delete;
+    SlotLease& operator=(const SlotLease&) = delete;
+    uint32_t* counters() const;
+    BatchPart* host_parts() const;  // requires ensure_part_tables(ctx)
+    BatchPart* device_parts() const;
+    void finish(hipStream_t stream);  // eager slots: record the completion event
+    uint32_t index() const { return k_; }
+    bool captured() const { return captured_; }
+
+private:
+    nbvh_ctx* ctx_;
+    uint32_t k_ = 0;
+    bool captured_ = false;
+    std::unique_lock<std::mutex> lock_;
+};
+
+// Allocates the context's part tables on first use.
+void ensure_part_tables(nbvh_ctx* ctx);
+
+// Queue one query launch (counter reset + kernel) over the rays of `batch` on `stream`, with
+// the scratch of `lease` (one is taken and finished here when NULL). Validates the parameters
+// against the scene; throws Error on failure.
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-               hipStream_t stream);
+               hipStream_t stream, SlotLease* lease = nullptr);
 
 }  // namespace nbvh

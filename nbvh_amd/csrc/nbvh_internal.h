@@ -52,6 +52,8 @@ struct NetArgs {
     const uint16_t* mlp_frag;          // MFMA fragment-ordered half/bf16 weights
     const uint8_t*  mlp_frag_fp8;      // FP8 e4m3 hidden-layer fragments
     float    fp8_wscale[8];            // per-layer weight dequantisation scales (FP8 variant)
+    uint32_t n_dense;                  // levels [n_dense, n_levels) are all hashed (grid resolution
+                                       // grows with the level, so the dense levels are a prefix)
     GridLevel level[kMaxLevels];
 };
 
@@ -113,9 +115,9 @@ struct TrainArgs {
     float*       loss;
 };
 
-// Adam step constants; inv_bias1/2 = 1 / (1 - beta^t).
+// Adam step constants (the bias correction uses each parameter's own update count).
 struct AdamArgs {
-    float lr, beta1, beta2, epsilon, l2_reg, inv_bias1, inv_bias2;
+    float lr, beta1, beta2, epsilon, l2_reg;
 };
 
 // One live path of the path-trace driver (32 B): its pixel, throughput and radiance so far.

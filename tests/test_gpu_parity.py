@@ -300,6 +300,86 @@ def test_graph_replays_match_eager(nb, orc, bench):
     s.destroy()
 
 
+def test_scratch_slots_are_not_shared(nb, orc, bench):
+    """Per-query scratch (queue counters and part tables): a multi-part query recorded into a
+    graph keeps its own slot, so replays stay equal to the eager result after 2,100 further
+    multi-part queries (more than the 2,048 round-robin slots, each with its own part table);
+    those 2,100 queries, queued on 4 streams without waiting, each give the eager result of
+    their own rays although far more than one ring of them is in flight."""
+    from nbvh_amd import hip
+    mod = bench["module"]
+    _, cam = camera_rays(nb, orc, 96, 64)
+    shadow = random_rays(3000, 97, inside_frac=0.4)
+    p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=nb.PREC_FP16)
+    dc, ds = hip.Buffer.from_array(cam), hip.Buffer.from_array(shadow)
+    hc, hs = hip.Buffer(len(cam) * 48), hip.Buffer(len(shadow) * 48)
+    parts = [(dc.ptr, len(cam), hc.ptr, False), (ds.ptr, len(shadow), hs.ptr, True)]
+    s = hip.Stream()
+    mod.query_parts_device(p, parts, stream=s.handle)
+    s.synchronize()
+    ref_c, ref_s = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
+    with hip.Graph.capture(s) as g:
+        mod.query_parts_device(p, parts, stream=s.handle)
+    # 2,100 small two-part queries: rays [8i, 8i + 64) of the camera rays as closest hits and of the
+    # shadow rays as any hits, each into its own hit arrays.
+    n_q, m = 2100, 64
+    out_c, out_s = hip.Buffer(n_q * m * 48), hip.Buffer(n_q * m * 48)
+    streams = [hip.Stream() for _ in range(4)]
+    for i in range(n_q):
+        o = 8 * (i % 700)
+        mod.query_parts_device(p, [(dc.ptr + 32 * o, m, out_c.ptr + 48 * m * i, False),
+                                   (ds.ptr + 32 * o, m, out_s.ptr + 48 * m * i, True)],
+                               stream=streams[i % 4].handle)
+    hip.synchronize()
+    got_c = out_c.download(nb.HIT_DTYPE).reshape(n_q, m)
+    got_s = out_s.download(nb.HIT_DTYPE).reshape(n_q, m)
+    for i in range(n_q):
+        o = 8 * (i % 700)
+        assert hits_equal(got_c[i], ref_c[o:o + m]) and hits_equal(got_s[i], ref_s[o:o + m]), i
+    for rep in range(3):
+        hc.zero()
+        hs.zero()
+        g.launch(s)
+        s.synchronize()
+        assert hits_equal(hc.download(nb.HIT_DTYPE), ref_c), rep
+        assert hits_equal(hs.download(nb.HIT_DTYPE), ref_s), rep
+    g.destroy()
+    for st in streams:
+        st.destroy()
+    s.destroy()
+
+
+def test_captured_query_slots_are_bounded(nb, orc):
+    """A context hands at most 1,024 scratch slots to queries recorded into graphs (they are never
+    reused); the next capture fails with NBVH_ERR_CAPACITY before it enqueues anything, and eager
+    queries keep working."""
+    from nbvh_amd import hip
+    ctx = nb.Context()
+    v, t = nb.icosphere(2, seed=7, amplitude=0.1)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(16)
+    scene.init_network(nb.network_config(log2_hashmap_size=14), seed=1, table_range=1e-4)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    rays = random_rays(256, 5, inside_frac=0.2)
+    dr, dh = hip.Buffer.from_array(rays), hip.Buffer(len(rays) * 48)
+    p = mod.params(precision=nb.PREC_FP16)
+    s = hip.Stream()
+    with hip.Graph.capture(s) as g:
+        for _ in range(1024):
+            mod.query_device(p, dr.ptr, len(rays), dh.ptr, stream=s.handle)
+        with pytest.raises(nb.NbvhError) as e:
+            mod.query_device(p, dr.ptr, len(rays), dh.ptr, stream=s.handle)
+        assert e.value.status == nb._ffi.ERR_CAPACITY
+    g.launch(s)
+    s.synchronize()
+    exp = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    assert hits_equal(dh.download(nb.HIT_DTYPE), exp)
+    g.destroy()
+    s.destroy()
+    scene.close()
+    ctx.close()
+
+
 @pytest.mark.parametrize("kind", ["neural", "triangle"])
 def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
     """The bounce kernel against its restatement: same inactive rays, origins within 2e-6 and
