@@ -4,7 +4,10 @@ sample of it is traced again by the oracle (the oracle finishes a sample in seco
   - the 1024x1024 headline frame: FP32 query bit-exact on every 61st ray, FP16 hit/miss and depth
     against FP64 on the same sample;
   - the 64-sphere scene (2.17M triangles, N-BVH 4096): triangle closest and any hit bit-exact and
-    the FP32 neural query bit-exact on samples of 1M incoherent rays;
+    the FP32 neural query bit-exact on samples of 1M incoherent rays; on the whole 4M-ray batch
+    (2M incoherent closest-hit + 2M shadow any-hit rays in one launch), the FP16 hit/miss against
+    the FP64 oracle on samples, and the autotuned launch shape and a graph replay equal to the
+    eager launch;
   - size-independent properties of the full batches: every ray gets a record, kinds are valid,
     any-hit occlusion implies a closest hit before tmax.
 """
@@ -78,4 +81,61 @@ def test_sphere_field_full_batches(nb, orc, ctx):
     small = np.arange(0, 1 << 20, 499)
     n32 = mod.neural_bvh_traversal(rc, precision=nb.PREC_FP32, sort_rays=True)
     assert hits_equal(n32[small], net.neural_query(scene.nbvh(), rc[small], mode=orc.FP32))
+    scene.close()
+
+
+def test_sphere_field_fp16_mixed_launch(nb, orc, ctx):
+    """The 4M-ray batch of the 64-sphere workload as one two-part FP16 launch: closest-hit and
+    shadow any-hit kinds within 0.5 % of the FP64 oracle on samples; after the autotuner picks a
+    launch shape, and replayed from a HIP graph (with eager work between replays), the records
+    are bit-identical to the first eager launch."""
+    import bench
+    from nbvh_amd import hip
+    v, t = nb.sphere_field(64, seed=5)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(4096)
+    scene.init_network(nb.network_config(), seed=2, table_range=1e-4)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    n = 1 << 21
+    rc, rs = bench.wavefront_rays(nb, v, t, n, n)
+    dc, ds = hip.Buffer.from_array(rc), hip.Buffer.from_array(rs)
+    hc, hs = hip.Buffer(n * 48), hip.Buffer(n * 48)
+    p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=nb.PREC_FP16)
+    parts = [(dc.ptr, n, hc.ptr, False), (ds.ptr, n, hs.ptr, True)]
+    s = hip.Stream()
+    mod.query_parts_device(p, parts, stream=s.handle)
+    s.synchronize()
+    ref_c, ref_s = hc.download(nb.HIT_DTYPE), hs.download(nb.HIT_DTYPE)
+    print("[sphere field] eager 4M-ray FP16 launch done", flush=True)
+    cfg, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, bmin, bmax)
+    sample = np.arange(0, n, 1021)
+    e_c = net.neural_query(scene.nbvh(), rc[sample], mode=orc.FP64)
+    e_s = net.neural_query(scene.nbvh(), rs[sample], mode=orc.FP64, any_hit=True)
+    kc, ks = nb.hit_kind(ref_c[sample]), nb.hit_kind(ref_s[sample])
+    assert 0.05 < (nb.hit_kind(e_c) != 0).mean() < 0.95 and 0.05 < (nb.hit_kind(e_s) != 0).mean() < 0.95
+    assert (kc != nb.hit_kind(e_c)).mean() <= 0.005 and (ks != nb.hit_kind(e_s)).mean() <= 0.005
+    both = (kc == 1) & (nb.hit_kind(e_c) == 1) & (ref_c[sample]["node_id"] == e_c["node_id"])
+    rel = np.abs(ref_c[sample]["t"][both] - e_c["t"][both]) / np.maximum(e_c["t"][both], 1e-3)
+    assert both.sum() > 100 and np.percentile(rel, 99) < 1e-2
+    # The autotuner times launch shapes on the closest-hit rays and keeps the fastest for the scene.
+    shape, ms = mod.autotune(p, dc.ptr, n, hc.ptr, stream=s.handle)
+    assert ms > 0
+    hc.zero()
+    hs.zero()
+    mod.query_parts_device(p, parts, stream=s.handle)
+    s.synchronize()
+    assert hits_equal(hc.download(nb.HIT_DTYPE), ref_c) and hits_equal(hs.download(nb.HIT_DTYPE), ref_s), shape
+    with hip.Graph.capture(s) as g:
+        mod.query_parts_device(p, parts, stream=s.handle)
+    for rep in range(3):
+        hc.zero()
+        hs.zero()
+        g.launch(s)
+        s.synchronize()
+        assert hits_equal(hc.download(nb.HIT_DTYPE), ref_c) and hits_equal(hs.download(nb.HIT_DTYPE), ref_s), rep
+        mod.neural_bvh_traversal(rs[:4096], kind=nb.QUERY_NEURAL_ANY, precision=nb.PREC_FP32)
+    g.destroy()
+    s.destroy()
     scene.close()
