@@ -105,70 +105,109 @@ __device__ __forceinline__ void load_corner(__amdgpu_buffer_rsrc_t tab, uint32_t
 // hardware range check returns 0 for any offset past the table instead of faulting. Levels are
 // processed in groups of G: all corner offsets of the group, then all its loads, then the
 // corner weights and the FP32 accumulation over corners 0..7 (tcnn's trilinear weights).
-template <int P, int L, int F, bool BF16, bool EXACT>
+// Byte offsets of the 8 corners of one level at point (x, y, z), and the fractional position.
+// HASHED: the level is hashed (a power-of-two size), so only the coherent prime hash is formed:
+// two multiplies per level, then one xor-and and one add-shift per corner. The hash of any
+// position is in range, so this form is exact for points outside the unit cube too. Otherwise
+// the general form (dense or hashed at run time, grid_corners) with its EXACT modulo switch.
+template <int F, bool HASHED, bool EXACT>
+__device__ __forceinline__ void level_offsets(const GridLevel& lv, float x, float y, float z, uint32_t off[8],
+                                              float fr[3])
+{
+    uint32_t g[3];
+    grid_cell(lv, x, y, z, g, fr);
+    constexpr uint32_t kShift = F == 2 ? 2u : 3u;  // log2 of the entry bytes (F halves)
+    if constexpr (HASHED) {
+        const uint32_t hy0 = g[1] * 2654435761u, hy1 = hy0 + 2654435761u;
+        const uint32_t hz0 = g[2] * 805459861u, hz1 = hz0 + 805459861u;
+        const uint32_t yz[4] = {hy0 ^ hz0, hy1 ^ hz0, hy0 ^ hz1, hy1 ^ hz1};
+        const uint32_t mask = lv.size - 1u, x1 = g[0] + 1u;
+#pragma unroll
+        for (int c = 0; c < 8; c++) off[c] = (((((c & 1) ? x1 : g[0]) ^ yz[c >> 1]) & mask) + lv.offset) << kShift;
+    } else {
+        uint32_t idx[8];
+        grid_corners<EXACT>(lv, g, idx);
+#pragma unroll
+        for (int c = 0; c < 8; c++) off[c] = (idx[c] + lv.offset) << kShift;
+    }
+}
+
+// One group of G levels of point p: all 8G corner offsets, then all 8G loads, then the corner
+// weights and the FP32 accumulation over corners 0..7 (tcnn's trilinear weights); the features
+// go to the layer-1 operand halves lo / hi as 16-bit values.
+template <int P, int L, int F, int G, bool BF16, bool HASHED, bool EXACT>
+__device__ __forceinline__ void gather_group(const NetArgs& a, __amdgpu_buffer_rsrc_t tab, const float* pt, int p,
+                                             int l0, typename Frag<BF16>::vec* lo, typename Frag<BF16>::vec* hi)
+{
+    constexpr int HALF = P * L * F / 2;
+    typedef typename Frag<BF16>::elem E;
+    uint32_t off[G][8];
+    float fr[G][3];
+#pragma unroll
+    for (int i = 0; i < G; i++) level_offsets<F, HASHED, EXACT>(a.level[l0 + i], pt[0], pt[1], pt[2], off[i], fr[i]);
+    // One load per corner. (Measured: serving x-neighbour corners with one 16-byte load plus a
+    // range-masked second load halves the throughput, so corners stay separate.)
+    uint32_t q[G][8][F / 2];
+#pragma unroll
+    for (int i = 0; i < G; i++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            if (l0 + i >= NBVH_GATHER_POLICY_FROM)
+                load_corner<F, NBVH_GATHER_POLICY>(tab, off[i][c], q[i][c]);
+            else
+                load_corner<F, 0>(tab, off[i][c], q[i][c]);
+        }
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        // tcnn's corner weights ((1 * w_x) * w_y) * w_z, the x-y products shared by the z pair.
+        const float wx[2] = {1.f - fr[i][0], fr[i][0]}, wy[2] = {1.f - fr[i][1], fr[i][1]};
+        const float wz[2] = {1.f - fr[i][2], fr[i][2]};
+        float acc[F];
+#pragma unroll
+        for (int k = 0; k < F; k++) acc[k] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float w = (wx[c & 1] * wy[(c >> 1) & 1]) * wz[c >> 2];
+#pragma unroll
+            for (int k = 0; k < F; k++) {
+                const uint32_t bits = (k & 1) ? (q[i][c][k / 2] >> 16) : (q[i][c][k / 2] & 0xFFFFu);
+                acc[k] = fmaf(w, half_bits_to_float(bits), acc[k]);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+            const int k = p * L * F + (l0 + i) * F + f;
+            if (k < HALF)
+                lo[k / 8][k % 8] = (E)acc[f];
+            else
+                hi[(k - HALF) / 8][(k - HALF) % 8] = (E)acc[f];
+        }
+    }
+}
+
+// ND >= 0: the kernel is specialised for grids whose levels [ND, L) are hashed (n_dense == ND,
+// checked at launch); groups of those levels take the hash-only form, the others the general
+// form. ND < 0: every level takes the general form (any grid). EXACT: the full modulo for
+// points outside the unit cube (dense levels only; the hash is exact everywhere).
+template <int P, int L, int F, bool BF16, bool EXACT, int ND>
 __device__ __forceinline__ void gather_operands(const NetArgs& a, __amdgpu_buffer_rsrc_t tab, const float* pts,
                                                 typename Frag<BF16>::vec* lo, typename Frag<BF16>::vec* hi)
 {
-    constexpr int HALF = P * L * F / 2;
     constexpr int G = (L % NBVH_GATHER_GROUP == 0) ? NBVH_GATHER_GROUP : 1;
-    typedef typename Frag<BF16>::elem E;
     static_assert(F == 2 || F == 4, "FP16 gather implemented for F = 2 and F = 4");
 #pragma unroll
     for (int p = 0; p < P; p++) {
 #pragma unroll
         for (int l0 = 0; l0 < L; l0 += G) {
-            uint32_t off[G][8];
-            float fr[G][3];
-#pragma unroll
-            for (int i = 0; i < G; i++) {
-                const GridLevel& lv = a.level[l0 + i];
-                uint32_t g[3], idx[8];
-                grid_cell(lv, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], g, fr[i]);
-                grid_corners<EXACT>(lv, g, idx);
-                const uint32_t base = lv.offset * (uint32_t)(2 * F);
-#pragma unroll
-                for (int c = 0; c < 8; c++) off[i][c] = base + idx[c] * (uint32_t)(2 * F);
-            }
-            // One load per corner. (Measured: serving x-neighbour corners with one 16-byte load
-            // plus a range-masked second load halves the throughput, so corners stay separate.)
-            uint32_t q[G][8][F / 2];
-#pragma unroll
-            for (int i = 0; i < G; i++)
-#pragma unroll
-                for (int c = 0; c < 8; c++) {
-                    if (l0 + i >= NBVH_GATHER_POLICY_FROM)
-                        load_corner<F, NBVH_GATHER_POLICY>(tab, off[i][c], q[i][c]);
-                    else
-                        load_corner<F, 0>(tab, off[i][c], q[i][c]);
-                }
-#pragma unroll
-            for (int i = 0; i < G; i++) {
-                float acc[F];
-#pragma unroll
-                for (int k = 0; k < F; k++) acc[k] = 0.f;
-#pragma unroll
-                for (int c = 0; c < 8; c++) {
-                    const float w = corner_weight((uint32_t)c, fr[i]);
-#pragma unroll
-                    for (int k = 0; k < F; k++) {
-                        const uint32_t bits = (k & 1) ? (q[i][c][k / 2] >> 16) : (q[i][c][k / 2] & 0xFFFFu);
-                        acc[k] = fmaf(w, half_bits_to_float(bits), acc[k]);
-                    }
-                }
-#pragma unroll
-                for (int f = 0; f < F; f++) {
-                    const int k = p * L * F + (l0 + i) * F + f;
-                    if (k < HALF)
-                        lo[k / 8][k % 8] = (E)acc[f];
-                    else
-                        hi[(k - HALF) / 8][(k - HALF) % 8] = (E)acc[f];
-                }
-            }
+            if (ND >= 0 && l0 >= ND)  // folded at compile time after unrolling
+                gather_group<P, L, F, G, BF16, true, false>(a, tab, pts + 3 * p, p, l0, lo, hi);
+            else
+                gather_group<P, L, F, G, BF16, false, EXACT>(a, tab, pts + 3 * p, p, l0, lo, hi);
         }
     }
 }
 
-template <int P, int L, int F, bool BF16>
+template <int P, int L, int F, bool BF16, int ND>
 __device__ __forceinline__ void layer1_operands(const NetArgs& a, const float* pts,
                                                 typename Frag<BF16>::vec lo[P * L * F / 16],
                                                 typename Frag<BF16>::vec hi[P * L * F / 16])
@@ -177,9 +216,9 @@ __device__ __forceinline__ void layer1_operands(const NetArgs& a, const float* p
     const __amdgpu_buffer_rsrc_t tab =
         __builtin_amdgcn_make_buffer_rsrc((void*)a.tables_f16, (short)0, (int)a.tables_f16_bytes, 0x00020000);
     if (__all(in_unit_cube<3 * P>(pts)))
-        gather_operands<P, L, F, BF16, false>(a, tab, pts, lo, hi);
+        gather_operands<P, L, F, BF16, false, ND>(a, tab, pts, lo, hi);
     else
-        gather_operands<P, L, F, BF16, true>(a, tab, pts, lo, hi);
+        gather_operands<P, L, F, BF16, true, ND>(a, tab, pts, lo, hi);
 #pragma unroll
     for (int s = 0; s < K0 / 16; s++) swap_halves(lo[s], hi[s]);
 }
@@ -219,7 +258,7 @@ __device__ __forceinline__ void acc_to_b(const floatx16 acc[2][2], typename Frag
 
 // The whole network for the 64 segments of a wave. own[3P]: this lane's segment points.
 // Returns this lane's 8 logits (descaled). wfrag: LDS fragments (16 B per lane each).
-template <int P, int L, int F, bool BF16>
+template <int P, int L, int F, bool BF16, int ND>
 __device__ __forceinline__ void mfma_network(const NetArgs& a, const uint4* __restrict__ wfrag, const float* own,
                                              float z[8])
 {
@@ -230,7 +269,7 @@ __device__ __forceinline__ void mfma_network(const NetArgs& a, const uint4* __re
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t g    = lane >> 5;
     V b1[2][S::KS1];
-    layer1_operands<P, L, F, BF16>(a, own, b1[0], b1[1]);
+    layer1_operands<P, L, F, BF16, ND>(a, own, b1[0], b1[1]);
     floatx16 acc[2][2];
 #pragma unroll
     for (int rb = 0; rb < 2; rb++)
@@ -366,7 +405,7 @@ __device__ __forceinline__ void stage_weights_fp8(const NetArgs& a, Fp8Lds<K0>& 
     __syncthreads();
 }
 
-template <int P, int L, int F>
+template <int P, int L, int F, int ND>
 __device__ __forceinline__ void mfma_network_fp8(const NetArgs& a, const Fp8Lds<P * L * F>& w, const float* own,
                                                  float z[8])
 {
@@ -378,7 +417,7 @@ __device__ __forceinline__ void mfma_network_fp8(const NetArgs& a, const Fp8Lds<
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t g    = lane >> 5;
     V b1[2][S::KS1];
-    layer1_operands<P, L, F, false>(a, own, b1[0], b1[1]);
+    layer1_operands<P, L, F, false, ND>(a, own, b1[0], b1[1]);
     floatx16 acc[2][2];
 #pragma unroll
     for (int rb = 0; rb < 2; rb++)
@@ -481,7 +520,7 @@ constexpr int kWavesPerBlock = NBVH_MFMA_BLOCK_WAVES;
 #define NBVH_MFMA_WAVES_PER_EU 2
 #endif
 
-template <int P, int L, int F, bool BF16, bool PARTS>
+template <int P, int L, int F, bool BF16, bool PARTS, int ND>
 __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void neural_query_mfma_kernel(NetArgs a,
                                                                                 RayBatch b,
                                                                                 uint32_t* __restrict__ queue)
@@ -508,12 +547,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
             for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;
         }
         float z[8];
-        mfma_network<P, L, F, BF16>(a, wfrag, own, z);
+        mfma_network<P, L, F, BF16, ND>(a, wfrag, own, z);
         if (s.state == SLOT_SEG) consume_segment<PARTS>(a, s, z, b);
     }
 }
 
-template <int P, int L, int F, bool BF16>
+template <int P, int L, int F, bool BF16, int ND>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetArgs a, const float* __restrict__ in,
                                                                            uint32_t n, float* __restrict__ out,
                                                                            float* __restrict__ logits)
@@ -529,7 +568,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetAr
 #pragma unroll
         for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
         float z[8];
-        mfma_network<P, L, F, BF16>(a, wfrag, own, z);
+        mfma_network<P, L, F, BF16, ND>(a, wfrag, own, z);
         if (i < n) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -540,7 +579,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_mfma_kernel(NetAr
     }
 }
 
-template <int P, int L, int F, bool PARTS>
+template <int P, int L, int F, bool PARTS, int ND>
 __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void neural_query_fp8_kernel(NetArgs a,
                                                                                RayBatch b,
                                                                                uint32_t* __restrict__ queue)
@@ -566,12 +605,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
             for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;
         }
         float z[8];
-        mfma_network_fp8<P, L, F>(a, w, own, z);
+        mfma_network_fp8<P, L, F, ND>(a, w, own, z);
         if (s.state == SLOT_SEG) consume_segment<PARTS>(a, s, z, b);
     }
 }
 
-template <int P, int L, int F>
+template <int P, int L, int F, int ND>
 __global__ __launch_bounds__(64 * kWavesPerBlock) void network_fp8_kernel(NetArgs a, const float* __restrict__ in,
                                                                           uint32_t n, float* __restrict__ out,
                                                                           float* __restrict__ logits)
@@ -584,7 +623,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void network_fp8_kernel(NetArg
 #pragma unroll
         for (int c = 0; c < 3 * P; c++) own[c] = i < n ? in[(size_t)i * (3 * P) + c] : 0.5f;
         float z[8];
-        mfma_network_fp8<P, L, F>(a, w, own, z);
+        mfma_network_fp8<P, L, F, ND>(a, w, own, z);
         if (i < n) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
@@ -610,10 +649,15 @@ uint32_t blocks_per_cu(K kernel, uint32_t threads)
     return (uint32_t)b;
 }
 
-template <int P, int L, int F, bool BF16, bool PARTS>
+// Dense-prefix length the kernels are specialised for (the hash-only gather of levels >= it):
+// 4 covers the 1024x1024 workload's grid (16 levels, base 16, scale 1.5, 2^19) and the chess
+// configuration (8 levels, base 8, scale 2, 2^18). Other grids take the general gather (ND = -1).
+constexpr int kDensePrefix = 4;
+
+template <int P, int L, int F, bool BF16, bool PARTS, int ND>
 hipError_t launch_query_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
 {
-    auto k = dev::neural_query_mfma_kernel<P, L, F, BF16, PARTS>;
+    auto k = dev::neural_query_mfma_kernel<P, L, F, BF16, PARTS, ND>;
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
     uint32_t blocks = (rb.n + per_block - 1) / per_block;
@@ -623,31 +667,47 @@ hipError_t launch_query_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue,
     return hipGetLastError();
 }
 
+template <int P, int L, int F, bool BF16, int ND>
+hipError_t launch_query_nd(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
+{
+    return rb.parts ? launch_query_k<P, L, F, BF16, true, ND>(a, rb, queue, n_cus, stream)
+                    : launch_query_k<P, L, F, BF16, false, ND>(a, rb, queue, n_cus, stream);
+}
+
 template <int P, int L, int F, bool BF16>
 hipError_t launch_query(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
 {
-    return rb.parts ? launch_query_k<P, L, F, BF16, true>(a, rb, queue, n_cus, stream)
-                    : launch_query_k<P, L, F, BF16, false>(a, rb, queue, n_cus, stream);
+    return a.n_dense == (uint32_t)kDensePrefix ? launch_query_nd<P, L, F, BF16, kDensePrefix>(a, rb, queue, n_cus, stream)
+                                               : launch_query_nd<P, L, F, BF16, -1>(a, rb, queue, n_cus, stream);
+}
+
+template <int P, int L, int F, bool BF16, int ND>
+hipError_t launch_network_nd(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
+                             hipStream_t stream)
+{
+    auto k = dev::network_mfma_kernel<P, L, F, BF16, ND>;
+    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
+    const uint32_t per_block = 64 * dev::kWavesPerBlock;
+    uint32_t blocks = (n + per_block - 1) / per_block;
+    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, in, n, out, logits);
+    return hipGetLastError();
 }
 
 template <int P, int L, int F, bool BF16>
 hipError_t launch_network(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
                           hipStream_t stream)
 {
-    auto k = dev::network_mfma_kernel<P, L, F, BF16>;
-    static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
-    const uint32_t per_block = 64 * dev::kWavesPerBlock;
-    uint32_t blocks = (n + per_block - 1) / per_block;
-    blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, in, n, out, logits);
-    return hipGetLastError();
+    return a.n_dense == (uint32_t)kDensePrefix
+               ? launch_network_nd<P, L, F, BF16, kDensePrefix>(a, in, n, out, logits, n_cus, stream)
+               : launch_network_nd<P, L, F, BF16, -1>(a, in, n, out, logits, n_cus, stream);
 }
 
-template <int P, int L, int F, bool PARTS>
+template <int P, int L, int F, bool PARTS, int ND>
 hipError_t launch_query_fp8_k(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
                               hipStream_t stream)
 {
-    auto k = dev::neural_query_fp8_kernel<P, L, F, PARTS>;
+    auto k = dev::neural_query_fp8_kernel<P, L, F, PARTS, ND>;
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
     uint32_t blocks = (rb.n + per_block - 1) / per_block;
@@ -657,24 +717,41 @@ hipError_t launch_query_fp8_k(const NetArgs& a, const RayBatch& rb, uint32_t* qu
     return hipGetLastError();
 }
 
-template <int P, int L, int F>
-hipError_t launch_query_fp8(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
+template <int P, int L, int F, int ND>
+hipError_t launch_query_fp8_nd(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus,
+                               hipStream_t stream)
 {
-    return rb.parts ? launch_query_fp8_k<P, L, F, true>(a, rb, queue, n_cus, stream)
-                    : launch_query_fp8_k<P, L, F, false>(a, rb, queue, n_cus, stream);
+    return rb.parts ? launch_query_fp8_k<P, L, F, true, ND>(a, rb, queue, n_cus, stream)
+                    : launch_query_fp8_k<P, L, F, false, ND>(a, rb, queue, n_cus, stream);
 }
 
 template <int P, int L, int F>
-hipError_t launch_network_fp8(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
-                              hipStream_t stream)
+hipError_t launch_query_fp8(const NetArgs& a, const RayBatch& rb, uint32_t* queue, uint32_t n_cus, hipStream_t stream)
 {
-    auto k = dev::network_fp8_kernel<P, L, F>;
+    return a.n_dense == (uint32_t)kDensePrefix ? launch_query_fp8_nd<P, L, F, kDensePrefix>(a, rb, queue, n_cus, stream)
+                                               : launch_query_fp8_nd<P, L, F, -1>(a, rb, queue, n_cus, stream);
+}
+
+template <int P, int L, int F, int ND>
+hipError_t launch_network_fp8_nd(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits,
+                                 uint32_t n_cus, hipStream_t stream)
+{
+    auto k = dev::network_fp8_kernel<P, L, F, ND>;
     static const uint32_t bpc = blocks_per_cu(k, 64 * dev::kWavesPerBlock);
     const uint32_t per_block = 64 * dev::kWavesPerBlock;
     uint32_t blocks = (n + per_block - 1) / per_block;
     blocks = blocks < n_cus * bpc ? blocks : n_cus * bpc;
     hipLaunchKernelGGL(k, dim3(blocks), dim3(per_block), 0, stream, a, in, n, out, logits);
     return hipGetLastError();
+}
+
+template <int P, int L, int F>
+hipError_t launch_network_fp8(const NetArgs& a, const float* in, uint32_t n, float* out, float* logits, uint32_t n_cus,
+                              hipStream_t stream)
+{
+    return a.n_dense == (uint32_t)kDensePrefix
+               ? launch_network_fp8_nd<P, L, F, kDensePrefix>(a, in, n, out, logits, n_cus, stream)
+               : launch_network_fp8_nd<P, L, F, -1>(a, in, n, out, logits, n_cus, stream);
 }
 
 }  // namespace
