@@ -309,7 +309,10 @@ def test_scratch_slots_are_not_shared(nb, orc, bench):
     from nbvh_amd import hip
     mod = bench["module"]
     _, cam = camera_rays(nb, orc, 96, 64)
-    shadow = random_rays(3000, 97, inside_frac=0.4)
+    shadow = random_rays(len(cam), 97, inside_frac=0.4)
+    n_q, m = 2100, 64
+    # every window [8 (i % 700), 8 (i % 700) + m) of the small queries lies inside both ray arrays
+    assert 8 * 699 + m <= min(len(cam), len(shadow))
     p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=nb.PREC_FP16)
     dc, ds = hip.Buffer.from_array(cam), hip.Buffer.from_array(shadow)
     hc, hs = hip.Buffer(len(cam) * 48), hip.Buffer(len(shadow) * 48)
@@ -322,7 +325,6 @@ def test_scratch_slots_are_not_shared(nb, orc, bench):
         mod.query_parts_device(p, parts, stream=s.handle)
     # 2,100 small two-part queries: rays [8i, 8i + 64) of the camera rays as closest hits and of the
     # shadow rays as any hits, each into its own hit arrays.
-    n_q, m = 2100, 64
     out_c, out_s = hip.Buffer(n_q * m * 48), hip.Buffer(n_q * m * 48)
     streams = [hip.Stream() for _ in range(4)]
     for i in range(n_q):
