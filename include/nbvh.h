@@ -491,6 +491,117 @@ int32_t nbvh_trainer_get(nbvh_trainer* trainer, uint32_t what, float* out, uint6
 /* Write the trained network into the scene. Synchronous; no query may run on the scene meanwhile. */
 int32_t nbvh_trainer_commit(nbvh_trainer* trainer);
 
+/* ---------------------------------------------------------------------------------------------
+ * Tree-cut learning: the reference's training loop (src/cuda/neural/neural_bvh_renderer.cu:
+ * 141-297) around the trainer. Each step samples a batch of training segments from the N-BVH
+ * being learned (granular_sample_bvh_network_input_data, src/cuda/neural/
+ * neural_bvh_module_kernels.cu:119-264), runs one training step on them, adds every sample's loss
+ * to its sampled node (update_neural_tree_cut_loss, src/cuda/neural/neural_bvh_renderer_kernels.cu:
+ * 202-230), picks the leaves of largest loss heuristic (update_neural_tree_cut_decisions, :232-288)
+ * and splits as many of them as the split schedule asks (BvhSplitScheduler, include/cuda/neural/
+ * utils/bvh_split_scheduler.h:13-144; split_neural_bvh_at, src/cuda/neural/
+ * neural_bvh_renderer_accel.cpp:256-392), each split adding the two children of the node in the
+ * scene's triangle BVH; LoD assignment follows the schedule when more than one LoD is learned
+ * (assign_lod_to_current_neural_leaf_nodes, :395-440). The learner owns the N-BVH: creating it
+ * replaces the scene's N-BVH by the cut of its triangle BVH at neural_bvh_min_depth
+ * (build_and_upload_neural_bvh, :505-558), and every split is written back to the scene.
+ *
+ * Sampler semantics kept from the reference: one thread per 2^log2_bvh_max_traversals samples;
+ * its LCG is seeded per sample by get_lcg_rng(step * batch_size, 0, sample); a ray from a uniform
+ * point of the scene box scaled by sampling_bbox_scaling in a uniform direction is traversed to
+ * its next leaf at a LoD drawn among those learned, with boxes dilated by fixed_node_dilation +
+ * U * node_extent_dilation and the learning epsilons; a segment without a triangle continues that
+ * ray with the thread's next sample; with training_node_survival a sample is kept with
+ * probability max(min_rr_survival_prob, heuristic / previous max heuristic) (1 above it), and a
+ * dropped sample leaves its slot's inputs and targets as they were (the slot's node index is still
+ * the newly sampled one, as in the reference). Not supported (NBVH_ERR_UNSUPPORTED): training
+ * bounces (max_training_bounces > 1), view-dependent and leaf-boundary sampling, all off in the
+ * reference's shipped configurations. Node losses are summed per step in FP64 before they are
+ * added to the FP32 node loss (the reference adds FP32 atomics in an unspecified order); among
+ * leaves of equal heuristic the lowest node index is split (the reference keeps one of them, the
+ * last writer of a race).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct nbvh_learner nbvh_learner;
+
+typedef struct nbvh_learner_config {
+    uint32_t batch_size;               /* samples per step (m_batch_size), 1..2^24; 2^18 in the
+                                          shipped configurations */
+    uint32_t log2_bvh_max_traversals;  /* samples per sampler thread = 2^this (0) */
+    float    sampling_bbox_scaling;    /* 1.5 */
+    float    fixed_node_dilation;      /* 1e-5 */
+    float    node_extent_dilation;     /* 1e-4 */
+    float    learning_mint_epsilon;    /* 1e-3 */
+    float    learning_maxt_epsilon;    /* 1e-3 */
+    uint32_t max_training_bounces;     /* 0 or 1 (1) */
+    uint32_t training_node_survival;   /* 1 */
+    float    survival_probability_scale; /* 1 */
+    float    min_rr_survival_prob;     /* 0.005 */
+    uint32_t tree_cut_learning;        /* 1: split leaves on the schedule */
+    uint32_t neural_bvh_min_depth;     /* depth of the initial cut (0: the root alone) */
+    uint32_t num_lods_learned;         /* 1..7 (1) */
+    uint32_t max_training_steps;       /* steps planned (the schedule's split statistics) */
+    uint32_t lod_decision;             /* 0 EqualIterations, 1 EqualSplits */
+    int32_t  splitting_interval;       /* 50 */
+    int32_t  last_split_iteration;     /* 2000 */
+    int32_t  start_split_num;          /* 1 */
+    float    split_scaling;            /* 2 */
+    float    split_interval_scaling;   /* 2 */
+    uint32_t reserved[4];
+} nbvh_learner_config;
+
+typedef struct nbvh_learner_state {
+    uint32_t step;                /* steps done */
+    uint32_t n_nodes;             /* N-BVH nodes */
+    uint32_t current_learned_lod;
+    uint32_t all_lods_assigned;
+    uint32_t tree_cut_learning;   /* still splitting */
+    uint32_t current_num_splits;  /* split candidates picked per step */
+    uint32_t total_splits;        /* nodes split so far */
+    uint32_t sample_failures;     /* sampler threads that found no leaf within 65536 rays */
+    float    max_node_loss;       /* largest leaf heuristic after the last step */
+    float    last_loss;           /* the last step's training loss */
+    uint32_t expected_total_splits; /* the schedule's total over max_training_steps */
+    uint32_t reserved[5];
+} nbvh_learner_state;
+
+/* Tree-cut data of one node (NeuralTreeCutData, include/cuda/neural/neural_bvh_renderer.h:36-47). */
+typedef struct nbvh_tree_cut_node {
+    float    loss;
+    uint32_t iter_alive;
+    uint32_t depth;
+    uint32_t samples;
+} nbvh_tree_cut_node;
+
+enum nbvh_learner_array {
+    NBVH_LEARN_INPUTS    = 0, /* float, batch x 9: the last sampled batch's encoded segments */
+    NBVH_LEARN_TARGETS   = 1, /* float, batch x 8 */
+    NBVH_LEARN_NODES     = 2, /* uint32, batch: each slot's sampled node */
+    NBVH_LEARN_TREE_CUT  = 3, /* nbvh_tree_cut_node, one per N-BVH node */
+    NBVH_LEARN_SPLITS    = 4, /* uint32, current_num_splits: the last step's split candidates
+                                 (0 where fewer leaves had a positive heuristic) */
+    NBVH_LEARN_NBVH      = 5  /* nbvh_node, the N-BVH as learned (boxes not dilated) */
+};
+
+/* Fills cfg with the reference's defaults (include/cuda/neural/neural_bvh_renderer_constants.cuh,
+ * bvh_split_scheduler.h) and batch_size 2^18, max_training_steps 10000. */
+int32_t nbvh_learner_config_defaults(nbvh_learner_config* cfg);
+/* The scene needs its triangles (the targets and the splits come from its triangle BVH). */
+int32_t nbvh_learner_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_learner_config* cfg, nbvh_learner** out);
+int32_t nbvh_learner_destroy(nbvh_learner* learner);
+/* Sample the batch of training step `step` with the learner's current N-BVH, tree-cut data and
+ * max heuristic into its buffers (nbvh_learner_get); stream-ordered. */
+int32_t nbvh_learner_sample(nbvh_learner* learner, uint32_t step, void* stream);
+/* The tree-cut part of a step on the last sampled batch, given each slot's loss (d_sample_loss,
+ * batch floats, device): node losses, leaf ageing and heuristics, split candidates, splits and LoD
+ * assignment on the schedule; then the learner's step counter advances. Synchronises `stream`. */
+int32_t nbvh_learner_update(nbvh_learner* learner, const float* d_sample_loss, void* stream);
+/* One whole step: sample, train (trainer of the same scene), update. loss may be NULL. */
+int32_t nbvh_learner_step(nbvh_learner* learner, nbvh_trainer* trainer, float* loss, void* stream);
+int32_t nbvh_learner_get_state(const nbvh_learner* learner, nbvh_learner_state* state);
+/* Host copy of a learner array (nbvh_learner_array) into out (capacity in elements of its type);
+ * out NULL: only *count. Synchronous. */
+int32_t nbvh_learner_get(nbvh_learner* learner, uint32_t what, void* out, uint64_t capacity, uint64_t* count);
+
 /* Same with host buffers; synchronous. */
 int32_t nbvh_query_host(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                         const nbvh_ray* rays, uint32_t n_rays, nbvh_hit* hits);

@@ -18,7 +18,7 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
-__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "NbvhError", "network_config", "icosphere", "bvh_build",
+__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "Learner", "NbvhError", "network_config", "icosphere", "bvh_build",
            "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
            "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "leaf_segments", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
@@ -560,6 +560,81 @@ class Trainer:
     def close(self):
         if self.handle:
             _lib().nbvh_trainer_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+class Learner:
+    """Tree-cut learning of a scene's N-BVH (nbvh_learner_*), the reference's training loop
+    (src/cuda/neural/neural_bvh_renderer.cu:141-297): each step samples training segments from the
+    N-BVH being learned, trains the network on them, adds every sample's loss to its node and splits
+    the leaves of largest loss heuristic on the split schedule. Creating it replaces the scene's
+    N-BVH by the cut of its triangle BVH at neural_bvh_min_depth. Keyword arguments override the
+    reference's defaults (nbvh_learner_config field names)."""
+
+    def __init__(self, ctx, scene, **overrides):
+        self.ctx, self.scene = ctx, scene
+        cfg = _ffi.LearnerConfig()
+        _check(_lib().nbvh_learner_config_defaults(ctypes.byref(cfg)))
+        for k, v in overrides.items():
+            if not hasattr(cfg, k):
+                raise TypeError(f"unknown learner option {k}")
+            setattr(cfg, k, v)
+        self.config = cfg
+        self.handle = ctypes.c_void_p()
+        self.ctx.check(_lib().nbvh_learner_create(ctx.handle, scene.handle, ctypes.byref(cfg),
+                                                  ctypes.byref(self.handle)))
+
+    @staticmethod
+    def defaults():
+        cfg = _ffi.LearnerConfig()
+        _check(_lib().nbvh_learner_config_defaults(ctypes.byref(cfg)))
+        return {name: getattr(cfg, name) for name, _ in cfg._fields_ if name != "reserved"}
+
+    def sample(self, step, stream=None):
+        """Sample the batch of training step `step` into the learner's buffers."""
+        self.ctx.check(_lib().nbvh_learner_sample(self.handle, step, ctypes.c_void_p(stream or 0)))
+
+    def update_device(self, d_sample_loss_ptr, stream=None):
+        """Tree-cut update of the last sampled batch from per-slot losses (device floats)."""
+        self.ctx.check(_lib().nbvh_learner_update(self.handle, ctypes.c_void_p(d_sample_loss_ptr),
+                                                  ctypes.c_void_p(stream or 0)))
+
+    def update(self, sample_loss):
+        from . import hip
+        d = hip.Buffer.from_array(np.ascontiguousarray(sample_loss, np.float32))
+        self.update_device(d.ptr)
+
+    def step(self, trainer, stream=None):
+        """One whole step (sample, train, tree-cut update); returns the training loss."""
+        loss = ctypes.c_float()
+        self.ctx.check(_lib().nbvh_learner_step(self.handle, trainer.handle, ctypes.byref(loss),
+                                                ctypes.c_void_p(stream or 0)))
+        return loss.value
+
+    def state(self):
+        st = _ffi.LearnerState()
+        self.ctx.check(_lib().nbvh_learner_get_state(self.handle, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_ if name != "reserved"}
+
+    def get(self, what):
+        """A learner array: _ffi.LEARN_INPUTS / TARGETS (float, reshaped per slot), NODES (uint32),
+        TREE_CUT (TREE_CUT_DTYPE), SPLITS (uint32), NBVH (NODE_DTYPE)."""
+        dtype = {_ffi.LEARN_INPUTS: np.float32, _ffi.LEARN_TARGETS: np.float32, _ffi.LEARN_NODES: np.uint32,
+                 _ffi.LEARN_TREE_CUT: _ffi.TREE_CUT_DTYPE, _ffi.LEARN_SPLITS: np.uint32,
+                 _ffi.LEARN_NBVH: NODE_DTYPE}[what]
+        n = ctypes.c_uint64()
+        self.ctx.check(_lib().nbvh_learner_get(self.handle, what, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype)
+        self.ctx.check(_lib().nbvh_learner_get(self.handle, what, _ptr(out), n.value, ctypes.byref(n)))
+        if what == _ffi.LEARN_INPUTS:
+            return out.reshape(self.config.batch_size, -1)
+        if what == _ffi.LEARN_TARGETS:
+            return out.reshape(self.config.batch_size, 8)
+        return out
+
+    def close(self):
+        if self.handle:
+            _lib().nbvh_learner_destroy(self.handle)
             self.handle = ctypes.c_void_p()
 
 

@@ -96,6 +96,32 @@ class TrainConfig(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+class LearnerConfig(ctypes.Structure):
+    _fields_ = [("batch_size", ctypes.c_uint32), ("log2_bvh_max_traversals", ctypes.c_uint32),
+                ("sampling_bbox_scaling", ctypes.c_float), ("fixed_node_dilation", ctypes.c_float),
+                ("node_extent_dilation", ctypes.c_float), ("learning_mint_epsilon", ctypes.c_float),
+                ("learning_maxt_epsilon", ctypes.c_float), ("max_training_bounces", ctypes.c_uint32),
+                ("training_node_survival", ctypes.c_uint32), ("survival_probability_scale", ctypes.c_float),
+                ("min_rr_survival_prob", ctypes.c_float), ("tree_cut_learning", ctypes.c_uint32),
+                ("neural_bvh_min_depth", ctypes.c_uint32), ("num_lods_learned", ctypes.c_uint32),
+                ("max_training_steps", ctypes.c_uint32), ("lod_decision", ctypes.c_uint32),
+                ("splitting_interval", ctypes.c_int32), ("last_split_iteration", ctypes.c_int32),
+                ("start_split_num", ctypes.c_int32), ("split_scaling", ctypes.c_float),
+                ("split_interval_scaling", ctypes.c_float), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class LearnerState(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("current_learned_lod", ctypes.c_uint32),
+                ("all_lods_assigned", ctypes.c_uint32), ("tree_cut_learning", ctypes.c_uint32),
+                ("current_num_splits", ctypes.c_uint32), ("total_splits", ctypes.c_uint32),
+                ("sample_failures", ctypes.c_uint32), ("max_node_loss", ctypes.c_float), ("last_loss", ctypes.c_float),
+                ("expected_total_splits", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+
+
+TREE_CUT_DTYPE = np.dtype([("loss", "<f4"), ("iter_alive", "<u4"), ("depth", "<u4"), ("samples", "<u4")])
+LEARN_INPUTS, LEARN_TARGETS, LEARN_NODES, LEARN_TREE_CUT, LEARN_SPLITS, LEARN_NBVH = range(6)
+
+
 class LaunchShape(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("queue_bands", ctypes.c_uint32)]
 
@@ -181,6 +207,14 @@ SIGNATURES = {
     "nbvh_trainer_step": [P, P, P, U32, U32, ctypes.POINTER(F32), P],
     "nbvh_trainer_get": [P, U32, P, U64, PU64],
     "nbvh_trainer_commit": [P],
+    "nbvh_learner_config_defaults": [ctypes.POINTER(LearnerConfig)],
+    "nbvh_learner_create": [P, P, ctypes.POINTER(LearnerConfig), ctypes.POINTER(P)],
+    "nbvh_learner_destroy": [P],
+    "nbvh_learner_sample": [P, U32, P],
+    "nbvh_learner_update": [P, P, P],
+    "nbvh_learner_step": [P, P, ctypes.POINTER(F32), P],
+    "nbvh_learner_get_state": [P, ctypes.POINTER(LearnerState)],
+    "nbvh_learner_get": [P, U32, P, U64, PU64],
     "nbvh_path_trace": [P, P, P, ctypes.POINTER(PathParams), ctypes.POINTER(Camera), U32, U32, P,
                         ctypes.POINTER(PathStats), P],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],

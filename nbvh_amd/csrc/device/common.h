@@ -78,16 +78,18 @@ __device__ __forceinline__ void slab(const Box& b, const Ray& r, float& tn, floa
 // Same slab test with the reciprocal direction replaced by the division (kept as division:
 // the reference divides, and the FP32 path is bit-exact with it).
 
-// NeuralBVH2::intersect (include/cuda/neural/accel/neural_bvh2.cuh:170-292) with the inference
-// epsilons, on boxes that are already dilated by 1 + fixed_node_dilation about their centre.
-// Stack accessor S provides push/pop on whatever storage the kernel uses.
+// NeuralBVH2::intersect (include/cuda/neural/accel/neural_bvh2.cuh:170-292), by default with the
+// inference epsilons (the sampler of the training data passes the learning ones), on the boxes
+// the node source returns (the query's are dilated by 1 + fixed_node_dilation about their centre
+// once, on the host). Stack accessor S provides push/pop on whatever storage the kernel uses.
 // Returns 1 = leaf segment found, 0 = stack exhausted, -1 = stack overflow, 2 = suspended after
 // `budget` node visits with the next node pushed back (the next call resumes exactly there, so
 // suspension never changes which segments are found or their order).
 template <typename Nodes, typename Stack>
 __device__ __forceinline__ int nbvh_next_leaf(const Nodes& nodes, uint32_t lod, const Ray& ray, Stack& st,
                                               float current_maxt, float& seg_mint, float& seg_maxt,
-                                              uint32_t& seg_node, uint32_t budget = 0xFFFFFFFFu)
+                                              uint32_t& seg_node, uint32_t budget = 0xFFFFFFFFu,
+                                              float mint_eps = kInferenceMintEps, float maxt_eps = kInferenceMaxtEps)
 {
     uint32_t idx = st.pop();
     while (true) {
@@ -103,9 +105,9 @@ __device__ __forceinline__ int nbvh_next_leaf(const Nodes& nodes, uint32_t lod, 
             if (mint <= maxt) {
                 bool valid = mint < current_maxt;
                 if (mint < 0.f)
-                    valid = valid && (maxt > kInferenceMaxtEps);
+                    valid = valid && (maxt > maxt_eps);
                 else
-                    valid = valid && (mint > kInferenceMintEps);
+                    valid = valid && (mint > mint_eps);
                 if (valid) {
                     seg_mint = mint;
                     seg_maxt = maxt;
@@ -171,6 +173,32 @@ __device__ __forceinline__ float expf_cw(float x)
     return ldexpf(p, (int)n);
 }
 __device__ __forceinline__ float sigmoidf_cw(float z) { return 1.f / (1.f + expf_cw(-z)); }
+
+// sin and cos in float, written out so that the device and oracle/nbvh_oracle.cpp (sincosf_cw)
+// run the same operation sequence: Cody-Waite reduction by pi/2 with a three-part constant
+// (1.5703125 and a 12-bit second part, so q * part is exact for the small quadrant counts of
+// the sampler's angles in [0, 2 pi)), then the Taylor polynomials of sin (to r^9) and cos (to
+// r^10) on [-pi/4, pi/4] in Horner form, and the quadrant swap.
+__device__ __forceinline__ void sincosf_cw(float x, float& s, float& c)
+{
+    const float q = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-q, 1.5703125f, x);
+    r       = fmaf(-q, 4.838705062866211e-4f, r);
+    r       = fmaf(-q, -4.371138828673793e-8f, r);
+    const float r2 = r * r;
+    float sp = fmaf(r2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+    sp = fmaf(sp, r2, 8.3333333333333332e-3f);
+    sp = fmaf(sp, r2, -1.6666666666666666e-1f);
+    const float sr = fmaf(sp, r2 * r, r);
+    float cp = fmaf(r2, -2.7557319223985890e-7f, 2.4801587301587302e-5f);
+    cp = fmaf(cp, r2, -1.3888888888888889e-3f);
+    cp = fmaf(cp, r2, 4.1666666666666664e-2f);
+    cp = fmaf(cp, r2, -0.5f);
+    const float cr = fmaf(cp, r2, 1.0f);
+    const int quadrant = (int)q & 3;
+    s = quadrant == 0 ? sr : (quadrant == 1 ? cr : (quadrant == 2 ? -sr : -cr));
+    c = quadrant == 0 ? cr : (quadrant == 1 ? -sr : (quadrant == 2 ? -cr : sr));
+}
 
 // 30-bit Morton code of a point given in [0,1]^3 (10 bits per axis, x in the lowest bit;
 // coordinates clamped, NaN -> 0).

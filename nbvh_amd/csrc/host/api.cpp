@@ -739,7 +739,7 @@ NBVH_API int32_t nbvh_scene_get_bvh(const nbvh_scene* s, nbvh_node* nodes, uint3
     });
 }
 
-static void set_nbvh(nbvh_scene* s, std::vector<nbvh_node> nodes)
+void nbvh::set_scene_nbvh(nbvh_scene* s, std::vector<nbvh_node> nodes)
 {
     nbvh_validate(nodes.data(), (uint32_t)nodes.size());
     const auto dil = nbvh_dilate(nodes, kFixedNodeDilation);
@@ -764,7 +764,7 @@ NBVH_API int32_t nbvh_scene_build_nbvh(nbvh_scene* s, uint32_t n_leaves)
         if (s->bvh.empty()) fail(NBVH_ERR_STATE, "scene has no triangle BVH to cut");
         std::vector<nbvh_node> nd;
         nbvh_cut(s->bvh.data(), (uint32_t)s->bvh.size(), n_leaves, nd);
-        set_nbvh(s, std::move(nd));
+        set_scene_nbvh(s, std::move(nd));
     });
 }
 
@@ -774,7 +774,7 @@ NBVH_API int32_t nbvh_scene_load_nbvh(nbvh_scene* s, const void* bytes, size_t s
         require(s != nullptr, "scene is NULL");
         std::vector<nbvh_node> nd;
         nbvh_parse(bytes, size, nd);
-        set_nbvh(s, std::move(nd));
+        set_scene_nbvh(s, std::move(nd));
     });
 }
 
@@ -782,7 +782,7 @@ NBVH_API int32_t nbvh_scene_set_nbvh(nbvh_scene* s, const nbvh_node* nodes, uint
 {
     return guarded(s ? s->ctx : nullptr, [&] {
         require(s && nodes && n_nodes > 0, "NULL argument");
-        set_nbvh(s, std::vector<nbvh_node>(nodes, nodes + n_nodes));
+        set_scene_nbvh(s, std::vector<nbvh_node>(nodes, nodes + n_nodes));
     });
 }
 

@@ -113,6 +113,9 @@ struct nbvh_scene {
 
 namespace nbvh {
 
+// Replace the scene's N-BVH (validated; the device copy gets the dilated boxes).
+void set_scene_nbvh(nbvh_scene* s, std::vector<nbvh_node> nodes);
+
 // Kernel arguments of a scene from its host state; device copies of its network (all forms).
 void refresh_args(nbvh_scene* s);
 void upload_network(nbvh_scene* s);
@@ -137,6 +140,12 @@ private:
     bool captured_ = false;
     std::unique_lock<std::mutex> lock_;
 };
+
+// One training step (nbvh_trainer_step), optionally writing each sample's loss (the sum of its
+// 8 element values) to d_sample_loss; throws Error on failure.
+void trainer_step(nbvh_trainer* t, const float* d_inputs, const float* d_targets, uint32_t n, uint32_t flags,
+                  float* d_sample_loss, float* loss, hipStream_t stream);
+nbvh_scene* trainer_scene(const nbvh_trainer* t);
 
 // Allocates the context's part tables on first use.
 void ensure_part_tables(nbvh_ctx* ctx);

@@ -120,43 +120,55 @@ NBVH_API int32_t nbvh_trainer_destroy(nbvh_trainer* t)
     return NBVH_OK;
 }
 
+namespace nbvh {
+
+nbvh_scene* trainer_scene(const nbvh_trainer* t) { return t->scene; }
+
+void trainer_step(nbvh_trainer* t, const float* d_inputs, const float* d_targets, uint32_t n, uint32_t flags,
+                  float* d_sample_loss, float* loss, hipStream_t st)
+{
+    require(t != nullptr, "NULL trainer");
+    require((flags & ~(uint32_t)NBVH_TRAIN_GRADIENTS_ONLY) == 0u, "unknown training flags");
+    require(n <= t->cfg.max_batch, "batch larger than the trainer's max_batch");
+    if (n > 0) require(d_inputs && d_targets, "NULL buffer");
+    check_generation(t);
+    hip_check(hipSetDevice(t->ctx->device), "hipSetDevice");
+    hip_check(train_zero(t->g_tables.as<float>(), t->n_table, st), "zero");
+    hip_check(train_zero(t->g_mlp.as<float>(), t->n_mlp, st), "zero");
+    hip_check(train_zero(t->loss.as<float>(), 1, st), "zero");
+    bool supported = false;
+    TrainArgs args = train_args(t);
+    args.sample_loss = d_sample_loss;
+    hip_check(train_gradients(args, d_inputs, d_targets, n, st, &supported), "training kernels");
+    if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no training kernels for this network shape");
+    if ((flags & NBVH_TRAIN_GRADIENTS_ONLY) == 0u && n > 0) {
+        t->step++;
+        AdamArgs h{};
+        h.lr        = t->cfg.learning_rate * decay_factor(t->cfg, t->step);
+        h.beta1     = t->cfg.beta1;
+        h.beta2     = t->cfg.beta2;
+        h.epsilon   = t->cfg.epsilon;
+        h.l2_reg    = t->cfg.l2_reg;
+        hip_check(train_adam(t->tables.as<float>(), t->g_tables.as<float>(), t->m_tables.as<float>(),
+                             t->v_tables.as<float>(), t->n_tables.as<uint32_t>(), t->n_table, false, h, st),
+                  "adam tables");
+        hip_check(train_adam(t->mlp.as<float>(), t->g_mlp.as<float>(), t->m_mlp.as<float>(), t->v_mlp.as<float>(),
+                             t->n_mlp_steps.as<uint32_t>(), t->n_mlp, true, h, st),
+                  "adam mlp");
+    }
+    if (loss) {
+        hip_check(hipMemcpyAsync(loss, t->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, st), "loss D2H");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+}
+
+}  // namespace nbvh
+
 NBVH_API int32_t nbvh_trainer_step(nbvh_trainer* t, const float* d_inputs, const float* d_targets, uint32_t n,
                                    uint32_t flags, float* loss, void* stream)
 {
-    return guarded(t ? t->ctx : nullptr, [&] {
-        require(t != nullptr, "NULL trainer");
-        require((flags & ~(uint32_t)NBVH_TRAIN_GRADIENTS_ONLY) == 0u, "unknown training flags");
-        require(n <= t->cfg.max_batch, "batch larger than the trainer's max_batch");
-        if (n > 0) require(d_inputs && d_targets, "NULL buffer");
-        check_generation(t);
-        hipStream_t st = (hipStream_t)stream;
-        hip_check(hipSetDevice(t->ctx->device), "hipSetDevice");
-        hip_check(train_zero(t->g_tables.as<float>(), t->n_table, st), "zero");
-        hip_check(train_zero(t->g_mlp.as<float>(), t->n_mlp, st), "zero");
-        hip_check(train_zero(t->loss.as<float>(), 1, st), "zero");
-        bool supported = false;
-        hip_check(train_gradients(train_args(t), d_inputs, d_targets, n, st, &supported), "training kernels");
-        if (!supported) fail(NBVH_ERR_UNSUPPORTED, "no training kernels for this network shape");
-        if ((flags & NBVH_TRAIN_GRADIENTS_ONLY) == 0u && n > 0) {
-            t->step++;
-            AdamArgs h{};
-            h.lr        = t->cfg.learning_rate * decay_factor(t->cfg, t->step);
-            h.beta1     = t->cfg.beta1;
-            h.beta2     = t->cfg.beta2;
-            h.epsilon   = t->cfg.epsilon;
-            h.l2_reg    = t->cfg.l2_reg;
-            hip_check(train_adam(t->tables.as<float>(), t->g_tables.as<float>(), t->m_tables.as<float>(),
-                                 t->v_tables.as<float>(), t->n_tables.as<uint32_t>(), t->n_table, false, h, st),
-                      "adam tables");
-            hip_check(train_adam(t->mlp.as<float>(), t->g_mlp.as<float>(), t->m_mlp.as<float>(), t->v_mlp.as<float>(),
-                                 t->n_mlp_steps.as<uint32_t>(), t->n_mlp, true, h, st),
-                      "adam mlp");
-        }
-        if (loss) {
-            hip_check(hipMemcpyAsync(loss, t->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, st), "loss D2H");
-            hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        }
-    });
+    return guarded(t ? t->ctx : nullptr,
+                   [&] { trainer_step(t, d_inputs, d_targets, n, flags, nullptr, loss, (hipStream_t)stream); });
 }
 
 NBVH_API int32_t nbvh_trainer_get(nbvh_trainer* t, uint32_t what, float* out, uint64_t capacity, uint64_t* count)

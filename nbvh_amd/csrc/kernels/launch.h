@@ -83,4 +83,13 @@ hipError_t query_mfma(const NetArgs& a, uint32_t precision, const RayBatch& rb, 
 hipError_t network_mfma(const NetArgs& a, uint32_t precision, const float* in, uint32_t n, float* out, float* logits,
                         uint32_t n_cus, hipStream_t stream, bool* supported);
 
+// Tree-cut learning (learn.hip): the training-data sampler; the per-node loss update, the leaves'
+// loss heuristics and their descending sort (keys_sorted / vals_sorted: heuristic, node).
+hipError_t learn_sample(const SampleArgs& a, hipStream_t stream);
+size_t treecut_sort_scratch_bytes(uint32_t n_nodes);
+hipError_t treecut_update(const uint32_t* node_idx, const float* sample_loss, uint32_t n, const nbvh_node* nodes,
+                          uint32_t n_nodes, TreeCutNode* tc, double* sums, uint32_t* counts, float* keys,
+                          uint32_t* vals, float* keys_sorted, uint32_t* vals_sorted, void* tmp, size_t tmp_bytes,
+                          hipStream_t stream);
+
 }  // namespace nbvh

@@ -182,6 +182,7 @@ __global__ __launch_bounds__(64) void train_backward_kernel(TrainArgs t, const f
             dz[j] = (g * inv_n) * (y * (1.f - y));  // through the Sigmoid output activation
             row(act, S, kD5 + j, i) = dz[j];
         }
+        if (t.sample_loss) t.sample_loss[i] = lsum;  // this sample's element values, in element order
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);

@@ -113,6 +113,38 @@ struct TrainArgs {
     float*       grad_tables;
     float*       grad_mlp;
     float*       loss;
+    float*       sample_loss;  // optional: per-sample sum of the 8 element loss values
+};
+
+// Tree-cut learning data of one N-BVH node (NeuralTreeCutData,
+// include/cuda/neural/neural_bvh_renderer.h:36-47).
+struct TreeCutNode {
+    float    loss;
+    uint32_t iter_alive, depth, samples;
+};
+
+// Training-data sampler (granular_sample_bvh_network_input_data,
+// src/cuda/neural/neural_bvh_module_kernels.cu:119-264).
+struct SampleArgs {
+    const nbvh_node*   nodes;          // the learner's N-BVH, boxes as built (not dilated)
+    uint32_t           n_nodes;
+    TriArgs            tri;            // the scene's triangles (targets)
+    float              box_min[3], box_ext[3];  // neural scene bbox: origins are sampled in it
+    float              enc_center[3], enc_extent[3];
+    float              seg_t[4];
+    uint32_t           n_points;
+    float              scaling;        // sampling_bbox_scaling
+    float              fixed_dilation, extent_dilation;
+    float              mint_eps, maxt_eps;  // learning epsilons
+    uint32_t           current_lod;    // current_learned_lod
+    uint32_t           survival;       // training_node_survival
+    float              survival_scale, min_survival, current_max_loss;
+    const TreeCutNode* tree_cut;
+    uint32_t           n_sub_batch, sub_batch_size, batch_size, step;
+    float*             inputs;         // batch x (3 * n_points)
+    float*             targets;        // batch x 8
+    uint32_t*          node_idx;       // batch
+    uint32_t*          failures;       // threads that gave up after the attempt cap
 };
 
 // Adam step constants (the bias correction uses each parameter's own update count).

@@ -67,6 +67,8 @@ def lib():
             "oracle_neural_query": ([vp, vp, vp, u32, u32, u32, u32, u32, f32, vp, vp, u64, vp], None),
             "oracle_tri_query": ([vp, vp, vp, vp, vp, u32, u32, vp], None),
             "oracle_tri_brute": ([vp, vp, u32, vp, u32, vp], None),
+            "oracle_learn_sample": ([vp] * 15, None),
+            "oracle_sincosf": ([vp, u32, vp, vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -297,6 +299,60 @@ def path_trace(query, cam12, w, h, max_depth=5, rr_start_depth=0, sample=0, back
         rays, state = path_shade(depth, rays, hits, state, accum, max_depth, rr_start_depth, sample, background,
                                  background_color, env_albedo, normal_offset, verts, tris)
     return accum, counts
+
+
+class SampleCfg(ctypes.Structure):
+    _fields_ = [("batch_size", ctypes.c_uint32), ("n_sub_batch", ctypes.c_uint32), ("step", ctypes.c_uint32),
+                ("current_lod", ctypes.c_uint32), ("survival", ctypes.c_uint32), ("scaling", ctypes.c_float),
+                ("fixed_dilation", ctypes.c_float), ("extent_dilation", ctypes.c_float),
+                ("mint_eps", ctypes.c_float), ("maxt_eps", ctypes.c_float), ("survival_scale", ctypes.c_float),
+                ("min_survival", ctypes.c_float), ("current_max_loss", ctypes.c_float),
+                ("box_min", ctypes.c_float * 3), ("box_max", ctypes.c_float * 3), ("n_points", ctypes.c_uint32),
+                ("distribution", ctypes.c_uint32)]
+
+
+TREE_CUT_DTYPE = np.dtype([("loss", "<f4"), ("iter_alive", "<u4"), ("depth", "<u4"), ("samples", "<u4")])
+
+
+def learn_sample(nbvh_nodes, tree_cut, verts, tris, bvh_nodes, prims, box_min, box_max, step, batch_size,
+                 prev=None, n_sub_batch=1, current_lod=6, survival=True, current_max_loss=0.0, scaling=1.5,
+                 fixed_dilation=1e-5, extent_dilation=1e-4, mint_eps=1e-3, maxt_eps=1e-3, survival_scale=1.0,
+                 min_survival=0.005, n_points=3, distribution=1):
+    """Restatement of the reference's training-data sampler (granular_sample_bvh_network_input_data,
+    src/cuda/neural/neural_bvh_module_kernels.cu:119-264). prev: (inputs, targets, node_idx) the
+    slots held before the step (zeros when None). Returns dict(inputs, targets, node_idx, rays,
+    seg [mint, maxt], lod, dil, written) per slot."""
+    c = SampleCfg(batch_size, n_sub_batch, step, current_lod, int(survival), scaling, fixed_dilation,
+                  extent_dilation, mint_eps, maxt_eps, survival_scale, min_survival, current_max_loss,
+                  (ctypes.c_float * 3)(*np.asarray(box_min, np.float32)),
+                  (ctypes.c_float * 3)(*np.asarray(box_max, np.float32)), n_points, distribution)
+    n_in = 3 * n_points
+    if prev is None:
+        prev = (np.zeros((batch_size, n_in), np.float32), np.zeros((batch_size, 8), np.float32),
+                np.zeros(batch_size, np.uint32))
+    inputs = np.array(prev[0], np.float32, copy=True).reshape(batch_size, n_in)
+    targets = np.array(prev[1], np.float32, copy=True).reshape(batch_size, 8)
+    node_idx = np.array(prev[2], np.uint32, copy=True)
+    rays = np.zeros(batch_size, RAY_DTYPE)
+    seg = np.zeros((batch_size, 2), np.float32)
+    lod = np.zeros(batch_size, np.uint32)
+    dil = np.zeros(batch_size, np.float32)
+    written = np.zeros(batch_size, np.uint32)
+    tc = np.ascontiguousarray(tree_cut, TREE_CUT_DTYPE)
+    lib().oracle_learn_sample(ctypes.byref(c), _p(np.ascontiguousarray(nbvh_nodes, NODE_DTYPE)), _p(tc),
+                              _p(np.ascontiguousarray(verts, np.float32)), _p(np.ascontiguousarray(tris, np.uint32)),
+                              _p(np.ascontiguousarray(bvh_nodes, NODE_DTYPE)), _p(np.ascontiguousarray(prims, np.uint32)),
+                              _p(inputs), _p(targets), _p(node_idx), _p(rays), _p(seg), _p(lod), _p(dil), _p(written))
+    return dict(inputs=inputs, targets=targets, node_idx=node_idx, rays=rays, seg=seg, lod=lod, dil=dil,
+                written=written)
+
+
+def sincosf(x):
+    """The sampler's float sin and cos (Cody-Waite reduction + Taylor polynomials)."""
+    x = np.ascontiguousarray(x, np.float32)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    lib().oracle_sincosf(_p(x), len(x), _p(s), _p(c))
+    return s, c
 
 
 def tri_brute(verts, tris, rays):

@@ -153,6 +153,32 @@ inline float expf_cw(float x)
     return std::ldexp(p, (int)n);
 }
 inline float sigmoid_f(float z) { return 1.f / (1.f + expf_cw(-z)); }
+
+// sin and cos in float: Cody-Waite reduction by pi/2 with the three-part constant 1.5703125 +
+// 4.838705062866211e-4 - 4.371138828673793e-8 (the first two with at most 12 significant bits,
+// so q * part is exact for small quadrant counts q), then the Taylor polynomials of sin (to r^9)
+// and cos (to r^10) on [-pi/4, pi/4] in Horner form and the quadrant swap. The sampler kernel
+// evaluates the identical sequence.
+inline void sincosf_cw(float x, float &s, float &c)
+{
+    const float q = std::rint(x * 0.636619772367581343f);
+    float r       = std::fma(-q, 1.5703125f, x);
+    r             = std::fma(-q, 4.838705062866211e-4f, r);
+    r             = std::fma(-q, -4.371138828673793e-8f, r);
+    const float r2 = r * r;
+    float sp = std::fma(r2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);  // 1/9!, -1/7!
+    sp       = std::fma(sp, r2, 8.3333333333333332e-3f);                       // 1/5!
+    sp       = std::fma(sp, r2, -1.6666666666666666e-1f);                      // -1/3!
+    const float sr = std::fma(sp, r2 * r, r);
+    float cp = std::fma(r2, -2.7557319223985890e-7f, 2.4801587301587302e-5f);  // -1/10!, 1/8!
+    cp       = std::fma(cp, r2, -1.3888888888888889e-3f);                      // -1/6!
+    cp       = std::fma(cp, r2, 4.1666666666666664e-2f);                       // 1/4!
+    cp       = std::fma(cp, r2, -0.5f);
+    const float cr = std::fma(cp, r2, 1.0f);
+    const int quadrant = (int)q & 3;
+    s = quadrant == 0 ? sr : (quadrant == 1 ? cr : (quadrant == 2 ? -sr : -cr));
+    c = quadrant == 0 ? cr : (quadrant == 1 ? -sr : (quadrant == 2 ? -cr : sr));
+}
 inline double sigmoid_d(double z) { return 1.0 / (1.0 + std::exp(-z)); }
 
 // ------------------------------------------------------------------------------------------
@@ -188,11 +214,14 @@ inline void dilate(const Node &n, float dilation, float dmin[3], float dmax[3])
 inline bool is_leaf_at_lod(const Node &n, uint32_t lod) { return n.prim_count > 0 && n.prim_count <= 2u + lod; }
 
 // ------------------------------------------------------------------------------------------
-// NeuralBVH2::intersect (neural_bvh2.cuh:170-292), inference epsilons, local stack.
+// NeuralBVH2::intersect (neural_bvh2.cuh:170-292), local stack; by default the inference
+// epsilons and the fixed dilation (the training sampler passes a sampled dilation and the
+// learning epsilons).
 // Returns 1 (leaf segment found), 0 (stack exhausted), -1 (stack overflow).
 // ------------------------------------------------------------------------------------------
 int nbvh_intersect(const Node *nodes, uint32_t lod, const Ray &ray_in, uint32_t *stack, int &ss, float current_maxt,
-                   float &seg_mint, float &seg_maxt, uint32_t &seg_node)
+                   float &seg_mint, float &seg_maxt, uint32_t &seg_node, float dilation = FIXED_NODE_DILATION,
+                   float mint_eps = INFERENCE_MINT_EPSILON, float maxt_eps = INFERENCE_MAXT_EPSILON)
 {
     Ray ray  = ray_in;
     ray.tmin = -FLT_MAX;
@@ -202,14 +231,14 @@ int nbvh_intersect(const Node *nodes, uint32_t lod, const Ray &ray_in, uint32_t 
         const Node &node = nodes[idx];
         if (is_leaf_at_lod(node, lod) || node.left == NO_NEURAL_CHILD) {
             float dmin[3], dmax[3], mint, maxt;
-            dilate(node, FIXED_NODE_DILATION, dmin, dmax);
+            dilate(node, dilation, dmin, dmax);
             slab(dmin, dmax, ray, mint, maxt);
             if (mint <= maxt) {
                 bool valid = mint < ray.tmax;
                 if (mint < 0.f)
-                    valid = valid && (maxt > INFERENCE_MAXT_EPSILON);
+                    valid = valid && (maxt > maxt_eps);
                 else
-                    valid = valid && (mint > INFERENCE_MINT_EPSILON);
+                    valid = valid && (mint > mint_eps);
                 if (valid) {
                     seg_mint = mint;
                     seg_maxt = maxt;
@@ -225,11 +254,11 @@ int nbvh_intersect(const Node *nodes, uint32_t lod, const Ray &ray_in, uint32_t 
         float    m1, m2;
         {
             float dmin[3], dmax[3], tn, tf;
-            dilate(nodes[c1], FIXED_NODE_DILATION, dmin, dmax);
+            dilate(nodes[c1], dilation, dmin, dmax);
             slab(dmin, dmax, ray, tn, tf);
             float mint = gmax(ray.tmin, tn), maxt = gmin(ray.tmax, tf);
             m1         = (mint <= maxt) ? mint : FLT_MAX;
-            dilate(nodes[c2], FIXED_NODE_DILATION, dmin, dmax);
+            dilate(nodes[c2], dilation, dmin, dmax);
             slab(dmin, dmax, ray, tn, tf);
             mint = gmax(ray.tmin, tn);
             maxt = gmin(ray.tmax, tf);
@@ -1085,4 +1114,175 @@ ORACLE_API void oracle_tri_brute(const float *verts, const uint32_t *indices, ui
         }
         H[i] = h;
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Training-data sampler: granular_sample_bvh_network_input_data
+// (src/cuda/neural/neural_bvh_module_kernels.cu:119-264) with
+// stack_based_sample_uniform_bvh_density (include/cuda/neural/neural_bvh_module_common.cuh:703-780),
+// square_to_uniform_sphere (include/cuda/utils/sampling.cuh:8-14), local_to_bbox
+// (include/base/bbox.h:81-85), encode_segment_from_world_po_pi (neural_bvh_module_kernels.cu:10-97,
+// no swap of a negative entry) and generate_single_bvh_network_output_reference
+// (include/cuda/neural/neural_bvh_module_common.cuh:783-861), for at most one training bounce.
+// The LCG draws are taken in the reference's textual order (glm's vector constructors read
+// left to right). tree_cut: 4 words per node (loss, iter_alive, depth, samples).
+// Outputs: inputs / targets / node_idx of the slots written (the caller passes their previous
+// contents); per slot the debug record of its last sampled segment: ray, [mint, maxt], LoD,
+// dilation, and whether the slot's inputs were written (1) or the sample was dropped (0).
+// ------------------------------------------------------------------------------------------
+extern "C" {
+typedef struct oracle_sample_cfg {
+    uint32_t batch_size, n_sub_batch, step, current_lod, survival;
+    float    scaling, fixed_dilation, extent_dilation, mint_eps, maxt_eps;
+    float    survival_scale, min_survival, current_max_loss;
+    float    box_min[3], box_max[3];
+    uint32_t n_points, distribution;
+} oracle_sample_cfg;
+}
+
+namespace {
+inline float loss_heuristic(const uint32_t *tc, uint32_t node)
+{
+    float loss;
+    std::memcpy(&loss, tc + 4 * node, 4);
+    const uint32_t iter_alive = tc[4 * node + 1], samples = tc[4 * node + 3];
+    return samples == 0 ? -FLT_MAX : loss / ((float)iter_alive * std::sqrt((float)samples / (float)iter_alive));
+}
+}  // namespace
+
+ORACLE_API void oracle_learn_sample(const oracle_sample_cfg *c, const void *nbvh, const uint32_t *tree_cut,
+                                    const float *verts, const uint32_t *tris, const void *bvh, const uint32_t *prims,
+                                    float *inputs, float *targets, uint32_t *node_idx, void *dbg_rays,
+                                    float *dbg_seg, uint32_t *dbg_lod, float *dbg_dil, uint32_t *written)
+{
+    const Node *N   = (const Node *)nbvh;
+    const Node *B   = (const Node *)bvh;
+    Ray *DR         = (Ray *)dbg_rays;
+    const uint32_t P = c->n_points, n_in = 3 * P;
+    float seg_t[4];
+    {
+        float delta, offset;
+        if (c->distribution == 1) {
+            delta  = (P == 1) ? 1.f : 1.f / (float)(P - 1);
+            offset = 0.f;
+        } else {
+            delta  = 1.f / (float)P;
+            offset = 0.5f;
+        }
+        for (uint32_t k = 0; k < 4; k++) seg_t[k] = delta * ((float)k + offset);
+    }
+    float ext[3], center[3], enc_ext[3];
+    const float enc_dil = (1.f + NODE_EXTENT_DILATION) + FIXED_NODE_DILATION;
+    for (int k = 0; k < 3; k++) {
+        ext[k]     = c->box_max[k] - c->box_min[k];
+        center[k]  = (c->box_max[k] + c->box_min[k]) * 0.5f;
+        enc_ext[k] = ext[k] * enc_dil;
+    }
+    const uint32_t sub_batch = c->batch_size / c->n_sub_batch;
+    const uint32_t n_lods    = 7u - c->current_lod;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < (int64_t)sub_batch; ii++) {
+        uint32_t stack[NEURAL_STACK_SIZE];
+        int ss = 0;
+        Ray ray{{0.f, 0.f, 0.f}, 1e-4f, {1.f, 0.f, 0.f}, FLT_MAX};
+        bool cont = false;
+        uint32_t slot = (uint32_t)ii * c->n_sub_batch;
+        for (uint32_t inner = 0; inner < c->n_sub_batch; inner++) {
+            uint32_t rng = lcg_seed(c->step * c->batch_size, 0u, slot);
+            // square_to_uniform_sphere(lcg_random2f(rng))
+            const float u0 = lcg_float(rng), u1 = lcg_float(rng);
+            const float phi = (2.0f * 3.14159265358979323846f) * u0;
+            const float ct  = u1 * 2.0f - 1.0f;
+            const float st  = std::sqrt(1.0f - ct * ct);
+            float sp, cp;
+            sincosf_cw(phi, sp, cp);
+            const float dir[3] = {st * cp, st * sp, ct};
+            // sampled_lod = current + min(uint(u * n), n - 1); sampled_dilation
+            const uint32_t k   = (uint32_t)(lcg_float(rng) * (float)n_lods);
+            const uint32_t lod = c->current_lod + std::min(k, n_lods - 1u);
+            const float dil    = c->fixed_dilation + lcg_float(rng) * c->extent_dilation;
+            bool valid = cont && ss != 0;
+            float mint = 0.f, maxt = 0.f;
+            uint32_t node = 0;
+            uint32_t attempts = 0;
+            do {
+                if (!valid) {
+                    stack[0] = 0;
+                    ss       = 1;
+                    float p[3];
+                    const float h = (c->scaling - 1.f) * 0.5f;
+                    for (int q = 0; q < 3; q++) p[q] = c->scaling * lcg_float(rng) - h;
+                    for (int q = 0; q < 3; q++) {
+                        ray.o[q] = p[q] * ext[q] + c->box_min[q];
+                        ray.d[q] = dir[q];
+                    }
+                    ray.tmin = 1e-4f;
+                    ray.tmax = FLT_MAX;
+                }
+                valid = nbvh_intersect(N, lod, ray, stack, ss, FLT_MAX, mint, maxt, node, dil, c->mint_eps,
+                                       c->maxt_eps) == 1;
+                if (!valid) ++attempts;
+            } while (!valid && attempts < 65536u);
+            DR[slot]              = ray;
+            dbg_seg[2 * slot]     = mint;
+            dbg_seg[2 * slot + 1] = maxt;
+            dbg_lod[slot]         = lod;
+            dbg_dil[slot]         = dil;
+            written[slot]         = 0;
+            if (!valid) {  // the sampler gives up on this slot (never reached in practice)
+                ss   = 0;
+                cont = false;
+                continue;
+            }
+            node_idx[slot] = node;
+            if (c->survival) {
+                const float est = loss_heuristic(tree_cut, node);
+                float keep      = std::max(c->min_survival, est / c->current_max_loss);
+                if (!(est <= c->current_max_loss)) keep = 1.f;
+                if (c->survival_scale * keep < lcg_float(rng)) continue;
+            }
+            float eo[3], ei[3];
+            for (int q = 0; q < 3; q++) {
+                const float po = ray.o[q] + mint * ray.d[q];
+                const float pi = ray.o[q] + maxt * ray.d[q];
+                eo[q]          = (po - center[q]) / enc_ext[q] + 0.5f;
+                ei[q]          = (pi - center[q]) / enc_ext[q] + 0.5f;
+            }
+            for (uint32_t pt = 0; pt < P; pt++) {
+                const float t = seg_t[pt];
+                for (int q = 0; q < 3; q++) inputs[(size_t)slot * n_in + 3 * pt + q] = (1.f - t) * eo[q] + t * ei[q];
+            }
+            Ray lr  = ray;
+            lr.tmin = mint;
+            lr.tmax = maxt;
+            Hit h;
+            tri_ray(verts, tris, B, prims, lr, 0u, h);
+            float *out     = targets + (size_t)slot * 8;
+            const bool hit = (h.info & 0xFFu) == 2u;
+            for (int j = 0; j < 8; j++) out[j] = 0.f;
+            if (hit) {
+                out[0] = 1.f;
+                out[1] = (float)((double)(h.t - mint) / ((double)(maxt - mint) + 1e-3));
+                const float *a = verts + 3 * tris[3 * h.prim_id], *b = verts + 3 * tris[3 * h.prim_id + 1],
+                            *d = verts + 3 * tris[3 * h.prim_id + 2];
+                const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                const float e2[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+                const float nr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                     e1[0] * e2[1] - e1[1] * e2[0]};
+                const float inv = 1.f / std::sqrt((nr[0] * nr[0] + nr[1] * nr[1]) + nr[2] * nr[2]);
+                for (int q = 0; q < 3; q++) {
+                    out[2 + q] = (nr[q] * inv + 1.f) * 0.5f;
+                    out[5 + q] = 0.9f;
+                }
+            }
+            written[slot] = 1;
+            cont          = !hit;
+            slot++;
+        }
+    }
+}
+
+ORACLE_API void oracle_sincosf(const float *x, uint32_t n, float *s, float *c)
+{
+    for (uint32_t i = 0; i < n; i++) sincosf_cw(x[i], s[i], c[i]);
 }
