@@ -331,6 +331,19 @@ int32_t nbvh_segment_labels(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_r
 int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
                    uint32_t n_rays, nbvh_hit* d_hits, void* stream);
 
+/* nbvh_query without any allocation. No counterpart in the reference: this entry point is for
+ * callers that cannot allocate device memory (code under HIP graph capture, allocation-free
+ * serving loops). Same arguments and hit records as nbvh_query; it calls no hipMalloc, hipFree or
+ * allocating library routine. Its scratch (the queue counters and, with NBVH_QUERY_SORT_RAYS, the
+ * sort keys and claim order) is one static 24 MiB device array per GPU, set up by nbvh_create and
+ * held until the process ends. Limits: a sorted query takes at most NBVH_NOALLOC_MAX_SORTED_RAYS
+ * rays (NBVH_ERR_CAPACITY beyond); NBVH_QUERY_WAVEFRONT is refused (NBVH_ERR_UNSUPPORTED). Every
+ * call on one GPU shares the array, whatever its context: calls that overlap on one GPU (other
+ * streams, other threads, other contexts) race and must be ordered by the caller. */
+#define NBVH_NOALLOC_MAX_SORTED_RAYS 524288u
+int32_t nbvh_query_noalloc(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
+                           const nbvh_ray* d_rays, uint32_t n_rays, nbvh_hit* d_hits, void* stream);
+
 /* One launch over several device ray arrays (parts), each closest-hit or any-hit: e.g. the
  * closest-hit and shadow rays of a wavefront bounce in one persistent kernel. params->kind picks
  * the family (neural or triangle) and its precision / lod / limits; each part's any_hit picks

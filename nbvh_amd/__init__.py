@@ -360,6 +360,13 @@ class NeuralBVHModule:
                                          ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
                                          ctypes.c_void_p(stream or 0)))
 
+    def query_device_noalloc(self, params, d_rays_ptr, n, d_hits_ptr, stream=None):
+        """query_device through nbvh_query_noalloc: no allocation, scratch in the library's static
+        per-GPU array (calls overlapping on one GPU race; sorted queries up to 524,288 rays)."""
+        self.ctx.check(_lib().nbvh_query_noalloc(self.ctx.handle, self.scene.handle, ctypes.byref(params),
+                                                 ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
+                                                 ctypes.c_void_p(stream or 0)))
+
     def autotune(self, params, d_rays_ptr, n, d_hits_ptr, stream=None):
         """Time the candidate launch shapes on these device rays; keep the fastest for the scene and
         precision. Returns ((blocks_per_cu, queue_bands), ms)."""

@@ -27,6 +27,7 @@ ERR_CAPACITY = 9
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
 QUERY_SORT_RAYS = 1
 QUERY_WAVEFRONT = 2
+NOALLOC_MAX_SORTED_RAYS = 524288  # NBVH_NOALLOC_MAX_SORTED_RAYS
 BACKGROUND_CONSTANT, BACKGROUND_SKY = 0, 1
 TRAIN_GRADIENTS_ONLY = 1
 TRAIN_TABLES, TRAIN_MLP, TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP = 0, 1, 2, 3
@@ -197,6 +198,7 @@ SIGNATURES = {
     "nbvh_scene_get_network": [P, ctypes.POINTER(NetworkConfig), P, U64, P, U64],
     "nbvh_raygen": [P, ctypes.POINTER(Camera), U32, U32, U32, U32, P, P],
     "nbvh_query": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
+    "nbvh_query_noalloc": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P],
     "nbvh_scatter_diffuse": [P, P, P, P, U32, U32, U32, P, P],
     "nbvh_segment_labels": [P, P, P, P, P, U32, P, P],
     "nbvh_query_host": [P, P, ctypes.POINTER(QueryParams), P, U32, P],

@@ -227,7 +227,7 @@ void ensure_part_tables(nbvh_ctx* ctx)
 }
 
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-                       hipStream_t stream, SlotLease* lease);
+                       hipStream_t stream, SlotLease* lease, uint32_t* fixed_counters = nullptr);
 void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                          hipStream_t stream);
 
@@ -360,8 +360,10 @@ void run_query_wavefront(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_pa
     hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
 }
 
+// fixed_counters: the query's counter words are there (the allocation-free entry point) and no
+// slot is taken; otherwise they come from `lease` (or a slot taken here when it is NULL).
 void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-                       hipStream_t stream, SlotLease* lease)
+                       hipStream_t stream, SlotLease* lease, uint32_t* fixed_counters)
 {
     require(p != nullptr, "query params are NULL");
     const uint32_t n = batch.n;
@@ -387,11 +389,11 @@ void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_para
     a.blocks_per_cu = s->shape[p->precision].blocks_per_cu;
     a.queue_bands   = s->shape[p->precision].queue_bands;
     std::unique_ptr<SlotLease> own;
-    if (lease == nullptr) {
+    if (lease == nullptr && fixed_counters == nullptr) {
         own.reset(new SlotLease(ctx, stream));
         lease = own.get();
     }
-    uint32_t* counter = lease->counters();
+    uint32_t* counter = fixed_counters ? fixed_counters : lease->counters();
     hip_check(zero_u32(counter, nbvh_ctx::kCounterWords, stream), "counter reset");
     bool supported = false;
     if (p->precision == NBVH_PREC_FP32)
@@ -485,6 +487,9 @@ NBVH_API int32_t nbvh_create(const nbvh_create_info* info, nbvh_ctx** out)
         hip_check(hipMalloc(&p, counter_bytes), "hipMalloc counters");
         ctx->d_counters = (uint32_t*)p;
         hip_check(hipMemset(p, 0, counter_bytes), "hipMemset counters");
+        // Device address of nbvh_query_noalloc's static scratch on this device (looked up once
+        // per device, with the device current).
+        hip_check(noalloc_scratch_init(dev), "hipGetSymbolAddress of the no-allocation scratch");
         *out = ctx.release();
     });
 }
@@ -927,6 +932,52 @@ NBVH_API int32_t nbvh_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query
         require(ctx && s, "NULL argument");
         require(s->ctx == ctx, "scene belongs to another context");
         run_query(ctx, s, p, RayBatch{d_rays, d_hits, nullptr, 0, n}, (hipStream_t)stream);
+    });
+}
+
+NBVH_API int32_t nbvh_query_noalloc(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p,
+                                    const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits, void* stream)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s && p, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        require((p->flags & ~(uint32_t)(NBVH_QUERY_SORT_RAYS | NBVH_QUERY_WAVEFRONT)) == 0u, "unknown query flags");
+        if ((p->flags & NBVH_QUERY_WAVEFRONT) != 0u)
+            fail(NBVH_ERR_UNSUPPORTED, "nbvh_query_noalloc: the wavefront-round schedule needs allocated scratch");
+        if (n == 0) return;
+        require(d_rays && d_hits, "ray or hit buffer is NULL");
+        const bool sort = (p->flags & NBVH_QUERY_SORT_RAYS) != 0u && n >= 2;
+        if (sort && n > NBVH_NOALLOC_MAX_SORTED_RAYS)
+            fail(NBVH_ERR_CAPACITY, "nbvh_query_noalloc: a sorted query takes at most " +
+                                        std::to_string(NBVH_NOALLOC_MAX_SORTED_RAYS) + " rays, got " + std::to_string(n));
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        char* base = static_cast<char*>(noalloc_scratch(ctx->device));
+        if (base == nullptr) fail(NBVH_ERR_STATE, "no-allocation scratch not set up on this device");
+        // Layout: [0, 256) the counter words; [256, 256 + sort bytes) keys, ids and the radix
+        // sort's storage; then the claim order (n words).
+        constexpr size_t kCounterBytes = 256;
+        static_assert(nbvh_ctx::kCounterWords * sizeof(uint32_t) <= kCounterBytes, "counter words exceed their area");
+        uint32_t* counters = reinterpret_cast<uint32_t*>(base);
+        const hipStream_t st = (hipStream_t)stream;
+        RayBatch batch{d_rays, d_hits, nullptr, 0, n};
+        if (!sort) {
+            run_query_ordered(ctx, s, p, batch, st, nullptr, counters);
+            return;
+        }
+        const size_t sort_bytes = (ray_order_scratch_bytes(n) + 255) / 256 * 256;
+        const size_t need = kCounterBytes + sort_bytes + (size_t)n * sizeof(uint32_t);
+        if (need > kNoAllocScratchBytes)
+            fail(NBVH_ERR_CAPACITY, "nbvh_query_noalloc: sorting " + std::to_string(n) + " rays needs " +
+                                        std::to_string(need) + " B of scratch, more than its " +
+                                        std::to_string(kNoAllocScratchBytes) + " B");
+        void* sort_mem = base + kCounterBytes;
+        uint32_t* order = reinterpret_cast<uint32_t*>(base + kCounterBytes + sort_bytes);
+        float ext[3];
+        for (int c = 0; c < 3; c++) ext[c] = s->bbox_max[c] - s->bbox_min[c];
+        hip_check(ray_order(batch, s->bbox_min, ext, order, sort_mem, sort_bytes, st), "ray order");
+        RayBatch sorted = batch;
+        sorted.order    = order;
+        run_query_ordered(ctx, s, p, sorted, st, nullptr, counters);
     });
 }
 

@@ -92,4 +92,11 @@ hipError_t treecut_update(const uint32_t* node_idx, const float* sample_loss, ui
                           uint32_t* vals, float* keys_sorted, uint32_t* vals_sorted, void* tmp, size_t tmp_bytes,
                           hipStream_t stream);
 
+// Static scratch of the allocation-free query (noalloc.hip): kNoAllocScratchBytes per device.
+// noalloc_scratch_init looks up the array's device address on `device` (which must be current)
+// once; noalloc_scratch returns it (NULL before the lookup).
+constexpr size_t kNoAllocScratchBytes = 24u << 20;
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+
 }  // namespace nbvh
