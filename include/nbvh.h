@@ -402,7 +402,11 @@ enum nbvh_background {
 };
 
 typedef struct nbvh_path_params {
-    nbvh_query_params query;      /* neural query of the primary rays (kind: NBVH_QUERY_NEURAL_CLOSEST) */
+    nbvh_query_params query;      /* query of the primary rays. kind NBVH_QUERY_NEURAL_CLOSEST: the
+                                   * object through its N-BVH and network; NBVH_QUERY_TRIANGLE_CLOSEST:
+                                   * the object's own triangles instead, diffuse with
+                                   * environment_albedo (the reference's ignore_neural_blas, the
+                                   * triangle rendering its validation compares with; no environment) */
     uint32_t bounce_query_flags;  /* nbvh_query_flags of the bounce rays (query.flags is for depth 1) */
     uint32_t max_depth;           /* path vertices, 1..16; primary hits have depth 1 (m_max_depth) */
     uint32_t rr_start_depth;      /* Russian roulette at depths >= this (rr_terminate,
@@ -417,7 +421,11 @@ typedef struct nbvh_path_params {
                                    * query.lod (path_lod_switch / lod_switch,
                                    * include/cuda/neural/neural_bvh_module_common.cuh:333-335) */
     uint32_t lod_switch;
-    uint32_t reserved[2];
+    float secondary_visibility_threshold; /* visibility threshold of the rays at depth > 1
+                                   * (secondary_visibility_threshold, neural_bvh_module.h:268-269, picked
+                                   * after the first vertex, neural_bvh_module_common.cuh:171-173);
+                                   * 0: query.visibility_threshold at every depth */
+    uint32_t reserved;
 } nbvh_path_params;
 
 typedef struct nbvh_path_stats {
@@ -503,6 +511,20 @@ int32_t nbvh_trainer_step(nbvh_trainer* trainer, const float* d_inputs, const fl
 int32_t nbvh_trainer_get(nbvh_trainer* trainer, uint32_t what, float* out, uint64_t capacity, uint64_t* count);
 /* Write the trained network into the scene. Synchronous; no query may run on the scene meanwhile. */
 int32_t nbvh_trainer_commit(nbvh_trainer* trainer);
+
+/* LDR-FLIP error of a rendering against a reference rendering, the image metric of the
+ * reference's validation mode (compute_and_display_flip_error,
+ * src/cuda/neural/neural_bvh_renderer.cu:40-92: both accumulation buffers divided by their sample
+ * counts, linear RGB converted to sRGB, FLIP, the mean of the error map). d_reference / d_test:
+ * width*height float4 accumulation buffers (rgb sum, sample count; as nbvh_path_trace writes them).
+ * The metric follows the published method (Andersson et al., "FLIP: A Difference Evaluator for
+ * Alternating Images", HPG 2020); sRGB values are clamped to [0, 1] (LDR-FLIP's input range) and
+ * image borders replicate the edge pixels. ppd: pixels per degree, 0 => the reference's 0.7 m wide
+ * 3840-pixel monitor at 0.7 m (67.02); at most about 118. d_error_map (optional): width*height
+ * floats. *mean_error: the mean error. Synchronises `stream`; transient stream-ordered memory of
+ * 32 B per pixel. */
+int32_t nbvh_flip(nbvh_ctx* ctx, const float* d_reference, const float* d_test, uint32_t width, uint32_t height,
+                  float ppd, float* d_error_map, float* mean_error, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Tree-cut learning: the reference's training loop (src/cuda/neural/neural_bvh_renderer.cu:

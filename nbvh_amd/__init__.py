@@ -18,7 +18,7 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
-__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "Learner", "NbvhError", "network_config", "icosphere", "bvh_build",
+__all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "Learner", "flip", "NbvhError", "network_config", "icosphere", "bvh_build",
            "nbvh_cut", "nbvh_parse", "nbvh_serialize", "geosphere", "sphere_field", "network_param_counts", "network_init_params",
            "camera_look_at", "tcnn_params_to_network", "network_to_tcnn_params", "read_tcnn_model", "leaf_segments", "hit_kind", "hit_flags", "hit_segments", "HIT_DTYPE", "NODE_DTYPE", "RAY_DTYPE"]
 
@@ -429,14 +429,18 @@ class NeuralBVHModule:
                           max_depth=5, rr_start_depth=0, sample=0, background=BACKGROUND_SKY,
                           background_color=(0.0, 0.0, 0.0), environment_albedo=(0.8, 0.8, 0.8), normal_offset=0.0,
                           sort_bounces=True, lod=0, bounce_lod=None, max_traversal_steps=0, visibility_threshold=0.0,
-                          stream=None):
+                          secondary_visibility_threshold=0.0, triangles=False, stream=None):
         """One frame of 1-spp diffuse path tracing into the device float4 buffer d_accum_ptr
         (width*height; rgb sum and sample count). `environment`: a triangle Scene traced with the
         neural object (hybrid), or None. bounce_lod: N-BVH LoD of the rays at depth > 1 (None: lod).
-        Returns the rays traced per depth (list)."""
+        secondary_visibility_threshold: threshold of the rays at depth > 1 (0: visibility_threshold).
+        triangles=True renders the object's own triangles with environment_albedo instead of its
+        N-BVH and network (the reference's ignore_neural_blas). Returns the rays traced per depth."""
         p = _ffi.PathParams()
-        p.query = self.params(QUERY_NEURAL_CLOSEST, precision, lod, max_traversal_steps, visibility_threshold,
+        p.query = self.params(QUERY_TRIANGLE_CLOSEST if triangles else QUERY_NEURAL_CLOSEST, precision, lod,
+                              max_traversal_steps, visibility_threshold,
                               image_width=width if width % 8 == 0 and height % 8 == 0 else 0)
+        p.secondary_visibility_threshold = secondary_visibility_threshold
         p.bounce_query_flags = _ffi.QUERY_SORT_RAYS if sort_bounces else 0
         p.max_depth, p.rr_start_depth, p.sample, p.background = max_depth, rr_start_depth, sample, background
         p.background_color[:] = [float(c) for c in background_color]
@@ -568,6 +572,22 @@ class Trainer:
         if self.handle:
             _lib().nbvh_trainer_destroy(self.handle)
             self.handle = ctypes.c_void_p()
+
+
+def flip(ctx, reference, test, ppd=0.0):
+    """LDR-FLIP of two host accumulation images [h, w, 4] (rgb sum, sample count), the reference's
+    validation metric: (mean error, error map [h, w]). ppd 0: the reference's viewing geometry."""
+    from . import hip
+    ref = np.ascontiguousarray(reference, np.float32)
+    tst = np.ascontiguousarray(test, np.float32)
+    if ref.shape != tst.shape or ref.ndim != 3 or ref.shape[2] != 4:
+        raise ValueError("flip takes two [h, w, 4] accumulation images of one size")
+    h, w = ref.shape[:2]
+    dr, dt, dm = hip.Buffer.from_array(ref), hip.Buffer.from_array(tst), hip.Buffer(h * w * 4)
+    mean = ctypes.c_float()
+    ctx.check(_lib().nbvh_flip(ctx.handle, ctypes.c_void_p(dr.ptr), ctypes.c_void_p(dt.ptr), w, h, ppd,
+                               ctypes.c_void_p(dm.ptr), ctypes.byref(mean), None))
+    return mean.value, dm.download(np.float32, h * w).reshape(h, w)
 
 
 class Learner:

@@ -83,7 +83,8 @@ class PathParams(ctypes.Structure):
                 ("rr_start_depth", ctypes.c_uint32), ("sample", ctypes.c_uint32), ("background", ctypes.c_uint32),
                 ("background_color", ctypes.c_float * 3), ("environment_albedo", ctypes.c_float * 3),
                 ("normal_offset", ctypes.c_float), ("path_lod_switch", ctypes.c_uint32),
-                ("lod_switch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("lod_switch", ctypes.c_uint32), ("secondary_visibility_threshold", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
 
 
 class PathStats(ctypes.Structure):
@@ -209,6 +210,7 @@ SIGNATURES = {
     "nbvh_trainer_step": [P, P, P, U32, U32, ctypes.POINTER(F32), P],
     "nbvh_trainer_get": [P, U32, P, U64, PU64],
     "nbvh_trainer_commit": [P],
+    "nbvh_flip": [P, P, P, U32, U32, F32, P, ctypes.POINTER(F32), P],
     "nbvh_learner_config_defaults": [ctypes.POINTER(LearnerConfig)],
     "nbvh_learner_create": [P, P, ctypes.POINTER(LearnerConfig), ctypes.POINTER(P)],
     "nbvh_learner_destroy": [P],

@@ -1136,7 +1136,14 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
     return guarded(ctx, [&] {
         require(ctx && neural && p && cam, "NULL argument");
         require(neural->ctx == ctx && (!environment || environment->ctx == ctx), "scene belongs to another context");
-        require(p->query.kind == NBVH_QUERY_NEURAL_CLOSEST, "path tracing takes NBVH_QUERY_NEURAL_CLOSEST");
+        const bool tri_object = p->query.kind == NBVH_QUERY_TRIANGLE_CLOSEST;
+        require(p->query.kind == NBVH_QUERY_NEURAL_CLOSEST || tri_object,
+                "path tracing takes NBVH_QUERY_NEURAL_CLOSEST or NBVH_QUERY_TRIANGLE_CLOSEST");
+        require(!(tri_object && environment), "a triangle rendering of the object takes no environment");
+        require(p->secondary_visibility_threshold == 0.f ||
+                    (p->secondary_visibility_threshold > 0.f && p->secondary_visibility_threshold < 1.f),
+                "secondary_visibility_threshold must be 0 or in (0, 1)");
+        if (tri_object && neural->bvh.empty()) fail(NBVH_ERR_STATE, "the object has no triangles");
         require(p->max_depth >= 1 && p->max_depth <= 16, "max_depth must be 1..16");
         require(p->background <= NBVH_BACKGROUND_SKY, "unknown background");
         require((uint64_t)width * height <= 0xFFFFFFFFull, "image too large");
@@ -1162,9 +1169,10 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
         nbvh_hit* hits = reinterpret_cast<nbvh_hit*>(mem + o_hits);
         uint32_t* cnt  = reinterpret_cast<uint32_t*>(mem + o_cnt);
         PathArgs a{};
-        if (environment)
-            a.tri = TriArgs{environment->d_verts.as<float>(), environment->d_tris.as<uint32_t>(),
-                            environment->d_bvh.as<nbvh_node>(), environment->d_prims.as<uint32_t>(), 0u};
+        const nbvh_scene* tri_scene = tri_object ? neural : environment;
+        if (tri_scene)
+            a.tri = TriArgs{tri_scene->d_verts.as<float>(), tri_scene->d_tris.as<uint32_t>(),
+                            tri_scene->d_bvh.as<nbvh_node>(), tri_scene->d_prims.as<uint32_t>(), 0u};
         a.accum          = d_accum;
         a.max_depth      = p->max_depth;
         a.rr_start_depth = p->rr_start_depth;
@@ -1187,6 +1195,8 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
                     q.flags       = p->bounce_query_flags;
                     q.image_width = 0;
                     if (p->path_lod_switch) q.lod = p->lod_switch;
+                    if (p->secondary_visibility_threshold > 0.f)
+                        q.visibility_threshold = p->secondary_visibility_threshold;
                 }
                 if (environment)
                     run_hybrid(ctx, neural, environment, &q, rays[c], cur, hits, st);

@@ -92,6 +92,11 @@ hipError_t treecut_update(const uint32_t* node_idx, const float* sample_loss, ui
                           uint32_t* vals, float* keys_sorted, uint32_t* vals_sorted, void* tmp, size_t tmp_bytes,
                           hipStream_t stream);
 
+// LDR-FLIP (flip.hip): the error map (optional) and its mean (device double) of two accumulation
+// buffers; flip_tiles(w, h) partial sums in a.partials.
+uint32_t flip_tiles(uint32_t w, uint32_t h);
+hipError_t flip_error(const FlipArgs& a, double* mean, hipStream_t stream);
+
 // Static scratch of the allocation-free query (noalloc.hip): kNoAllocScratchBytes per device.
 // noalloc_scratch_init looks up the array's device address on `device` (which must be current)
 // once; noalloc_scratch returns it (NULL before the lookup).

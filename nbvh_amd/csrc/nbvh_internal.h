@@ -147,6 +147,26 @@ struct SampleArgs {
     uint32_t*          failures;       // threads that gave up after the attempt cap
 };
 
+// LDR-FLIP (flip.hip). Accumulation buffers are float4 per pixel (rgb sum, sample count); the
+// prepared buffers hold (Yy, Cx, Cz, Y / Yn) per pixel; kernels are (2r+1)^2 row-major with row =
+// y offset (the y feature kernels are the transposes of the x ones).
+constexpr int kFlipMaxRadius = 16;
+struct FlipArgs {
+    const float* ref;   // w*h float4
+    const float* test;
+    uint32_t     w, h;
+    int          rc, rf;  // colour (contrast sensitivity) and feature kernel radii
+    const float* csf;     // 3 kernels: achromatic, red-green, blue-yellow
+    const float* edge;    // x kernel of the edge detector
+    const float* point;   // x kernel of the point detector
+    float        cmax;    // HyAB^0.7 of Hunt-adjusted green against blue
+    float        rgb2xyz[9], xyz2rgb[9], white[3];
+    float*       prep_ref;   // w*h float4
+    float*       prep_test;
+    float*       map;        // optional w*h error map
+    double*      partials;   // one per 16x16 tile
+};
+
 // Adam step constants (the bias correction uses each parameter's own update count).
 struct AdamArgs {
     float lr, beta1, beta2, epsilon, l2_reg;

@@ -37,7 +37,7 @@ def setup(nb, orc, ctx):
     env = nb.Scene(ctx, ev, et)
     enodes, eprims = env.bvh()
     yield dict(scene=scene, net=net, nbvh=scene.nbvh(), mod=nb.NeuralBVHModule(ctx, scene), env=env, ev=ev, et=et,
-               enodes=enodes, eprims=eprims)
+               enodes=enodes, eprims=eprims, v=v, t=t, nodes=nodes, prims=prims)
     env.close()
     scene.close()
 
@@ -75,6 +75,41 @@ def test_hybrid_path_trace_fp32_matches_oracle(nb, orc, setup):
     ref, rc = orc.path_trace(q, nb.camera_array(cam), W, H, max_depth=4, rr_start_depth=3, env_albedo=albedo,
                              verts=setup["ev"], tris=setup["et"])
     compare(img, ref, counts[0], rc)
+
+
+def test_triangle_rendering_of_the_object_matches_oracle(nb, orc, setup):
+    """The reference's ignore_neural_blas rendering (the object's own triangles, one albedo): the
+    image the validation mode compares the neural rendering with."""
+    cam = nb.camera_look_at((0.4, 0.3, 2.8), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    albedo = (0.9, 0.9, 0.9)
+    img, counts = setup["mod"].path_trace(cam, W, H, triangles=True, max_depth=4, rr_start_depth=3,
+                                          environment_albedo=albedo)
+
+    def q(r):
+        return orc.tri_query(setup["v"], setup["t"], setup["nodes"], setup["prims"], r)
+    ref, rc = orc.path_trace(q, nb.camera_array(cam), W, H, max_depth=4, rr_start_depth=3, env_albedo=albedo,
+                             verts=setup["v"], tris=setup["t"])
+    compare(img, ref, counts[0], rc)
+    assert counts[0][1] > 0  # bounces off the triangles
+    with pytest.raises(nb.NbvhError):  # an environment is not combined with the triangle object
+        setup["mod"].path_trace(cam, W, H, triangles=True, environment=setup["env"])
+
+
+def test_secondary_visibility_threshold(nb, orc, setup):
+    """primary_visibility_threshold at depth 1, secondary_visibility_threshold after
+    (neural_bvh_module_common.cuh:171-173)."""
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    img, counts = setup["mod"].path_trace(cam, W, H, precision=nb.PREC_FP32, max_depth=3,
+                                          visibility_threshold=0.5, secondary_visibility_threshold=0.500004)
+
+    def q(r, depth):
+        thr = 0.5 if depth == 1 else 0.500004
+        return setup["net"].neural_query(setup["nbvh"], r, mode=orc.FP32, threshold=thr)
+    ref, rc = orc.path_trace(q, nb.camera_array(cam), W, H, max_depth=3, query_takes_depth=True)
+    compare(img, ref, counts[0], rc)
+    base, bc = setup["mod"].path_trace(cam, W, H, precision=nb.PREC_FP32, max_depth=3)
+    # the bounce threshold changes what the bounces hit, not the primary rays
+    assert bc[0][:2] == counts[0][:2] and bc[0][2] != counts[0][2]
 
 
 def test_path_trace_deterministic_and_accumulating(nb, setup):
