@@ -277,11 +277,59 @@ public:
         return loss;
     }
     void commit() { check(nbvh_trainer_commit(t_), ctx_); }
+    nbvh_trainer* get() const { return t_; }
 
 private:
     nbvh_ctx* ctx_ = nullptr;
     nbvh_trainer* t_ = nullptr;
 };
+
+// Tree-cut learning (the reference's training loop around the trainer): the learner grows the
+// scene's N-BVH from the cut at cfg.neural_bvh_min_depth while the trainer trains the network.
+class Learner {
+public:
+    static nbvh_learner_config defaults()
+    {
+        nbvh_learner_config c{};
+        check(nbvh_learner_config_defaults(&c));
+        return c;
+    }
+    Learner(Context& ctx, Scene& scene, const nbvh_learner_config& cfg) : ctx_(ctx.get())
+    {
+        check(nbvh_learner_create(ctx_, scene.get(), &cfg, &l_), ctx_);
+    }
+    ~Learner() { nbvh_learner_destroy(l_); }
+    Learner(const Learner&) = delete;
+    Learner& operator=(const Learner&) = delete;
+
+    // One step: sample the training segments, train on them, split leaves on the schedule.
+    float step(Trainer& trainer, void* stream = nullptr)
+    {
+        float loss = 0.f;
+        check(nbvh_learner_step(l_, trainer.get(), &loss, stream), ctx_);
+        return loss;
+    }
+    nbvh_learner_state state() const
+    {
+        nbvh_learner_state s{};
+        check(nbvh_learner_get_state(l_, &s), ctx_);
+        return s;
+    }
+
+private:
+    nbvh_ctx* ctx_ = nullptr;
+    nbvh_learner* l_ = nullptr;
+};
+
+// LDR-FLIP mean error of a rendering against a reference rendering (device float4 accumulation
+// buffers); ppd 0: the reference's viewing geometry.
+inline float flip(Context& ctx, const float* d_reference, const float* d_test, uint32_t width, uint32_t height,
+                  float* d_error_map = nullptr, float ppd = 0.f, void* stream = nullptr)
+{
+    float mean = 0.f;
+    check(nbvh_flip(ctx.get(), d_reference, d_test, width, height, ppd, d_error_map, &mean, stream), ctx.get());
+    return mean;
+}
 
 // Path-trace defaults of the reference's path module: max depth 5 (4 bounces), sky background.
 inline nbvh_path_params path_params(uint32_t precision = NBVH_PREC_FP16, uint32_t max_depth = 5,

@@ -7,6 +7,10 @@
 //                                         the hit records (48 B each) to --out
 // options: --width W --height H --precision fp32|fp16|bf16 --kind closest|any|tri|tri-any
 //          --leaves K --seed S --out FILE --repeat R (prints ms per query)
+//   nbvh_driver path [options]            path-traced frames of the icosphere workload
+//   nbvh_driver learn [--steps N --batch B --width W --height H --spp S]
+//                                         tree-cut learning + training from the N-BVH root, then
+//                                         FLIP of the neural against the triangle rendering
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -199,18 +203,76 @@ int path(int argc, char** argv)
     return 0;
 }
 
+// The reference's way of producing a network, from C++: tree-cut learning of the icosphere's
+// N-BVH from its root with the training step, then the FLIP of a neural rendering against the
+// triangle rendering of the object (the validation mode's comparison).
+int learn(int argc, char** argv)
+{
+    uint32_t steps = 1000, batch = 1u << 16, W = 320, H = 240, spp = 4;
+    for (int i = 2; i + 1 < argc; i += 2) {
+        const std::string a = argv[i], v = argv[i + 1];
+        if (a == "--steps") steps = (uint32_t)std::stoul(v);
+        else if (a == "--batch") batch = (uint32_t)std::stoul(v);
+        else if (a == "--width") W = (uint32_t)std::stoul(v);
+        else if (a == "--height") H = (uint32_t)std::stoul(v);
+        else if (a == "--spp") spp = (uint32_t)std::stoul(v);
+        else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 2;
+        }
+    }
+    nbvh::Context ctx;
+    nbvh::Mesh m = nbvh::icosphere(6, 7, 0.1f);
+    nbvh::Scene scene(ctx, m);
+    scene.init_network(nbvh::network_config(8, 4, 18, 8, 2.f), 1, 1e-4f);  // the shipped chess network shape
+    nbvh_learner_config cfg = nbvh::Learner::defaults();
+    cfg.batch_size           = batch;
+    cfg.max_training_steps   = steps;
+    cfg.last_split_iteration = (int32_t)(steps * 3 / 4);
+    cfg.splitting_interval   = 25;
+    nbvh::Trainer trainer(ctx, scene, batch);
+    nbvh::Learner learner(ctx, scene, cfg);
+    const auto t0 = std::chrono::steady_clock::now();
+    float loss = 0.f;
+    for (uint32_t s = 0; s < steps; s++) loss = learner.step(trainer);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    trainer.commit();
+    const nbvh_learner_state st = learner.state();
+    std::printf("learned %u steps in %.1f ms (%.3f ms/step): loss %.5f, N-BVH %u nodes (%u splits), "
+                "%u sampler failures\n", steps, ms, ms / steps, loss, st.n_nodes, st.total_splits, st.sample_failures);
+    nbvh::NeuralBVHModule module(ctx, scene);
+    const float eye[3] = {0, 0, 3}, target[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    const nbvh_camera cam = nbvh::camera_look_at(eye, target, up, 45.f, W, H);
+    float *d_neural = nullptr, *d_tri = nullptr;
+    hip_ok(hipMalloc((void**)&d_neural, (size_t)W * H * 16), "hipMalloc accum");
+    hip_ok(hipMalloc((void**)&d_tri, (size_t)W * H * 16), "hipMalloc accum");
+    for (uint32_t s = 0; s < spp; s++) {
+        nbvh_path_params p = nbvh::path_params(NBVH_PREC_FP16, 3, 2, s);
+        module.neural_path_tracing(cam, W, H, d_neural, p);
+        p.query.kind = NBVH_QUERY_TRIANGLE_CLOSEST;  // the object's own triangles (ignore_neural_blas)
+        for (int c = 0; c < 3; c++) p.environment_albedo[c] = 0.9f;
+        module.neural_path_tracing(cam, W, H, d_tri, p);
+    }
+    const float err = nbvh::flip(ctx, d_tri, d_neural, W, H);
+    std::printf("FLIP of the neural rendering against the triangle rendering (%ux%u, %u spp): %.4f\n", W, H, spp, err);
+    (void)hipFree(d_neural);
+    (void)hipFree(d_tri);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: nbvh_driver host | query [options] | path [options]\n");
+        std::fprintf(stderr, "usage: nbvh_driver host | query [options] | path [options] | learn [options]\n");
         return 2;
     }
     try {
         if (std::strcmp(argv[1], "host") == 0) return host_selftest();
         if (std::strcmp(argv[1], "query") == 0) return query(argc, argv);
         if (std::strcmp(argv[1], "path") == 0) return path(argc, argv);
+        if (std::strcmp(argv[1], "learn") == 0) return learn(argc, argv);
     } catch (const nbvh::Error& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;
