@@ -358,10 +358,16 @@ def _sig(z):
     return 1.0 / (1.0 + np.exp(-np.asarray(z, np.float64)))
 
 
+def sig4(v):
+    """A reported error to 4 significant digits (never rounded to zero)."""
+    return float(f"{float(v):.4g}")
+
+
 def output_errors(z, z64):
     """Per-output error of network logits z against FP64 logits z64, in what the query uses:
-    visibility decision flips, |depth - depth64| (sigmoid output, LocalT fraction of the segment),
-    normal angle (degrees) and |albedo - albedo64| (sigmoid outputs)."""
+    visibility decision flips; |depth - depth64| (sigmoid output, LocalT fraction of the segment);
+    normal angle (degrees); |albedo - albedo64| (sigmoid outputs); and the logits' error relative
+    to each output's range over the sample (p99 over segments, worst output of each group)."""
     s, s64 = _sig(z), _sig(z64)
     n = 2 * s[:, 2:5] - 1
     n64 = 2 * s64[:, 2:5] - 1
@@ -370,9 +376,14 @@ def output_errors(z, z64):
     ang = np.degrees(np.arccos(np.clip(np.einsum("ij,ij->i", n, n64), -1, 1)))
     dd = np.abs(s[:, 1] - s64[:, 1])
     da = np.abs(s[:, 5:8] - s64[:, 5:8]).max(axis=1)
-    return {"vis_flip_rate": float(np.mean((z[:, 0] > 0) != (z64[:, 0] > 0))),
-            "depth_abs_p99": float(np.percentile(dd, 99)), "depth_abs_max": float(dd.max()),
-            "normal_deg_p99": float(np.percentile(ang, 99)), "albedo_abs_p99": float(np.percentile(da, 99))}
+    rng = np.maximum(z64.max(axis=0) - z64.min(axis=0), 1e-30)
+    rel = np.percentile(np.abs(np.asarray(z, np.float64) - z64) / rng, 99, axis=0)
+    out = {"vis_flip_rate": float(np.mean((z[:, 0] > 0) != (z64[:, 0] > 0))),
+           "depth_abs_p99": float(np.percentile(dd, 99)), "depth_abs_max": float(dd.max()),
+           "normal_deg_p99": float(np.percentile(ang, 99)), "albedo_abs_p99": float(np.percentile(da, 99)),
+           "logit_rel_range_p99": {"visibility": float(rel[0]), "depth": float(rel[1]), "normal": float(rel[2:5].max()),
+                                   "albedo": float(rel[5:8].max())}}
+    return {k: ({kk: sig4(vv) for kk, vv in v.items()} if isinstance(v, dict) else sig4(v)) for k, v in out.items()}
 
 
 def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, every=64):
@@ -426,10 +437,15 @@ def path_traced_bench(nb, hip, ctx, scene, oracle, W=1920, H=1080, bounces=4, ev
             for name, prec in (("fp16", nb.PREC_FP16), ("bf16", nb.PREC_BF16), ("fp8", nb.PREC_FP8)):
                 _, z = mod.network_inference(x, precision=prec)
                 err = output_errors(z, z64)
-                row[name] = {k: round(v, 5) for k, v in err.items()}
-                w = worst.setdefault(name, {k: 0.0 for k in err})
+                row[name] = err
+                w = worst.setdefault(name, {})
                 for k, v in err.items():
-                    w[k] = round(max(w[k], v), 5)
+                    if isinstance(v, dict):
+                        wk = w.setdefault(k, {})
+                        for kk, vv in v.items():
+                            wk[kk] = max(wk.get(kk, 0.0), vv)
+                    else:
+                        w[k] = max(w.get(k, 0.0), v)
         rows.append(row)
         log_msg = f"path-traced bounce {b}: {cur} rays, fp16 {row['fp16_mrays_per_s']} / fp32 " \
                   f"{row['fp32_mrays_per_s']} Mrays/s"
@@ -703,6 +719,81 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
     }
 
 
+def precision_report(nb, hip, ctx, scene, oracle, module, d_rays, d_hits, stream, W, H, every_row=16, steps=10):
+    """On one scene and its 1024x1024 primary rays: per precision the fused query's time, and on
+    every 16th image row its hit/miss decisions against the FP64 oracle query of the same rays;
+    on those rows' FP64 segments, the per-output errors of the network kernels."""
+    cfg, tables, mlp = scene.network()
+    bmin, bmax = scene.bbox()
+    net = oracle.Net(cfg, tables, mlp, bmin, bmax)
+    rays = d_rays.download(nb.RAY_DTYPE)
+    rows = np.arange(0, H, every_row)
+    idx = (rows[:, None] * W + np.arange(W)[None, :]).reshape(-1)
+    e64, seglog = net.neural_query(scene.nbvh(), rays[idx], mode=oracle.FP64, log_segments=True)
+    x = net.encode(rays[idx][seglog["ray"]], seglog["mint"], seglog["maxt"], mode=oracle.FP32)
+    z64 = seglog["logit"].astype(np.float64)
+    out = {"sample": f"every {every_row}th row ({len(idx)} rays, {len(x)} FP64 segments)",
+           "hit_fraction_fp64": sig4((nb.hit_kind(e64) != 0).mean())}
+    for name, prec in (("fp32", nb.PREC_FP32), ("fp16", nb.PREC_FP16), ("bf16", nb.PREC_BF16), ("fp8", nb.PREC_FP8)):
+        p = module.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, image_width=W)
+        module.query_device(p, d_rays.ptr, W * H, d_hits.ptr, stream=stream.handle)
+        ev = [hip.Event() for _ in range(steps + 1)]
+        ev[0].record(stream)
+        for i in range(steps):
+            module.query_device(p, d_rays.ptr, W * H, d_hits.ptr, stream=stream.handle)
+            ev[i + 1].record(stream)
+        stream.synchronize()
+        ms = float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]))
+        hits = d_hits.download(nb.HIT_DTYPE)
+        row = {"ms": round(ms, 4), "mrays_per_s": round(W * H / (ms * 1e-3) / 1e6, 1),
+               "query_kind_mismatch_vs_fp64": sig4(np.mean(nb.hit_kind(hits[idx]) != nb.hit_kind(e64)))}
+        if prec != nb.PREC_FP32 and len(x):
+            _, z = module.network_inference(x, precision=prec)
+            row["network_errors_vs_fp64"] = output_errors(z, z64)
+        out[name] = row
+    for name in ("fp16", "bf16", "fp8"):
+        out[name]["speedup_vs_fp32"] = round(out["fp32"]["ms"] / out[name]["ms"], 3)
+    return out
+
+
+def trained_network_bench(nb, hip, ctx, oracle, W=1024, H=1024, steps=1500, batch=1 << 16):
+    """The headline network trained the reference's way (tree-cut learning of the N-BVH from its
+    root with the training step, nbvh_learner_* + nbvh_trainer_*), then the precision report on
+    that network and learned N-BVH."""
+    v, t = nb.icosphere(6, seed=7, amplitude=0.1)
+    scene = nb.Scene(ctx, v, t)
+    scene.init_network(nb.network_config(), seed=1, table_range=1e-4)
+    lr = nb.Learner(ctx, scene, batch_size=batch, max_training_steps=steps, splitting_interval=25,
+                    last_split_iteration=int(steps * 0.75))
+    tr = nb.Trainer(ctx, scene, max_batch=batch)
+    losses = []
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        losses.append(lr.step(tr))
+    el = time.perf_counter() - t0
+    tr.commit()
+    st = lr.state()
+    tr.close()
+    lr.close()
+    log(f"trained network: {steps} learner steps in {el:.2f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}, "
+        f"N-BVH {st['n_nodes']} nodes")
+    module = nb.NeuralBVHModule(ctx, scene)
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    stream = hip.Stream()
+    d_rays, d_hits = hip.Buffer(W * H * 32), hip.Buffer(W * H * 48)
+    ctx.raygen(cam, W, H, d_rays.ptr, stream=stream.handle)
+    stream.synchronize()
+    rep = precision_report(nb, hip, ctx, scene, oracle, module, d_rays, d_hits, stream, W, H)
+    stream.destroy()
+    scene.close()
+    return {"workload": f"headline network and icosphere, trained from the N-BVH root by {steps} learner steps of "
+                        f"{batch} samples (tree-cut learning + training step), {W}x{H} primary rays",
+            "learner_ms_per_step": round(el / steps * 1e3, 3), "loss_first": sig4(losses[0]),
+            "loss_last": sig4(np.mean(losses[-20:])), "nbvh_nodes": st["n_nodes"],
+            "sample_failures": st["sample_failures"], "precisions": rep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -827,6 +918,15 @@ def main():
         result["vis_flip_rate"] = flips
         result["vis_flip_sample_segments"] = int(len(x))
         log(f"visibility flips vs FP64 oracle: {flips:.5f} over {len(x)} segments")
+        # The fused query's own hit/miss decisions against the FP64 oracle query, per precision,
+        # with BF16 and FP8 timed beside FP16 and FP32.
+        rep = precision_report(nb, hip, ctx, scene, oracle, module, d_rays, d_hits, stream, W, H)
+        result["precisions"] = rep
+        result["fp16_query_kind_mismatch"] = rep["fp16"]["query_kind_mismatch_vs_fp64"]
+        log("precisions: " + ", ".join(f"{k} {rep[k]['mrays_per_s']} Mrays/s mismatch "
+                                       f"{rep[k]['query_kind_mismatch_vs_fp64']}" for k in ("fp32", "fp16", "bf16", "fp8")))
+        if not args.no_train:
+            result["trained_network"] = trained_network_bench(nb, hip, ctx, oracle)
 
         if not args.no_cpu_baseline:
             result["plain_triangles_512"] = plain_triangles(nb, hip, ctx, scene, oracle)
