@@ -543,7 +543,9 @@ def path_trace_driver_bench(nb, hip, ctx, scene, W=1920, H=1080, max_depth=5, rr
     """Native path-trace driver (nbvh_path_trace): 1920x1080 at 1 spp, 5 path vertices (4 diffuse
     bounces), Russian roulette from depth 3, sky background, device-side compaction, bounces in
     coherence order. Neural icosphere alone and inside the triangle environment (hybrid).
-    Frame time with HIP events around the whole call (it synchronises once per depth)."""
+    Frame time with HIP events around the whole call (live-path counts stay on the device; the
+    per-depth counts are read back once at the end), and of the same frame captured into a HIP
+    graph (caller workspace, no read-back) and replayed; the replay must equal the eager frame."""
     env = nb.Scene(ctx, *hybrid_environment(nb))
     mod = nb.NeuralBVHModule(ctx, scene)
     acc = hip.Buffer(W * H * 16)
@@ -572,10 +574,31 @@ def path_trace_driver_bench(nb, hip, ctx, scene, W=1920, H=1080, max_depth=5, rr
             out[f"{scene_name}_{name}"] = {"ms_per_frame": round(m, 3), "rays_per_depth": counts,
                                            "mrays_per_s": round(sum(counts) / (m * 1e-3) / 1e6, 1),
                                            "mean_radiance": round(float((img[:, :3] / img[:, 3:4]).mean()), 5)}
+            # The frame of sample 0 recorded into one HIP graph and replayed.
+            pk = dict(precision=prec, max_depth=max_depth, rr_start_depth=rr_start_depth, sample=0)
+            ws = hip.Buffer(mod.path_trace_workspace_size(mod.path_params(W, H, **pk), W, H, environment))
+            pg = mod.path_params(W, H, workspace=ws, **pk)
+            mod.path_trace_params_device(pg, cam, W, H, acc.ptr, environment, stream=s.handle)
+            s.synchronize()
+            ref = acc.download(np.float32)
+            with hip.Graph.capture(s) as g:
+                mod.path_trace_params_device(pg, cam, W, H, acc.ptr, environment, stream=s.handle, stats=False)
+            ev = [hip.Event() for _ in range(frames + 1)]
+            ev[0].record(s)
+            for f in range(frames):
+                g.launch(s)
+                ev[f + 1].record(s)
+            s.synchronize()
+            gms = float(np.median([ev[f].elapsed_ms(ev[f + 1]) for f in range(frames)]))
+            out[f"{scene_name}_{name}"]["graph_ms_per_frame"] = round(gms, 3)
+            out[f"{scene_name}_{name}"]["graph_equals_eager"] = bool(np.array_equal(acc.download(np.float32), ref))
+            g.destroy()
+            ws.free()
     s.destroy()
     env.close()
-    log("path-trace driver: " + ", ".join(f"{k} {v['ms_per_frame']} ms ({v['mrays_per_s']} Mrays/s)"
-                                          for k, v in out.items() if isinstance(v, dict)))
+    log("path-trace driver: " + ", ".join(f"{k} {v['ms_per_frame']} ms ({v['mrays_per_s']} Mrays/s, graph "
+                                          f"{v['graph_ms_per_frame']} ms)" for k, v in out.items()
+                                          if isinstance(v, dict)))
     return out
 
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NBVH_ABI_VERSION 1u
+#define NBVH_ABI_VERSION 2u
 
 typedef enum nbvh_status {
     NBVH_OK                  = 0,
@@ -426,6 +426,10 @@ typedef struct nbvh_path_params {
                                    * after the first vertex, neural_bvh_module_common.cuh:171-173);
                                    * 0: query.visibility_threshold at every depth */
     uint32_t reserved;
+    void*    workspace;           /* device memory of at least nbvh_path_trace_workspace_size bytes
+                                   * for the frame's buffers; NULL: transient stream-ordered memory
+                                   * (not while the stream is captured) */
+    uint64_t workspace_bytes;
 } nbvh_path_params;
 
 typedef struct nbvh_path_stats {
@@ -445,11 +449,24 @@ typedef struct nbvh_path_stats {
  * once, so the frame is deterministic. Random numbers: the raygen LCG with frame = depth and
  * linear = pixel (two draws for the direction, one for Russian roulette), independent of the
  * compaction order. Primary rays: nbvh_raygen with (sample, frame 0).
- * Synchronises `stream` once per depth (the live-path count); uses transient stream-ordered
- * device memory of about 180 B per pixel. stats may be NULL. */
+ * The live-path counts stay on the device: every depth's launches are sized for width*height
+ * rays and process the live ones, so the frame does not synchronise `stream` (the reference reads
+ * the count back every step, src/cuda/neural/path/path_module.cu:124-129). With a workspace the
+ * call allocates nothing and may be captured into a HIP graph (stats NULL; replays rerun the
+ * frame with the parameters captured). stats may be NULL; when given, the call reads the counts
+ * back once at the end (one synchronisation). bounce_query_flags with NBVH_QUERY_WAVEFRONT read
+ * each depth's count back instead (no capture). */
 int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
                         const nbvh_path_params* params, const nbvh_camera* camera, uint32_t width, uint32_t height,
                         float* d_accum, nbvh_path_stats* stats, void* stream);
+
+/* Bytes of device memory a frame of nbvh_path_trace with these arguments needs as its workspace
+ * (about 180 B per pixel, more for hybrid scenes and sorted queries). No counterpart in the
+ * reference, which sizes its path buffers at renderer set-up
+ * (src/cuda/neural/path/path_module.cu:139-245). */
+int32_t nbvh_path_trace_workspace_size(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
+                                       const nbvh_path_params* params, uint32_t width, uint32_t height,
+                                       uint64_t* bytes);
 
 /* ---------------------------------------------------------------------------------------------
  * Training: the replacement of the reference's trainer->training_step(stream, inputs[9 x n],

@@ -240,6 +240,26 @@ public:
               ctx_);
         return stats;
     }
+    // The same frame without reading the per-depth counts back (no synchronisation): with
+    // params.workspace set it allocates nothing and may be captured into a HIP graph.
+    void neural_path_tracing_async(const nbvh_camera& camera, uint32_t width, uint32_t height, float* d_accum,
+                                   const nbvh_path_params& params, const Scene* environment = nullptr,
+                                   void* stream = nullptr)
+    {
+        check(nbvh_path_trace(ctx_, scene_, environment ? environment->get() : nullptr, &params, &camera, width,
+                              height, d_accum, nullptr, stream),
+              ctx_);
+    }
+    // Device bytes params.workspace must hold for a frame of width x height.
+    uint64_t path_trace_workspace_size(const nbvh_path_params& params, uint32_t width, uint32_t height,
+                                       const Scene* environment = nullptr) const
+    {
+        uint64_t bytes = 0;
+        check(nbvh_path_trace_workspace_size(ctx_, scene_, environment ? environment->get() : nullptr, &params, width,
+                                             height, &bytes),
+              ctx_);
+        return bytes;
+    }
 
 private:
     nbvh_ctx* ctx_;

@@ -425,36 +425,60 @@ class NeuralBVHModule:
         hip.synchronize()
         return dh.download(HIT_DTYPE)
 
-    def path_trace_device(self, camera, width, height, d_accum_ptr, environment=None, precision=PREC_FP32,
-                          max_depth=5, rr_start_depth=0, sample=0, background=BACKGROUND_SKY,
-                          background_color=(0.0, 0.0, 0.0), environment_albedo=(0.8, 0.8, 0.8), normal_offset=0.0,
-                          sort_bounces=True, lod=0, bounce_lod=None, max_traversal_steps=0, visibility_threshold=0.0,
-                          secondary_visibility_threshold=0.0, triangles=False, stream=None):
-        """One frame of 1-spp diffuse path tracing into the device float4 buffer d_accum_ptr
-        (width*height; rgb sum and sample count). `environment`: a triangle Scene traced with the
-        neural object (hybrid), or None. bounce_lod: N-BVH LoD of the rays at depth > 1 (None: lod).
-        secondary_visibility_threshold: threshold of the rays at depth > 1 (0: visibility_threshold).
-        triangles=True renders the object's own triangles with environment_albedo instead of its
-        N-BVH and network (the reference's ignore_neural_blas). Returns the rays traced per depth."""
+    def path_params(self, width, height, precision=PREC_FP32, max_depth=5, rr_start_depth=0, sample=0,
+                    background=BACKGROUND_SKY, background_color=(0.0, 0.0, 0.0), environment_albedo=(0.8, 0.8, 0.8),
+                    normal_offset=0.0, sort_bounces=True, lod=0, bounce_lod=None, max_traversal_steps=0,
+                    visibility_threshold=0.0, secondary_visibility_threshold=0.0, triangles=False,
+                    wavefront_bounces=False, workspace=None):
+        """nbvh_path_params of a frame (see path_trace_device). workspace: a device buffer of at
+        least path_trace_workspace_size bytes (hip.Buffer), or None."""
         p = _ffi.PathParams()
         p.query = self.params(QUERY_TRIANGLE_CLOSEST if triangles else QUERY_NEURAL_CLOSEST, precision, lod,
                               max_traversal_steps, visibility_threshold,
                               image_width=width if width % 8 == 0 and height % 8 == 0 else 0)
         p.secondary_visibility_threshold = secondary_visibility_threshold
-        p.bounce_query_flags = _ffi.QUERY_SORT_RAYS if sort_bounces else 0
+        p.bounce_query_flags = ((_ffi.QUERY_SORT_RAYS if sort_bounces else 0) |
+                                (_ffi.QUERY_WAVEFRONT if wavefront_bounces else 0))
         p.max_depth, p.rr_start_depth, p.sample, p.background = max_depth, rr_start_depth, sample, background
         p.background_color[:] = [float(c) for c in background_color]
         p.environment_albedo[:] = [float(c) for c in environment_albedo]
         p.normal_offset = normal_offset
         if bounce_lod is not None:
             p.path_lod_switch, p.lod_switch = 1, bounce_lod
-        stats = _ffi.PathStats()
+        if workspace is not None:
+            p.workspace, p.workspace_bytes = workspace.ptr, workspace.nbytes
+        return p
+
+    def path_trace_workspace_size(self, params, width, height, environment=None):
+        """Device bytes nbvh_path_trace needs as its workspace for these parameters."""
+        b = ctypes.c_uint64()
+        self.ctx.check(_lib().nbvh_path_trace_workspace_size(self.ctx.handle, self.scene.handle,
+                                                             environment.handle if environment is not None else None,
+                                                             ctypes.byref(params), width, height, ctypes.byref(b)))
+        return int(b.value)
+
+    def path_trace_params_device(self, params, camera, width, height, d_accum_ptr, environment=None, stream=None,
+                                 stats=True):
+        """nbvh_path_trace with ready-made params; returns the rays traced per depth (stats=True) or
+        None (stats=False: no read-back, as under graph capture)."""
+        st = _ffi.PathStats() if stats else None
         self.ctx.check(_lib().nbvh_path_trace(self.ctx.handle, self.scene.handle,
                                               environment.handle if environment is not None else None,
-                                              ctypes.byref(p), ctypes.byref(camera), width, height,
-                                              ctypes.c_void_p(d_accum_ptr), ctypes.byref(stats),
+                                              ctypes.byref(params), ctypes.byref(camera), width, height,
+                                              ctypes.c_void_p(d_accum_ptr), ctypes.byref(st) if st is not None else None,
                                               ctypes.c_void_p(stream or 0)))
-        return [int(r) for r in stats.rays[:max_depth]]
+        return [int(r) for r in st.rays[:params.max_depth]] if st is not None else None
+
+    def path_trace_device(self, camera, width, height, d_accum_ptr, environment=None, stream=None, **kw):
+        """One frame of 1-spp diffuse path tracing into the device float4 buffer d_accum_ptr
+        (width*height; rgb sum and sample count). `environment`: a triangle Scene traced with the
+        neural object (hybrid), or None. Keywords: those of path_params -- bounce_lod: N-BVH LoD of
+        the rays at depth > 1 (None: lod); secondary_visibility_threshold: threshold of the rays at
+        depth > 1 (0: visibility_threshold); triangles=True renders the object's own triangles with
+        environment_albedo instead of its N-BVH and network (the reference's ignore_neural_blas).
+        Returns the rays traced per depth."""
+        p = self.path_params(width, height, **kw)
+        return self.path_trace_params_device(p, camera, width, height, d_accum_ptr, environment, stream)
 
     def path_trace(self, camera, width, height, samples=1, **kw):
         """Host image [height, width, 4] (rgb sums, sample count) of `samples` frames."""

@@ -84,7 +84,7 @@ class PathParams(ctypes.Structure):
                 ("background_color", ctypes.c_float * 3), ("environment_albedo", ctypes.c_float * 3),
                 ("normal_offset", ctypes.c_float), ("path_lod_switch", ctypes.c_uint32),
                 ("lod_switch", ctypes.c_uint32), ("secondary_visibility_threshold", ctypes.c_float),
-                ("reserved", ctypes.c_uint32)]
+                ("reserved", ctypes.c_uint32), ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_uint64)]
 
 
 class PathStats(ctypes.Structure):
@@ -221,6 +221,7 @@ SIGNATURES = {
     "nbvh_learner_get": [P, U32, P, U64, PU64],
     "nbvh_path_trace": [P, P, P, ctypes.POINTER(PathParams), ctypes.POINTER(Camera), U32, U32, P,
                         ctypes.POINTER(PathStats), P],
+    "nbvh_path_trace_workspace_size": [P, P, P, ctypes.POINTER(PathParams), U32, U32, PU64],
     "nbvh_scene_set_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_scene_get_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_autotune": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P, ctypes.POINTER(LaunchShape),

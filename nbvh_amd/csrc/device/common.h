@@ -506,6 +506,14 @@ __device__ __forceinline__ uint32_t batch_part(const RayBatch& b, uint32_t i)
     return lo;
 }
 
+// Claims of the launch: n, or the device-side count when the batch carries one (at most n).
+__device__ __forceinline__ uint32_t batch_count(const RayBatch& b)
+{
+    if (b.live == nullptr) return b.n;
+    const uint32_t c = *b.live;
+    return c < b.n ? c : b.n;
+}
+
 // Claim index -> ray index of the launch (the coherence order, when the batch carries one).
 __device__ __forceinline__ uint32_t batch_ray(const RayBatch& b, uint32_t claim)
 {

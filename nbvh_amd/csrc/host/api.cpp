@@ -249,13 +249,20 @@ NetArgs neural_args(const nbvh_scene* s, const nbvh_query_params* p)
     return a;
 }
 
+size_t query_scratch_bytes(uint32_t n)
+{
+    const size_t sort = std::max(ray_order_scratch_bytes(n), ray_bucket_scratch_bytes(n));
+    return (sort + 255) / 256 * 256 + (size_t)n * sizeof(uint32_t);
+}
+
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-               hipStream_t stream, SlotLease* lease)
+               hipStream_t stream, SlotLease* lease, void* scratch)
 {
     require(p != nullptr, "query params are NULL");
     require((p->flags & ~(uint32_t)(NBVH_QUERY_SORT_RAYS | NBVH_QUERY_WAVEFRONT)) == 0u, "unknown query flags");
     if ((p->flags & NBVH_QUERY_WAVEFRONT) != 0u &&
         (p->kind == NBVH_QUERY_NEURAL_CLOSEST || p->kind == NBVH_QUERY_NEURAL_ANY)) {
+        if (batch.live) fail(NBVH_ERR_UNSUPPORTED, "wavefront rounds need the ray count on the host");
         run_query_wavefront(ctx, s, p, batch, stream);
         return;
     }
@@ -263,25 +270,36 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
         run_query_ordered(ctx, s, p, batch, stream, lease);
         return;
     }
-    // Coherence order: keys + radix sort into transient stream-ordered memory, freed after the launch.
+    // Coherence order into the caller's scratch or transient stream-ordered memory, freed after
+    // the launch: keys + radix sort; or, for a device-side count or under graph capture (the
+    // radix sort records memset nodes), the bucket order of kernels alone.
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-    const size_t scratch = ray_order_scratch_bytes(batch.n);
-    void* mem = nullptr;
-    hip_check(hipMallocAsync(&mem, scratch + (size_t)batch.n * sizeof(uint32_t), stream), "hipMallocAsync");
-    uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + scratch);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(stream, &cs), "hipStreamIsCapturing");
+    const bool buckets = batch.live != nullptr || cs != hipStreamCaptureStatusNone;
+    if (scratch == nullptr && cs != hipStreamCaptureStatusNone)
+        fail(NBVH_ERR_STATE, "a captured sorted query needs caller scratch (sort the rays outside the capture)");
+    const size_t sort_bytes = (std::max(ray_order_scratch_bytes(batch.n), ray_bucket_scratch_bytes(batch.n)) + 255) /
+                              256 * 256;
+    void* mem = scratch;
+    if (mem == nullptr) hip_check(hipMallocAsync(&mem, query_scratch_bytes(batch.n), stream), "hipMallocAsync");
+    uint32_t* order = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + sort_bytes);
     const float* bmin = s->bbox_min;
     float ext[3];
     for (int c = 0; c < 3; c++) ext[c] = s->bbox_max[c] - s->bbox_min[c];
     try {
-        hip_check(ray_order(batch, bmin, ext, order, mem, scratch, stream), "ray order");
+        if (buckets)
+            hip_check(ray_bucket_order(batch, bmin, ext, order, mem, stream), "ray bucket order");
+        else
+            hip_check(ray_order(batch, bmin, ext, order, mem, sort_bytes, stream), "ray order");
         RayBatch sorted = batch;
         sorted.order    = order;
         run_query_ordered(ctx, s, p, sorted, stream, lease);
     } catch (...) {
-        (void)hipFreeAsync(mem, stream);
+        if (scratch == nullptr) (void)hipFreeAsync(mem, stream);
         throw;
     }
-    hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
+    if (scratch == nullptr) hip_check(hipFreeAsync(mem, stream), "hipFreeAsync");
 }
 
 // Sorted wavefront rounds (NBVH_QUERY_WAVEFRONT): see kernels/wavefront.hip. Transient
@@ -382,8 +400,8 @@ void run_query_ordered(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_para
     NetArgs a = neural_args(s, p);
     // Tiled claim order only for a whole number of 8-scanline blocks (a hint: ignored otherwise).
     const uint64_t w = p->image_width;
-    a.image_width = (batch.parts == nullptr && batch.order == nullptr && w != 0 && w % 8 == 0 &&
-                     (uint64_t)n % (8 * w) == 0)
+    a.image_width = (batch.parts == nullptr && batch.order == nullptr && batch.live == nullptr && w != 0 &&
+                     w % 8 == 0 && (uint64_t)n % (8 * w) == 0)
                         ? (uint32_t)w
                         : 0u;
     a.blocks_per_cu = s->shape[p->precision].blocks_per_cu;
@@ -1085,29 +1103,42 @@ NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_pa
 
 namespace nbvh {
 
+size_t hybrid_scratch_bytes(uint32_t n, bool sort)
+{
+    const size_t bufs = ((size_t)n * (sizeof(nbvh_ray) + sizeof(nbvh_hit)) + 255) / 256 * 256;
+    return bufs + (sort ? query_scratch_bytes(n) : 0);
+}
+
 // Hybrid closest/any hit (see nbvh_query_hybrid): triangles first, the neural query in front of
 // each triangle hit, the neural record replacing the triangle's when there is one.
 void run_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles, const nbvh_query_params* p,
-                const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits, hipStream_t st)
+                const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits, hipStream_t st, const uint32_t* live,
+                void* scratch)
 {
     const bool any = p->kind == NBVH_QUERY_NEURAL_ANY;
     nbvh_query_params tp = *p;
     tp.kind  = any ? NBVH_QUERY_TRIANGLE_ANY : NBVH_QUERY_TRIANGLE_CLOSEST;
     tp.flags = p->flags & NBVH_QUERY_SORT_RAYS;
-    run_query(ctx, triangles, &tp, RayBatch{d_rays, d_hits, nullptr, 0, n}, st);
-    void* mem = nullptr;
-    hip_check(hipMallocAsync(&mem, (size_t)n * (sizeof(nbvh_ray) + sizeof(nbvh_hit)), st), "hipMallocAsync");
+    const size_t bufs = hybrid_scratch_bytes(n, false);
+    void* sort_scratch = scratch ? static_cast<char*>(scratch) + bufs : nullptr;
+    RayBatch tb{d_rays, d_hits, nullptr, 0, n};
+    tb.live = live;
+    run_query(ctx, triangles, &tp, tb, st, nullptr, sort_scratch);
+    void* mem = scratch;
+    if (mem == nullptr) hip_check(hipMallocAsync(&mem, bufs, st), "hipMallocAsync");
     nbvh_ray* clipped = static_cast<nbvh_ray*>(mem);
     nbvh_hit* nh      = reinterpret_cast<nbvh_hit*>(clipped + n);
     try {
-        hip_check(hybrid_clip(d_rays, d_hits, n, clipped, st), "hybrid clip");
-        run_query(ctx, neural, p, RayBatch{clipped, nh, nullptr, 0, n}, st);
-        hip_check(hybrid_merge(d_hits, nh, n, any, st), "hybrid merge");
+        hip_check(hybrid_clip(d_rays, d_hits, n, live, clipped, st), "hybrid clip");
+        RayBatch nb{clipped, nh, nullptr, 0, n};
+        nb.live = live;
+        run_query(ctx, neural, p, nb, st, nullptr, sort_scratch);
+        hip_check(hybrid_merge(d_hits, nh, n, live, any, st), "hybrid merge");
     } catch (...) {
-        (void)hipFreeAsync(mem, st);
+        if (scratch == nullptr) (void)hipFreeAsync(mem, st);
         throw;
     }
-    hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
+    if (scratch == nullptr) hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
 }
 
 }  // namespace nbvh
@@ -1129,45 +1160,105 @@ NBVH_API int32_t nbvh_query_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, cons
     });
 }
 
+namespace nbvh {
+namespace {
+
+// Device memory of one nbvh_path_trace frame, one block: ping-pong rays and path states, the hit
+// records, the live-path counts (counts[d]: paths that reach depth d + 1) and the queries' scratch.
+struct FrameLayout {
+    static constexpr uint32_t kCountWords = 64;
+    size_t rays[2], state[2], hits, counts, scratch, total;
+    FrameLayout(uint32_t n, const nbvh_path_params* p, bool hybrid)
+    {
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        const bool sort = ((p->query.flags | p->bounce_query_flags) & NBVH_QUERY_SORT_RAYS) != 0u;
+        size_t off = 0;
+        for (int k = 0; k < 2; k++) {
+            rays[k] = off;
+            off += al((size_t)n * sizeof(nbvh_ray));
+        }
+        for (int k = 0; k < 2; k++) {
+            state[k] = off;
+            off += al((size_t)n * sizeof(PathState));
+        }
+        hits = off;
+        off += al((size_t)n * sizeof(nbvh_hit));
+        counts = off;
+        off += kCountWords * sizeof(uint32_t);
+        scratch = off;
+        off += al(hybrid ? hybrid_scratch_bytes(n, sort) : sort ? query_scratch_bytes(n) : 0);
+        total = off;
+    }
+};
+
+void check_path_params(const nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
+                       const nbvh_path_params* p, uint32_t width, uint32_t height)
+{
+    require(ctx && neural && p, "NULL argument");
+    require(neural->ctx == ctx && (!environment || environment->ctx == ctx), "scene belongs to another context");
+    const bool tri_object = p->query.kind == NBVH_QUERY_TRIANGLE_CLOSEST;
+    require(p->query.kind == NBVH_QUERY_NEURAL_CLOSEST || tri_object,
+            "path tracing takes NBVH_QUERY_NEURAL_CLOSEST or NBVH_QUERY_TRIANGLE_CLOSEST");
+    require(!(tri_object && environment), "a triangle rendering of the object takes no environment");
+    require(p->secondary_visibility_threshold == 0.f ||
+                (p->secondary_visibility_threshold > 0.f && p->secondary_visibility_threshold < 1.f),
+            "secondary_visibility_threshold must be 0 or in (0, 1)");
+    if (tri_object && neural->bvh.empty()) fail(NBVH_ERR_STATE, "the object has no triangles");
+    require(p->max_depth >= 1 && p->max_depth <= 16, "max_depth must be 1..16");
+    require(p->background <= NBVH_BACKGROUND_SKY, "unknown background");
+    require((uint64_t)width * height <= 0xFFFFFFFFull, "image too large");
+    if (environment && environment->bvh.empty()) fail(NBVH_ERR_STATE, "the environment has no triangles");
+}
+
+}  // namespace
+}  // namespace nbvh
+
+NBVH_API int32_t nbvh_path_trace_workspace_size(nbvh_ctx* ctx, const nbvh_scene* neural,
+                                                const nbvh_scene* environment, const nbvh_path_params* p,
+                                                uint32_t width, uint32_t height, uint64_t* bytes)
+{
+    return guarded(ctx, [&] {
+        require(bytes != nullptr, "bytes is NULL");
+        check_path_params(ctx, neural, environment, p, width, height);
+        *bytes = (uint64_t)FrameLayout(width * height, p, environment != nullptr).total;
+    });
+}
+
 NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* environment,
                                  const nbvh_path_params* p, const nbvh_camera* cam, uint32_t width, uint32_t height,
                                  float* d_accum, nbvh_path_stats* stats, void* stream)
 {
     return guarded(ctx, [&] {
-        require(ctx && neural && p && cam, "NULL argument");
-        require(neural->ctx == ctx && (!environment || environment->ctx == ctx), "scene belongs to another context");
+        check_path_params(ctx, neural, environment, p, width, height);
+        require(cam != nullptr, "NULL argument");
         const bool tri_object = p->query.kind == NBVH_QUERY_TRIANGLE_CLOSEST;
-        require(p->query.kind == NBVH_QUERY_NEURAL_CLOSEST || tri_object,
-                "path tracing takes NBVH_QUERY_NEURAL_CLOSEST or NBVH_QUERY_TRIANGLE_CLOSEST");
-        require(!(tri_object && environment), "a triangle rendering of the object takes no environment");
-        require(p->secondary_visibility_threshold == 0.f ||
-                    (p->secondary_visibility_threshold > 0.f && p->secondary_visibility_threshold < 1.f),
-                "secondary_visibility_threshold must be 0 or in (0, 1)");
-        if (tri_object && neural->bvh.empty()) fail(NBVH_ERR_STATE, "the object has no triangles");
-        require(p->max_depth >= 1 && p->max_depth <= 16, "max_depth must be 1..16");
-        require(p->background <= NBVH_BACKGROUND_SKY, "unknown background");
-        require((uint64_t)width * height <= 0xFFFFFFFFull, "image too large");
-        if (environment && environment->bvh.empty()) fail(NBVH_ERR_STATE, "the environment has no triangles");
         if (stats) *stats = nbvh_path_stats{};
         const uint32_t n = width * height;
         if (n == 0) return;
         require(d_accum != nullptr, "NULL accumulation buffer");
         hipStream_t st = (hipStream_t)stream;
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-        size_t off = 0;
-        const size_t o_rays0 = off;  off += al((size_t)n * sizeof(nbvh_ray));
-        const size_t o_rays1 = off;  off += al((size_t)n * sizeof(nbvh_ray));
-        const size_t o_st0 = off;    off += al((size_t)n * sizeof(PathState));
-        const size_t o_st1 = off;    off += al((size_t)n * sizeof(PathState));
-        const size_t o_hits = off;   off += al((size_t)n * sizeof(nbvh_hit));
-        const size_t o_cnt = off;    off += 256;
-        char* mem = nullptr;
-        hip_check(hipMallocAsync((void**)&mem, off, st), "hipMallocAsync");
-        nbvh_ray* rays[2] = {reinterpret_cast<nbvh_ray*>(mem + o_rays0), reinterpret_cast<nbvh_ray*>(mem + o_rays1)};
-        PathState* state[2] = {reinterpret_cast<PathState*>(mem + o_st0), reinterpret_cast<PathState*>(mem + o_st1)};
-        nbvh_hit* hits = reinterpret_cast<nbvh_hit*>(mem + o_hits);
-        uint32_t* cnt  = reinterpret_cast<uint32_t*>(mem + o_cnt);
+        // Wavefront rounds read their ray counts on the host: with them, the frame reads each
+        // depth's live-path count back; otherwise the counts stay on the device.
+        const bool host_counts = (p->bounce_query_flags & NBVH_QUERY_WAVEFRONT) != 0u;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
+        const bool capturing = cs != hipStreamCaptureStatusNone;
+        const FrameLayout lay(n, p, environment != nullptr);
+        if (capturing) {
+            require(p->workspace != nullptr, "a captured frame needs a workspace (nbvh_path_trace_workspace_size)");
+            require(stats == nullptr, "stats must be NULL while the stream is captured");
+            require(!host_counts, "wavefront-round bounces cannot be captured");
+        }
+        if (p->workspace) require(p->workspace_bytes >= lay.total, "workspace smaller than nbvh_path_trace_workspace_size");
+        char* mem = static_cast<char*>(p->workspace);
+        if (mem == nullptr) hip_check(hipMallocAsync((void**)&mem, lay.total, st), "hipMallocAsync");
+        nbvh_ray* rays[2] = {reinterpret_cast<nbvh_ray*>(mem + lay.rays[0]), reinterpret_cast<nbvh_ray*>(mem + lay.rays[1])};
+        PathState* state[2] = {reinterpret_cast<PathState*>(mem + lay.state[0]),
+                               reinterpret_cast<PathState*>(mem + lay.state[1])};
+        nbvh_hit* hits  = reinterpret_cast<nbvh_hit*>(mem + lay.hits);
+        uint32_t* cnt   = reinterpret_cast<uint32_t*>(mem + lay.counts);
+        void* scratch   = lay.total > lay.scratch ? mem + lay.scratch : nullptr;
         PathArgs a{};
         const nbvh_scene* tri_scene = tri_object ? neural : environment;
         if (tri_scene)
@@ -1184,12 +1275,17 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
         }
         a.normal_offset = p->normal_offset > 0.f ? p->normal_offset : 1e-4f;
         try {
+            hip_check(zero_u32(cnt, FrameLayout::kCountWords, st), "counter reset");
             hip_check(raygen(*cam, width, height, p->sample, 0u, rays[0], st), "raygen launch");
             hip_check(path_init(state[0], n, st), "path init");
-            uint32_t cur = n;
+            uint32_t cur = n;  // host-side count: depth 1, and every depth with host_counts
             int c = 0;
-            for (uint32_t depth = 1; depth <= p->max_depth && cur > 0; depth++) {
-                if (stats) stats->rays[depth - 1] = cur;
+            for (uint32_t depth = 1; depth <= p->max_depth; depth++) {
+                // Depth 1 traces all n primary rays; deeper rays are the live paths counted on
+                // the device (launches sized for n), or read back with host_counts.
+                const uint32_t* live = depth == 1 || host_counts ? nullptr : cnt + (depth - 1);
+                const uint32_t cap   = host_counts ? cur : n;
+                if (cap == 0) break;
                 nbvh_query_params q = p->query;
                 if (depth > 1) {
                     q.flags       = p->bounce_query_flags;
@@ -1198,23 +1294,35 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
                     if (p->secondary_visibility_threshold > 0.f)
                         q.visibility_threshold = p->secondary_visibility_threshold;
                 }
-                if (environment)
-                    run_hybrid(ctx, neural, environment, &q, rays[c], cur, hits, st);
-                else
-                    run_query(ctx, neural, &q, RayBatch{rays[c], hits, nullptr, 0, cur}, st);
-                hip_check(zero_u32(cnt, 1, st), "counter reset");
-                hip_check(path_shade(a, depth, cur, rays[c], hits, state[c], rays[1 - c], state[1 - c], cnt, st),
+                if (environment) {
+                    run_hybrid(ctx, neural, environment, &q, rays[c], cap, hits, st, live, scratch);
+                } else {
+                    RayBatch rb{rays[c], hits, nullptr, 0, cap};
+                    rb.live = live;
+                    run_query(ctx, neural, &q, rb, st, nullptr, scratch);
+                }
+                hip_check(path_shade(a, depth, cap, live, rays[c], hits, state[c], rays[1 - c], state[1 - c],
+                                     cnt + depth, st),
                           "path shade");
                 if (depth == p->max_depth) break;
-                hip_check(hipMemcpyAsync(&cur, cnt, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
-                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                if (host_counts) {
+                    hip_check(hipMemcpyAsync(&cur, cnt + depth, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+                    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                }
                 c = 1 - c;
             }
+            if (stats) {
+                uint32_t h[FrameLayout::kCountWords];
+                hip_check(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+                hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                stats->rays[0] = n;
+                for (uint32_t d = 2; d <= p->max_depth; d++) stats->rays[d - 1] = h[d - 1];
+            }
         } catch (...) {
-            (void)hipFreeAsync(mem, st);
+            if (p->workspace == nullptr) (void)hipFreeAsync(mem, st);
             throw;
         }
-        hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
+        if (p->workspace == nullptr) hip_check(hipFreeAsync(mem, st), "hipFreeAsync");
     });
 }
 

@@ -152,8 +152,19 @@ void ensure_part_tables(nbvh_ctx* ctx);
 
 // Queue one query launch (counter reset + kernel) over the rays of `batch` on `stream`, with
 // the scratch of `lease` (one is taken and finished here when NULL). Validates the parameters
-// against the scene; throws Error on failure.
+// against the scene; throws Error on failure. scratch: device memory of at least
+// query_scratch_bytes(batch.n) for the coherence sort of NBVH_QUERY_SORT_RAYS (NULL: stream-ordered
+// allocation).
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
-               hipStream_t stream, SlotLease* lease = nullptr);
+               hipStream_t stream, SlotLease* lease = nullptr, void* scratch = nullptr);
+size_t query_scratch_bytes(uint32_t n);
+
+// Hybrid closest/any hit of rays [0, min(*live, n)) (all n without live): triangles first, the
+// neural query in front of each triangle hit. scratch: at least hybrid_scratch_bytes(n, sort)
+// (NULL: stream-ordered allocation).
+void run_hybrid(nbvh_ctx* ctx, const nbvh_scene* neural, const nbvh_scene* triangles, const nbvh_query_params* p,
+                const nbvh_ray* d_rays, uint32_t n, nbvh_hit* d_hits, hipStream_t st, const uint32_t* live = nullptr,
+                void* scratch = nullptr);
+size_t hybrid_scratch_bytes(uint32_t n, bool sort);
 
 }  // namespace nbvh

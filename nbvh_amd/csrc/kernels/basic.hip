@@ -62,11 +62,17 @@ __global__ void scatter_diffuse_kernel(TriArgs tri, const nbvh_ray* __restrict__
 // Hybrid query helpers. clip: the neural pass only looks in front of the triangle hit
 // (tmax = min(tmax, t_tri)); merge: the neural hit replaces the triangle record when there is one
 // (it is then closer, or for any-hit queries the ray was not occluded by triangles).
+// live: optional device-side count (at most n) of the rays to process.
+__device__ __forceinline__ uint32_t live_count(const uint32_t* live, uint32_t n)
+{
+    return live && *live < n ? *live : n;
+}
+
 __global__ void hybrid_clip_kernel(const nbvh_ray* __restrict__ rays, const nbvh_hit* __restrict__ tri, uint32_t n,
-                                   nbvh_ray* __restrict__ out)
+                                   const uint32_t* __restrict__ live, nbvh_ray* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= live_count(live, n)) return;
     Ray r = load_ray(rays, i);
     const nbvh_hit h = tri[i];
     if ((h.info & 0xFFu) != 0u && h.t < r.tmax) r.tmax = h.t;
@@ -76,10 +82,10 @@ __global__ void hybrid_clip_kernel(const nbvh_ray* __restrict__ rays, const nbvh
 }
 
 __global__ void hybrid_merge_kernel(nbvh_hit* __restrict__ hits, const nbvh_hit* __restrict__ neural, uint32_t n,
-                                    uint32_t any_hit)
+                                    const uint32_t* __restrict__ live, uint32_t any_hit)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= live_count(live, n)) return;
     const nbvh_hit nh = neural[i];
     const uint32_t tri_kind = hits[i].info & 0xFFu;
     if ((nh.info & 0xFFu) != 0u && !(any_hit && tri_kind != 0u)) store_hit(hits, i, nh);
@@ -95,7 +101,7 @@ template <bool PARTS>
 __global__ __launch_bounds__(256) void tri_query_kernel(TriArgs a, RayBatch b)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= b.n) return;
+    if (g >= batch_count(b)) return;
     uint32_t part, i;
     batch_locate<PARTS>(b, batch_ray(b, g), part, i);
     const Ray ray = load_ray(batch_rays<PARTS>(b, part), i);
@@ -159,15 +165,17 @@ hipError_t segment_labels(const TriArgs& a, const nbvh_ray* rays, const float* m
     return hipGetLastError();
 }
 
-hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream)
+hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, const uint32_t* live, nbvh_ray* out,
+                       hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::hybrid_clip_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rays, tri, n, out);
+    hipLaunchKernelGGL(dev::hybrid_clip_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, rays, tri, n, live, out);
     return hipGetLastError();
 }
 
-hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream)
+hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, const uint32_t* live, bool any_hit,
+                        hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::hybrid_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, hits, neural, n,
+    hipLaunchKernelGGL(dev::hybrid_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, hits, neural, n, live,
                        any_hit ? 1u : 0u);
     return hipGetLastError();
 }

@@ -27,10 +27,19 @@ hipError_t segment_labels(const TriArgs& a, const nbvh_ray* rays, const float* m
 size_t ray_order_scratch_bytes(uint32_t n);
 hipError_t ray_order(const RayBatch& rb, const float enc_min[3], const float enc_ext[3], uint32_t* order,
                      void* scratch, size_t scratch_bytes, hipStream_t stream);
+// The same grouping from kernels alone (recordable into a HIP graph): 2^16 buckets of (octant,
+// coarse origin Morton code), order[claim] for the claims [0, count of rb); no order inside a
+// bucket. Scratch: ray_bucket_scratch_bytes(n).
+size_t ray_bucket_scratch_bytes(uint32_t n);
+hipError_t ray_bucket_order(const RayBatch& rb, const float enc_min[3], const float enc_ext[3], uint32_t* order,
+                            void* scratch, hipStream_t stream);
 
 // Hybrid query steps: rays clipped to their triangle hits; neural records merged over triangle ones.
-hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, nbvh_ray* out, hipStream_t stream);
-hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, bool any_hit, hipStream_t stream);
+// live: optional device-side count (at most n) of the rays to process.
+hipError_t hybrid_clip(const nbvh_ray* rays, const nbvh_hit* tri, uint32_t n, const uint32_t* live, nbvh_ray* out,
+                       hipStream_t stream);
+hipError_t hybrid_merge(nbvh_hit* hits, const nbvh_hit* neural, uint32_t n, const uint32_t* live, bool any_hit,
+                        hipStream_t stream);
 
 // Sorted wavefront rounds (wavefront.hip): state init, traversal to the next leaf per active ray
 // (segments keyed by the Morton code of their midpoints), segment sort by key, encoding in key
@@ -59,11 +68,11 @@ hipError_t train_adam(float* w, const float* g, float* m, float* v, uint32_t* st
                       const AdamArgs& h, hipStream_t stream);
 
 // Path-trace driver: per-path state init (pixel = index, throughput 1) and one depth's shading
-// and compaction (live paths appended at *counter).
+// and compaction (live paths appended at *counter) of min(*live, n) paths (n without live).
 hipError_t path_init(PathState* st, uint32_t n, hipStream_t stream);
-hipError_t path_shade(const PathArgs& a, uint32_t depth, uint32_t n, const nbvh_ray* rays, const nbvh_hit* hits,
-                      const PathState* st, nbvh_ray* next_rays, PathState* next_st, uint32_t* counter,
-                      hipStream_t stream);
+hipError_t path_shade(const PathArgs& a, uint32_t depth, uint32_t n, const uint32_t* live, const nbvh_ray* rays,
+                      const nbvh_hit* hits, const PathState* st, nbvh_ray* next_rays, PathState* next_st,
+                      uint32_t* counter, hipStream_t stream);
 
 // Triangle BVH closest / any hit.
 hipError_t query_triangles(const TriArgs& a, const RayBatch& rb, hipStream_t stream);

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width, a.queue_bands);
+    q.init(queue, batch_count(b), a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, GlobalNodes{a.nodes}, s, st, b, q)) {
         float x[K0];
         if (s.state == SLOT_SEG) {

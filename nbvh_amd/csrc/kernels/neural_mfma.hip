@@ -537,7 +537,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width, a.queue_bands);
+    q.init(queue, batch_count(b), a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     Slot s;
     s.state = SLOT_EMPTY;
     RayQueue q;
-    q.init(queue, b.n, a.image_width, a.queue_bands);
+    q.init(queue, batch_count(b), a.image_width, a.queue_bands);
     while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(256) void path_init_kernel(PathState* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void path_shade_kernel(PathArgs a, uint32_t depth, uint32_t n,
+                                                         const uint32_t* __restrict__ live,
                                                          const nbvh_ray* __restrict__ rays,
                                                          const nbvh_hit* __restrict__ hits,
                                                          const PathState* __restrict__ st,
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void path_shade_kernel(PathArgs a, uint32_t de
                                                          uint32_t* __restrict__ counter)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (live && *live < n ? *live : n)) return;
     const Ray r = load_ray(rays, i);
     const nbvh_hit h = hits[i];
     PathState s = st[i];
@@ -120,12 +121,12 @@ hipError_t path_init(PathState* st, uint32_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t path_shade(const PathArgs& a, uint32_t depth, uint32_t n, const nbvh_ray* rays, const nbvh_hit* hits,
-                      const PathState* st, nbvh_ray* next_rays, PathState* next_st, uint32_t* counter,
-                      hipStream_t stream)
+hipError_t path_shade(const PathArgs& a, uint32_t depth, uint32_t n, const uint32_t* live, const nbvh_ray* rays,
+                      const nbvh_hit* hits, const PathState* st, nbvh_ray* next_rays, PathState* next_st,
+                      uint32_t* counter, hipStream_t stream)
 {
-    hipLaunchKernelGGL(dev::path_shade_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, depth, n, rays, hits,
-                       st, next_rays, next_st, counter);
+    hipLaunchKernelGGL(dev::path_shade_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, depth, n, live, rays,
+                       hits, st, next_rays, next_st, counter);
     return hipGetLastError();
 }
 

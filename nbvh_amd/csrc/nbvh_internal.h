@@ -87,6 +87,8 @@ struct RayBatch {
     uint32_t         n_parts;
     uint32_t         n;
     const uint32_t*  order;  // optional claim order: claim i traces ray order[i] (a permutation of [0, n))
+    const uint32_t*  live;   // optional device-side count: the batch is claims [0, min(*live, n)), n being
+                             // the capacity the launch is sized for (no parts, no image order)
 };
 
 // One leaf segment of the sorted wavefront rounds (NBVH_QUERY_WAVEFRONT).

@@ -135,6 +135,80 @@ def test_path_trace_deterministic_and_accumulating(nb, setup):
         assert np.array_equal(a01[:, :3], a0.reshape(-1, 4)[:, :3] + a1[:, :3])
 
 
+@pytest.mark.parametrize("hybrid", [False, True])
+def test_path_trace_graph_replays_match_eager(nb, setup, hybrid):
+    """A frame recorded into one HIP graph (live-path counts on the device, buffers in a caller
+    workspace, sorted bounces in bucket order) replays bit-identical to the eager frame, with eager
+    frames and synchronisations between the replays; the eager frame with a workspace equals the
+    one with transient memory and the one whose bounces read their counts back (wavefront rounds)."""
+    from nbvh_amd import hip
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    mod, env = setup["mod"], (setup["env"] if hybrid else None)
+    kw = dict(precision=nb.PREC_FP16, max_depth=4, rr_start_depth=3, sample=0)
+    p0 = mod.path_params(W, H, **kw)
+    size = mod.path_trace_workspace_size(p0, W, H, env)
+    assert size >= W * H * 150
+    ws = hip.Buffer(size)
+    p = mod.path_params(W, H, workspace=ws, **kw)
+    s = hip.Stream()
+    acc_e, acc_g, acc_x = hip.Buffer(W * H * 16), hip.Buffer(W * H * 16), hip.Buffer(W * H * 16)
+    counts = mod.path_trace_params_device(p, cam, W, H, acc_e.ptr, env, stream=s.handle)
+    s.synchronize()
+    eager = acc_e.download(np.float32)
+    assert counts[0] == W * H and counts[1] > 0 and np.all(eager.reshape(-1, 4)[:, 3] == 1)
+    assert mod.path_trace_params_device(p0, cam, W, H, acc_x.ptr, env, stream=s.handle) == counts
+    s.synchronize()
+    assert np.array_equal(acc_x.download(np.float32), eager)
+    pw = mod.path_params(W, H, wavefront_bounces=True, **kw)
+    assert mod.path_trace_params_device(pw, cam, W, H, acc_x.ptr, env, stream=s.handle) == counts
+    s.synchronize()
+    assert np.array_equal(acc_x.download(np.float32), eager)
+    with hip.Graph.capture(s) as g:
+        mod.path_trace_params_device(p, cam, W, H, acc_g.ptr, env, stream=s.handle, stats=False)
+    try:
+        for k in range(4):
+            acc_g.zero()
+            g.launch(s)
+            s.synchronize()
+            assert np.array_equal(acc_g.download(np.float32), eager), k
+            # eager work between the replays: a frame of another sample into another buffer
+            mod.path_trace_params_device(mod.path_params(W, H, **dict(kw, sample=0, rr_start_depth=2)), cam, W, H,
+                                         acc_x.ptr, env, stream=s.handle)
+            s.synchronize()
+    finally:
+        g.destroy()
+        s.destroy()
+
+
+def test_path_trace_workspace_errors(nb, setup):
+    from nbvh_amd import hip
+    cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
+    mod = setup["mod"]
+    acc = hip.Buffer(W * H * 16)
+    p0 = mod.path_params(W, H, precision=nb.PREC_FP16)
+    size = mod.path_trace_workspace_size(p0, W, H)
+    small = hip.Buffer(size - 256)
+    with pytest.raises(nb.NbvhError):
+        mod.path_trace_params_device(mod.path_params(W, H, precision=nb.PREC_FP16, workspace=small), cam, W, H,
+                                     acc.ptr)
+    s = hip.Stream()
+    ws = hip.Buffer(size)
+    try:
+        with pytest.raises(nb.NbvhError):  # no workspace under capture
+            with hip.Graph.capture(s):
+                mod.path_trace_params_device(p0, cam, W, H, acc.ptr, stream=s.handle, stats=False)
+        with pytest.raises(nb.NbvhError):  # stats read back under capture
+            with hip.Graph.capture(s):
+                mod.path_trace_params_device(mod.path_params(W, H, precision=nb.PREC_FP16, workspace=ws), cam, W, H,
+                                             acc.ptr, stream=s.handle)
+        # the stream is usable afterwards
+        mod.path_trace_params_device(mod.path_params(W, H, precision=nb.PREC_FP16, workspace=ws), cam, W, H, acc.ptr,
+                                     stream=s.handle)
+        s.synchronize()
+    finally:
+        s.destroy()
+
+
 def test_path_trace_fp16_close_to_fp32(nb, setup):
     cam = nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, W, H)
     a32, _ = setup["mod"].path_trace(cam, W, H, precision=nb.PREC_FP32, samples=4)

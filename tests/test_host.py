@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol(nb):
     assert set(names) <= exported
     # Python binding covers the whole header.
     assert set(names) == set(_ffi.SIGNATURES)
-    assert L.nbvh_abi_version() == 1
+    assert L.nbvh_abi_version() == 2
 
 
 def test_status_strings_and_null_args(nb):
