@@ -163,7 +163,10 @@ enum nbvh_query_flags {
      * code of the origin (per part of a multi-part launch) before the launch. Hits stay at each
      * ray's own index and are identical; worthwhile for incoherent batches (bounces, shadow rays).
      * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray, so a
-     * sorted query is not meant for HIP graph capture. */
+     * sorted query is not meant for HIP graph capture. nbvh_query, nbvh_query_host and
+     * nbvh_query_parts apply it by themselves to batches of at least 65,536 rays that carry no
+     * image_width hint, except while the stream is captured and unless NBVH_QUERY_ARRAY_ORDER is
+     * set (nbvh_query_noalloc sorts only when asked). */
     NBVH_QUERY_SORT_RAYS = 1u,
     /* Neural kinds: evaluate in sorted wavefront rounds instead of the persistent kernel, the
      * step-loop schedule of NeuralPathModule::neural_bvh_traversal
@@ -175,7 +178,9 @@ enum nbvh_query_flags {
      * that want the reference's round structure. Synchronises the stream once per round (two
      * 4-byte reads) and uses stream-ordered transient memory of about 300 B per ray; not for
      * graph capture. */
-    NBVH_QUERY_WAVEFRONT = 2u
+    NBVH_QUERY_WAVEFRONT = 2u,
+    /* Trace the rays in array order: no automatic coherence order (see NBVH_QUERY_SORT_RAYS). */
+    NBVH_QUERY_ARRAY_ORDER = 4u
 };
 
 /* Pinhole camera in the reference's CameraParams form (origin, du, dv, dir_top_left). */

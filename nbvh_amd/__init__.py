@@ -338,8 +338,11 @@ class NeuralBVHModule:
 
     @staticmethod
     def params(kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0, max_traversal_steps=0,
-               visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False):
-        flags = (_ffi.QUERY_SORT_RAYS if sort_rays else 0) | (_ffi.QUERY_WAVEFRONT if wavefront else 0)
+               visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False, array_order=False):
+        """nbvh_query_params. Batches of >= 65,536 rays without image_width are traced in the
+        coherence order unless array_order=True (sort_rays=True asks for it at any size)."""
+        flags = ((_ffi.QUERY_SORT_RAYS if sort_rays else 0) | (_ffi.QUERY_WAVEFRONT if wavefront else 0) |
+                 (_ffi.QUERY_ARRAY_ORDER if array_order else 0))
         return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, flags)
 
     def neural_bvh_traversal(self, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0,

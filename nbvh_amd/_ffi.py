@@ -27,6 +27,7 @@ ERR_CAPACITY = 9
 PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = 0, 1, 2, 3
 QUERY_SORT_RAYS = 1
 QUERY_WAVEFRONT = 2
+QUERY_ARRAY_ORDER = 4
 NOALLOC_MAX_SORTED_RAYS = 524288  # NBVH_NOALLOC_MAX_SORTED_RAYS
 BACKGROUND_CONSTANT, BACKGROUND_SKY = 0, 1
 TRAIN_GRADIENTS_ONLY = 1
