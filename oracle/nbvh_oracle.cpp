@@ -417,39 +417,59 @@ void grid_point(const oracle_net &net, const std::vector<Level> &lv, const R x[3
     }
 }
 
+// Largest network the oracle evaluates: P * L * F inputs, hidden width, outputs.
+constexpr uint32_t kMaxInputs = 4 * 32 * 8, kMaxNeurons = 256;
+
+// One dense layer y[j] = sum_k W[j * in + k] * a[k] (row-major W), each output's sum formed in k
+// order by its own chain of multiply-adds (the kernels' order); 8 outputs at a time, so that
+// their independent chains overlap. relu: max(y, 0).
+template <typename R>
+void dense_layer(const float *W, const R *a, uint32_t in, uint32_t out, R *y, bool relu)
+{
+    uint32_t j = 0;
+    for (; j + 8 <= out; j += 8) {
+        R acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t k = 0; k < in; k++) {
+            const R ak = a[k];
+            for (uint32_t u = 0; u < 8; u++) acc[u] = fma_r<R>((R)W[(j + u) * in + k], ak, acc[u]);
+        }
+        for (uint32_t u = 0; u < 8; u++) y[j + u] = relu ? (acc[u] > (R)0 ? acc[u] : (R)0) : acc[u];
+    }
+    for (; j < out; j++) {
+        R acc = 0;
+        for (uint32_t k = 0; k < in; k++) acc = fma_r<R>((R)W[j * in + k], a[k], acc);
+        y[j] = relu ? (acc > (R)0 ? acc : (R)0) : acc;
+    }
+}
+
 // MLP: logits z[n_outputs] from features x[P*L*F].
 template <typename R>
 void mlp(const oracle_net &net, const R *x, R *z)
 {
     const uint32_t K0 = net.n_points * net.n_levels * net.n_features;
     const uint32_t H  = net.n_neurons;
-    std::vector<R> a(x, x + K0), b(H);
-    const float   *W   = net.mlp;
-    uint32_t       in  = K0;
+    if (K0 > kMaxInputs || H > kMaxNeurons) std::abort();
+    R a[kMaxInputs], b[kMaxNeurons];
+    for (uint32_t k = 0; k < K0; k++) a[k] = x[k];
+    const float *W  = net.mlp;
+    uint32_t     in = K0;
     for (uint32_t layer = 0; layer < net.n_hidden_layers; layer++) {
-        for (uint32_t j = 0; j < H; j++) {
-            R acc = 0;
-            for (uint32_t k = 0; k < in; k++) acc = fma_r<R>((R)W[j * in + k], a[k], acc);
-            b[j] = acc > (R)0 ? acc : (R)0;
-        }
+        dense_layer<R>(W, a, in, H, b, true);
         W += (size_t)H * in;
         in = H;
-        a.assign(b.begin(), b.end());
+        for (uint32_t k = 0; k < H; k++) a[k] = b[k];
     }
-    for (uint32_t j = 0; j < net.n_outputs; j++) {
-        R acc = 0;
-        for (uint32_t k = 0; k < in; k++) acc = fma_r<R>((R)W[j * in + k], a[k], acc);
-        z[j] = acc;
-    }
+    dense_layer<R>(W, a, in, net.n_outputs, z, false);
 }
 
 template <typename R>
 void network_eval(const oracle_net &net, const std::vector<Level> &lv, const R *pts, R *z)
 {
     const uint32_t LF = net.n_levels * net.n_features;
-    std::vector<R> feat(net.n_points * LF);
-    for (uint32_t k = 0; k < net.n_points; k++) grid_point<R>(net, lv, pts + 3 * k, feat.data() + k * LF);
-    mlp<R>(net, feat.data(), z);
+    if (net.n_points * LF > kMaxInputs) std::abort();
+    R feat[kMaxInputs];
+    for (uint32_t k = 0; k < net.n_points; k++) grid_point<R>(net, lv, pts + 3 * k, feat + k * LF);
+    mlp<R>(net, feat, z);
 }
 
 inline float sig(float z) { return sigmoid_f(z); }
