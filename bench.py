@@ -121,16 +121,17 @@ def wavefront_bench(nb, hip, ctx, steps):
 
     out = {"workload": "64 displaced geodesic spheres, 2.17M triangles, N-BVH 4096 leaves; 4,194,304 rays in one "
                        "launch: 2M incoherent closest-hit + 2M shadow any-hit to 8 point lights; launch shape "
-                       "from the autotuner",
+                       "and ray order from the autotuner on that launch",
            "rays": 2 * n}
     for name, prec in (("fp16", nb.PREC_FP16), ("fp32", nb.PREC_FP32)):
         p = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec)
-        # Launch shape chosen by the autotuner on the closest-hit half of the batch (kept for the
-        # scene and precision, so the two-part launches below use it).
-        shape, tune_ms = mod.autotune(p, dc.ptr, n, hc.ptr, stream=s.handle)
-        out[name + "_autotuned_shape"] = {"blocks_per_cu": shape[0], "queue_bands": shape[1],
-                                          "closest_half_ms": round(tune_ms, 4)}
-        # Default parameters: the batch has no image order, so it is traced in the coherence order.
+        # Launch shape and ray order chosen by the autotuner on the two-part launch itself (kept
+        # for the scene and precision, so the launches below use them).
+        shape, tune_ms = mod.autotune_parts(p, parts, stream=s.handle)
+        out[name + "_autotuned_shape"] = {"blocks_per_cu": shape[0], "queue_bands": shape[1], "ray_order": shape[2],
+                                          "tuning_batch_ms": round(tune_ms, 4)}
+        # Default parameters: the batch has no image order, so it is traced in the order the
+        # autotuner kept for the scene (the coherence order where that was faster).
         ms = time_it(lambda: mod.query_parts_device(p, parts, stream=s.handle), steps if name == "fp16" else 3)
         out[name] = {"ms": round(ms, 4), "mrays_per_s": round(2 * n / (ms * 1e-3) / 1e6, 1)}
         pa = mod.params(kind=nb.QUERY_NEURAL_CLOSEST, precision=prec, array_order=True)
@@ -603,7 +604,8 @@ def path_trace_driver_bench(nb, hip, ctx, scene, W=1920, H=1080, max_depth=5, rr
     return out
 
 
-def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, max_batch_rays=1 << 21):
+def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, max_batch_rays=1 << 21,
+                  host_fraction=0.25):
     """Continuous-batching serving run over three resident scenes.
 
     Scenes: the 81,920-triangle icosphere (N-BVH 256), a field of 64 geodesic spheres (2.17M
@@ -611,10 +613,12 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
     16384), each with its own seeded FP16 network of the headline shape. Requests arrive as a
     Poisson process; sizes are lognormal (median 65,536 rays, sigma 1) clamped to [4,096, 2M];
     70 % closest-hit, 30 % any-hit (shadow) neural queries; the scene is drawn uniformly. Rays
-    are windows of a 2048x1024 camera frame of the request's scene, resident in HBM; hits go to
-    a device ring (zero-copy device requests). The arrival rate is set to `load` times the
-    measured single-launch capacity of the mix. Latency is submit -> hits complete, from the
-    server's own steady-clock timestamps."""
+    are windows of a 2048x1024 camera frame of the request's scene: resident in HBM with hits to a
+    device ring (zero-copy device requests), or, for `host_fraction` of the requests, in host
+    memory with hits to a host ring (staged through the server's arena, admitted by its
+    capacity). The arrival rate is set to `load` times the measured single-launch capacity of the
+    mix. Latency is submit -> hits complete (host requests: hits copied back), from the server's
+    own steady-clock timestamps."""
     import threading
     rng = np.random.default_rng(seed)
     log("serving: building the 8.4M-triangle scene")
@@ -635,8 +639,16 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
         ctx.raygen(cam, W, H, d.ptr)
         pool.append(d)
     hip.synchronize()
+    # Host-resident requests use pinned memory, as a serving front end would.
+    pinned = [hip.HostArray(nb.RAY_DTYPE, W * H) for _ in pool]
+    for d, h in zip(pool, pinned):
+        h.array[:] = d.download(nb.RAY_DTYPE)
+    host_pool = [h.array for h in pinned]
     ring_rays = 16 << 20
     d_hits = hip.Buffer(ring_rays * 48)
+    host_ring = 4 << 20
+    pinned_hits = hip.HostArray(nb.HIT_DTYPE, host_ring)
+    h_hits = pinned_hits.array
 
     def params(kind):
         return nb.NeuralBVHModule.params(kind=kind, precision=nb.PREC_FP16)
@@ -659,6 +671,19 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
     # Mix weights: scene uniform, 70/30 kinds -> time per ray is the weighted mean of 1/capacity.
     sec_per_ray = np.mean([0.7 / cap[2 * i] + 0.3 / cap[2 * i + 1] for i in range(3)])
     capacity = 1.0 / sec_per_ray
+    # Host requests also move 32 B in and 48 B out per ray over PCIe: measured as the time of a
+    # 1M-ray host request through a server alone, minus its device-request time.
+    with nb.Server(ctx, n_streams=1, max_batch_rays=max_batch_rays) as probe:
+        p0 = params(nb.QUERY_NEURAL_CLOSEST)
+        probe.wait(probe.submit(scenes[0], p0, host_pool[0][:1 << 20], h_hits[:1 << 20]))
+        t0 = time.perf_counter()
+        probe.wait(probe.submit(scenes[0], p0, host_pool[0][:1 << 20], h_hits[:1 << 20]))
+        t_host = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        probe.wait(probe.submit_device(scenes[0], p0, pool[0].ptr, 1 << 20, d_hits.ptr))
+        t_dev = time.perf_counter() - t0
+    copy_sec_per_ray = max(0.0, t_host - t_dev) / (1 << 20)
+    capacity = 1.0 / (sec_per_ray + host_fraction * copy_sec_per_ray)
 
     sizes = np.clip(rng.lognormal(np.log(65536), 1.0, 200000), 4096, 2 << 20).astype(np.int64)
     mean_size = float(sizes.mean())
@@ -671,7 +696,7 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
         sc = int(rng.integers(3))
         kind = nb.QUERY_NEURAL_CLOSEST if rng.random() < 0.7 else nb.QUERY_NEURAL_ANY
         off = int(rng.integers(0, W * H - n + 1))
-        reqs.append((sc, kind, off, n))
+        reqs.append((sc, kind, off, n, bool(rng.random() < host_fraction)))
     log(f"serving: capacity of the mix {capacity / 1e6:.0f} Mrays/s, mean request {mean_size:.0f} rays, "
         f"{rate:.0f} requests/s for {seconds:.1f} s ({n_req} requests, load {load})")
 
@@ -693,18 +718,18 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
                     cond.wait()
                 if i >= len(tickets):
                     return
-                t, n = tickets[i]
+                t, n, host = tickets[i]
             _, info = srv.wait(t)
-            infos.append((info, n))
+            infos.append((info, n, host))
             i += 1
 
     th = threading.Thread(target=collector)
     th.start()
-    ring = 0
+    ring, hring = 0, 0
     ps = {k: params(k) for k in (nb.QUERY_NEURAL_CLOSEST, nb.QUERY_NEURAL_ANY)}
     start = time.perf_counter()
     due = start
-    for (sc, kind, off, n), gap in zip(reqs, gaps):
+    for (sc, kind, off, n, host), gap in zip(reqs, gaps):
         due += gap
         while True:
             now = time.perf_counter()
@@ -712,12 +737,18 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
                 break
             if due - now > 2e-4:
                 time.sleep(due - now - 1e-4)
-        if ring + n > ring_rays:
-            ring = 0
-        t = srv.submit_device(scenes[sc], ps[kind], pool[sc].ptr + off * 32, n, d_hits.ptr + ring * 48)
-        ring += n
+        if host:
+            if hring + n > host_ring:
+                hring = 0
+            t = srv.submit(scenes[sc], ps[kind], host_pool[sc][off:off + n], h_hits[hring:hring + n])
+            hring += n
+        else:
+            if ring + n > ring_rays:
+                ring = 0
+            t = srv.submit_device(scenes[sc], ps[kind], pool[sc].ptr + off * 32, n, d_hits.ptr + ring * 48)
+            ring += n
         with cond:
-            tickets.append((t, n))
+            tickets.append((t, n, host))
             cond.notify()
     with cond:
         done[0] = True
@@ -725,21 +756,30 @@ def serving_bench(nb, hip, ctx, scene_a, seconds, load, seed=11, n_streams=4, ma
     th.join()
     st = srv.stats()
     srv.close()
-    lat = np.array([(i["complete_ns"] - i["submit_ns"]) * 1e-6 for i, _ in infos])
-    t_first = min(i["submit_ns"] for i, _ in infos)
-    t_last = max(i["complete_ns"] for i, _ in infos)
-    rays = sum(n for _, n in infos)
+    lat = np.array([(i["complete_ns"] - i["submit_ns"]) * 1e-6 for i, _, _ in infos])
+    is_host = np.array([h for _, _, h in infos])
+    t_first = min(i["submit_ns"] for i, _, _ in infos)
+    t_last = max(i["complete_ns"] for i, _, _ in infos)
+    rays = sum(n for _, n, _ in infos)
     mrays = rays / ((t_last - t_first) * 1e-9) / 1e6
     big.close()
+    for h in pinned + [pinned_hits]:
+        h.free()
+    by_kind = {}
+    for name, m in (("device_requests", ~is_host), ("host_requests", is_host)):
+        if m.any():
+            by_kind[name] = {"requests": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 3),
+                             "p99_ms": round(float(np.percentile(lat[m], 99)), 3)}
     return {
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "mean_ms": round(float(lat.mean()), 3), "mrays_per_s": round(mrays, 1), "requests": len(infos),
+        "mean_ms": round(float(lat.mean()), 3), **by_kind, "mrays_per_s": round(mrays, 1), "requests": len(infos),
         "rays": int(rays), "offered_requests_per_s": round(rate, 1), "load": load,
         "mix_capacity_mrays_per_s": round(capacity / 1e6, 1),
         "requests_per_launch": round(st["n_batched_requests"] / max(1, st["n_launches"]), 2),
         "workload": "3 resident scenes (81,920 / 2.17M / 8.4M triangles; N-BVH 256 / 4096 / 16384), FP16, "
                     "Poisson arrivals, lognormal sizes median 64k in [4k, 2M], 70% closest / 30% shadow any-hit, "
-                    f"device-resident rays, {n_streams} streams, launches of at most {max_batch_rays} rays",
+                    f"{100 - round(100 * host_fraction)}% device-resident / {round(100 * host_fraction)}% host-resident "
+                    f"rays, {n_streams} streams, launches of at most {max_batch_rays} rays",
     }
 
 
@@ -1025,7 +1065,9 @@ def main():
         result["serving"] = sv
         result["p99_ms"] = sv["p99_ms"]
         log(f"serving: p50 {sv['p50_ms']} ms, p99 {sv['p99_ms']} ms, {sv['mrays_per_s']} Mrays/s, "
-            f"{sv['requests']} requests, {sv['requests_per_launch']} requests per launch")
+            f"{sv['requests']} requests, {sv['requests_per_launch']} requests per launch; "
+            + ", ".join(f"{k}: {v['requests']} p50 {v['p50_ms']} p99 {v['p99_ms']} ms" for k, v in sv.items()
+                        if k.endswith("_requests")))
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

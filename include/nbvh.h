@@ -163,10 +163,9 @@ enum nbvh_query_flags {
      * code of the origin (per part of a multi-part launch) before the launch. Hits stay at each
      * ray's own index and are identical; worthwhile for incoherent batches (bounces, shadow rays).
      * Uses stream-ordered transient device memory (hipMallocAsync) of about 24 B per ray, so a
-     * sorted query is not meant for HIP graph capture. nbvh_query, nbvh_query_host and
-     * nbvh_query_parts apply it by themselves to batches of at least 65,536 rays that carry no
-     * image_width hint, except while the stream is captured and unless NBVH_QUERY_ARRAY_ORDER is
-     * set (nbvh_query_noalloc sorts only when asked). */
+     * sorted query is not meant for HIP graph capture. A scene whose launch shape has ray_order 1
+     * (nbvh_autotune sets it where sorting was faster) gets it without the flag for batches of at
+     * least 65,536 rays that carry no image_width hint (nbvh_query_noalloc sorts only when asked). */
     NBVH_QUERY_SORT_RAYS = 1u,
     /* Neural kinds: evaluate in sorted wavefront rounds instead of the persistent kernel, the
      * step-loop schedule of NeuralPathModule::neural_bvh_traversal
@@ -179,7 +178,7 @@ enum nbvh_query_flags {
      * 4-byte reads) and uses stream-ordered transient memory of about 300 B per ray; not for
      * graph capture. */
     NBVH_QUERY_WAVEFRONT = 2u,
-    /* Trace the rays in array order: no automatic coherence order (see NBVH_QUERY_SORT_RAYS). */
+    /* Trace the rays in array order, whatever the scene's launch shape says (ray_order). */
     NBVH_QUERY_ARRAY_ORDER = 4u
 };
 
@@ -364,23 +363,36 @@ typedef struct nbvh_query_part {
 int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* scene, const nbvh_query_params* params,
                          const nbvh_query_part* parts, uint32_t n_parts, void* stream);
 
-/* Launch shape of the persistent query kernels: how many workgroups per CU stay resident and
- * into how many bands the ray queue is split (each XCD starts on its own band, so neighbouring
- * rays share an L2). Results never depend on it. Kept per scene and precision. */
+/* Launch shape of the persistent query kernels: how many workgroups per CU stay resident, into
+ * how many bands the ray queue is split (each XCD starts on its own band, so neighbouring rays
+ * share an L2) and the order in which batches without an image-order hint are traced. Results
+ * never depend on it. Kept per scene and precision. */
 typedef struct nbvh_launch_shape {
     uint32_t blocks_per_cu; /* 0: as many as fit */
     uint32_t queue_bands;   /* 1..8; 0: 8 */
+    uint32_t ray_order;     /* 0: array order; 1: batches of at least 65,536 rays with image_width 0 and
+                             * neither NBVH_QUERY_SORT_RAYS nor NBVH_QUERY_ARRAY_ORDER are traced in
+                             * the coherence order of NBVH_QUERY_SORT_RAYS (not while the stream is
+                             * captured) */
+    uint32_t reserved;
 } nbvh_launch_shape;
 
 int32_t nbvh_scene_set_launch_shape(nbvh_scene* scene, uint32_t precision, const nbvh_launch_shape* shape);
 int32_t nbvh_scene_get_launch_shape(const nbvh_scene* scene, uint32_t precision, nbvh_launch_shape* shape);
 
 /* Autotuner: times the candidate launch shapes (workgroups per CU in {max, 1, 2, 3} x bands in
- * {1, 2, 4, 8}) of `params` on the given device rays and keeps the fastest for the scene and
- * params->precision. Synchronous on `stream`; d_hits receives the last run's (shape-independent)
- * hits. best / best_ms may be NULL. */
+ * {1, 2, 4, 8}) of `params` on the given device rays, then, for a batch the coherence order
+ * applies to (see ray_order), the fastest shape again with the rays sorted (the sort timed
+ * too), and keeps the fastest for the scene and params->precision. Synchronous on `stream`;
+ * d_hits receives the last run's (shape-independent) hits. best / best_ms may be NULL. */
 int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_query_params* params, const nbvh_ray* d_rays,
                       uint32_t n_rays, nbvh_hit* d_hits, void* stream, nbvh_launch_shape* best, float* best_ms);
+/* The same over the parts of an nbvh_query_parts launch (the batch the caller will run, e.g. a
+ * bounce's closest-hit and shadow rays together). No counterpart in the reference, whose launch
+ * shapes are fixed at compile time. */
+int32_t nbvh_autotune_parts(nbvh_ctx* ctx, nbvh_scene* scene, const nbvh_query_params* params,
+                            const nbvh_query_part* parts, uint32_t n_parts, void* stream, nbvh_launch_shape* best,
+                            float* best_ms);
 
 /* Hybrid query over one neural object and a scene of ordinary triangle meshes (the reference's
  * hybrid TLAS of the neural BLAS and mesh BLASes, tlas_traversal_and_fill_next_hybrid_traversal,

@@ -339,20 +339,20 @@ class NeuralBVHModule:
     @staticmethod
     def params(kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0, max_traversal_steps=0,
                visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False, array_order=False):
-        """nbvh_query_params. Batches of >= 65,536 rays without image_width are traced in the
-        coherence order unless array_order=True (sort_rays=True asks for it at any size)."""
+        """nbvh_query_params. sort_rays: trace in the coherence order; array_order: in array order
+        even where the scene's launch shape (ray_order, set by autotune) would sort."""
         flags = ((_ffi.QUERY_SORT_RAYS if sort_rays else 0) | (_ffi.QUERY_WAVEFRONT if wavefront else 0) |
                  (_ffi.QUERY_ARRAY_ORDER if array_order else 0))
         return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, flags)
 
     def neural_bvh_traversal(self, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0,
                              max_traversal_steps=0, visibility_threshold=0.0, image_width=0, sort_rays=False,
-                             wavefront=False):
+                             wavefront=False, array_order=False):
         """Host rays (RAY_DTYPE) -> host hits (HIT_DTYPE); synchronous."""
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
         p = self.params(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, sort_rays,
-                        wavefront)
+                        wavefront, array_order)
         self.ctx.check(_lib().nbvh_query_host(self.ctx.handle, self.scene.handle, ctypes.byref(p), _ptr(rays),
                                               len(rays), _ptr(hits)))
         return hits
@@ -371,22 +371,24 @@ class NeuralBVHModule:
                                                  ctypes.c_void_p(stream or 0)))
 
     def autotune(self, params, d_rays_ptr, n, d_hits_ptr, stream=None):
-        """Time the candidate launch shapes on these device rays; keep the fastest for the scene and
-        precision. Returns ((blocks_per_cu, queue_bands), ms)."""
+        """Time the candidate launch shapes (and, for batches without an image order, the
+        coherence order) on these device rays; keep the fastest for the scene and precision.
+        Returns ((blocks_per_cu, queue_bands, ray_order), ms)."""
         best, ms = _ffi.LaunchShape(), ctypes.c_float()
         self.ctx.check(_lib().nbvh_autotune(self.ctx.handle, self.scene.handle, ctypes.byref(params),
                                             ctypes.c_void_p(d_rays_ptr), n, ctypes.c_void_p(d_hits_ptr),
                                             ctypes.c_void_p(stream or 0), ctypes.byref(best), ctypes.byref(ms)))
-        return (best.blocks_per_cu, best.queue_bands), ms.value
+        return (best.blocks_per_cu, best.queue_bands, best.ray_order), ms.value
 
-    def set_launch_shape(self, precision, blocks_per_cu=0, queue_bands=0):
-        shape = _ffi.LaunchShape(blocks_per_cu, queue_bands)
+    def set_launch_shape(self, precision, blocks_per_cu=0, queue_bands=0, ray_order=0):
+        shape = _ffi.LaunchShape(blocks_per_cu, queue_bands, ray_order)
         self.ctx.check(_lib().nbvh_scene_set_launch_shape(self.scene.handle, precision, ctypes.byref(shape)))
 
     def launch_shape(self, precision):
+        """(blocks_per_cu, queue_bands, ray_order) of the scene for this precision."""
         shape = _ffi.LaunchShape()
         self.ctx.check(_lib().nbvh_scene_get_launch_shape(self.scene.handle, precision, ctypes.byref(shape)))
-        return shape.blocks_per_cu, shape.queue_bands
+        return shape.blocks_per_cu, shape.queue_bands, shape.ray_order
 
     def segment_labels(self, rays, mint, maxt):
         """Ground-truth targets [n, 8] of host segments (rays clipped to [mint, maxt)) from the
@@ -494,14 +496,27 @@ class NeuralBVHModule:
         img = accum.download(np.float32, width * height * 4).reshape(height, width, 4)
         return img, rays
 
-    def query_parts_device(self, params, parts, stream=None):
-        """One launch over several device ray arrays: parts = [(d_rays_ptr, n, d_hits_ptr, any_hit)].
-        params.kind selects the family (neural or triangle); any_hit is per part."""
+    @staticmethod
+    def _parts(parts):
         arr = (_ffi.QueryPart * len(parts))()
         for i, (r, n, h, any_hit) in enumerate(parts):
             arr[i] = _ffi.QueryPart(r, h, n, 1 if any_hit else 0)
-        self.ctx.check(_lib().nbvh_query_parts(self.ctx.handle, self.scene.handle, ctypes.byref(params), arr,
-                                               len(parts), ctypes.c_void_p(stream or 0)))
+        return arr
+
+    def query_parts_device(self, params, parts, stream=None):
+        """One launch over several device ray arrays: parts = [(d_rays_ptr, n, d_hits_ptr, any_hit)].
+        params.kind selects the family (neural or triangle); any_hit is per part."""
+        self.ctx.check(_lib().nbvh_query_parts(self.ctx.handle, self.scene.handle, ctypes.byref(params),
+                                               self._parts(parts), len(parts), ctypes.c_void_p(stream or 0)))
+
+    def autotune_parts(self, params, parts, stream=None):
+        """autotune over a multi-part launch (parts as for query_parts_device).
+        Returns ((blocks_per_cu, queue_bands, ray_order), ms)."""
+        best, ms = _ffi.LaunchShape(), ctypes.c_float()
+        self.ctx.check(_lib().nbvh_autotune_parts(self.ctx.handle, self.scene.handle, ctypes.byref(params),
+                                                  self._parts(parts), len(parts), ctypes.c_void_p(stream or 0),
+                                                  ctypes.byref(best), ctypes.byref(ms)))
+        return (best.blocks_per_cu, best.queue_bands, best.ray_order), ms.value
 
     def network_inference(self, inputs, precision=PREC_FP32):
         """inputs float32 [n, 3P] -> (outputs [n, 8] post-sigmoid, logits [n, 8])."""

@@ -126,7 +126,8 @@ LEARN_INPUTS, LEARN_TARGETS, LEARN_NODES, LEARN_TREE_CUT, LEARN_SPLITS, LEARN_NB
 
 
 class LaunchShape(ctypes.Structure):
-    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("queue_bands", ctypes.c_uint32)]
+    _fields_ = [("blocks_per_cu", ctypes.c_uint32), ("queue_bands", ctypes.c_uint32), ("ray_order", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -227,6 +228,8 @@ SIGNATURES = {
     "nbvh_scene_get_launch_shape": [P, U32, ctypes.POINTER(LaunchShape)],
     "nbvh_autotune": [P, P, ctypes.POINTER(QueryParams), P, U32, P, P, ctypes.POINTER(LaunchShape),
                       ctypes.POINTER(ctypes.c_float)],
+    "nbvh_autotune_parts": [P, P, ctypes.POINTER(QueryParams), ctypes.POINTER(QueryPart), U32, P,
+                            ctypes.POINTER(LaunchShape), ctypes.POINTER(ctypes.c_float)],
     "nbvh_network_inference": [P, P, U32, P, U32, P, P, P],
     "nbvh_network_inference_host": [P, P, U32, P, U32, P, P],
     "nbvh_server_create": [P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)],

@@ -257,8 +257,16 @@ size_t query_scratch_bytes(uint32_t n)
 
 constexpr uint32_t kQueryFlags = NBVH_QUERY_SORT_RAYS | NBVH_QUERY_WAVEFRONT | NBVH_QUERY_ARRAY_ORDER;
 // Batches of at least this many rays with no image-order hint are traced in the coherence order
-// unless NBVH_QUERY_ARRAY_ORDER is set (4M incoherent rays: 28.6 ms in array order, 20.5 ms sorted).
+// when the scene's launch shape asks for it (ray_order 1, set by the autotuner where it was
+// faster: 4M incoherent rays 28.6 ms in array order, 20.7 ms sorted; 1M camera rays are slower
+// sorted) and NBVH_QUERY_ARRAY_ORDER is not set.
 constexpr uint32_t kAutoSortRays = 1u << 16;
+
+bool auto_sort_applies(const nbvh_query_params* p, uint32_t n)
+{
+    return (p->flags & (NBVH_QUERY_SORT_RAYS | NBVH_QUERY_ARRAY_ORDER)) == 0u && p->image_width == 0 &&
+           n >= kAutoSortRays;
+}
 
 void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const RayBatch& batch,
                hipStream_t stream, SlotLease* lease, void* scratch)
@@ -274,8 +282,8 @@ void run_query(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, c
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     hip_check(hipStreamIsCapturing(stream, &cs), "hipStreamIsCapturing");
     bool sort = (p->flags & NBVH_QUERY_SORT_RAYS) != 0u;
-    if (!sort && (p->flags & NBVH_QUERY_ARRAY_ORDER) == 0u && p->image_width == 0 && batch.live == nullptr &&
-        batch.n >= kAutoSortRays && cs == hipStreamCaptureStatusNone)
+    if (!sort && p->precision <= NBVH_PREC_FP8 && s->shape[p->precision].ray_order == 1u &&
+        auto_sort_applies(p, batch.n) && batch.live == nullptr && cs == hipStreamCaptureStatusNone)
         sort = true;
     if (!sort || batch.n < 2 || batch.order != nullptr) {
         run_query_ordered(ctx, s, p, batch, stream, lease);
@@ -1008,6 +1016,34 @@ NBVH_API int32_t nbvh_query_noalloc(nbvh_ctx* ctx, const nbvh_scene* s, const nb
     });
 }
 
+namespace nbvh {
+namespace {
+
+// Queue one multi-part query (part table into a scratch slot, one launch).
+void run_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p, const nbvh_query_part* parts,
+               uint32_t n_parts, hipStream_t stream)
+{
+    ensure_part_tables(ctx);
+    SlotLease lease(ctx, stream);
+    BatchPart* h = lease.host_parts();
+    BatchPart* d = lease.device_parts();
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_parts; i++) {
+        require(parts[i].n_rays == 0 || (parts[i].rays && parts[i].hits), "NULL ray or hit buffer in a part");
+        require(parts[i].any_hit <= 1u, "any_hit must be 0 or 1");
+        h[i] = BatchPart{parts[i].rays, parts[i].hits, (uint32_t)total, parts[i].n_rays, parts[i].any_hit, 0u};
+        total += parts[i].n_rays;
+    }
+    require(total < (1ull << 32), "more than 2^32 - 1 rays in one query");
+    if (total == 0) return;
+    hip_check(hipMemcpyAsync(d, h, n_parts * sizeof(BatchPart), hipMemcpyHostToDevice, stream), "part table H2D");
+    run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, stream, &lease);
+    lease.finish(stream);
+}
+
+}  // namespace
+}  // namespace nbvh
+
 NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh_query_params* p,
                                   const nbvh_query_part* parts, uint32_t n_parts, void* stream)
 {
@@ -1016,23 +1052,7 @@ NBVH_API int32_t nbvh_query_parts(nbvh_ctx* ctx, const nbvh_scene* s, const nbvh
         require(s->ctx == ctx, "scene belongs to another context");
         require(n_parts >= 1 && n_parts <= kMaxBatchParts, "n_parts must be in 1..64");
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        ensure_part_tables(ctx);
-        SlotLease lease(ctx, (hipStream_t)stream);
-        BatchPart* h = lease.host_parts();
-        BatchPart* d = lease.device_parts();
-        uint64_t total = 0;
-        for (uint32_t i = 0; i < n_parts; i++) {
-            require(parts[i].n_rays == 0 || (parts[i].rays && parts[i].hits), "NULL ray or hit buffer in a part");
-            require(parts[i].any_hit <= 1u, "any_hit must be 0 or 1");
-            h[i] = BatchPart{parts[i].rays, parts[i].hits, (uint32_t)total, parts[i].n_rays, parts[i].any_hit, 0u};
-            total += parts[i].n_rays;
-        }
-        require(total < (1ull << 32), "more than 2^32 - 1 rays in one query");
-        if (total == 0) return;
-        hip_check(hipMemcpyAsync(d, h, n_parts * sizeof(BatchPart), hipMemcpyHostToDevice, (hipStream_t)stream),
-                  "part table H2D");
-        run_query(ctx, s, p, RayBatch{nullptr, nullptr, d, n_parts, (uint32_t)total}, (hipStream_t)stream, &lease);
-        lease.finish((hipStream_t)stream);
+        run_parts(ctx, s, p, parts, n_parts, (hipStream_t)stream);
     });
 }
 
@@ -1042,6 +1062,7 @@ NBVH_API int32_t nbvh_scene_set_launch_shape(nbvh_scene* s, uint32_t precision, 
         require(s && shape, "NULL argument");
         require(precision <= NBVH_PREC_FP8, "unknown precision");
         require(shape->queue_bands <= 8, "queue_bands must be 0..8");
+        require(shape->ray_order <= 1, "ray_order must be 0 or 1");
         s->shape[precision] = *shape;
     });
 }
@@ -1055,9 +1076,77 @@ NBVH_API int32_t nbvh_scene_get_launch_shape(const nbvh_scene* s, uint32_t preci
     });
 }
 
+namespace nbvh {
+namespace {
+
 // Launch-shape autotuner: persistent workgroups per CU x ray-queue bands, each candidate timed
-// with events over `reps` launches on the caller's rays; the fastest is kept for the scene and
-// precision. The shape only changes which wave traces which ray, never a result.
+// with events over kReps launches of `run` (one query of n rays under the scene's current shape);
+// then the winner in the coherence order where that applies to the batch. The fastest is kept
+// for the scene and precision. The shape only changes which wave traces which ray, never a result.
+template <typename Run>
+void autotune_batch(nbvh_scene* s, const nbvh_query_params* p, uint32_t n, hipStream_t st, const Run& run,
+                    nbvh_launch_shape* best, float* best_ms)
+{
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        fail(NBVH_ERR_HIP, "hipEventCreate failed");
+    }
+    const nbvh_launch_shape saved = s->shape[p->precision];
+    nbvh_launch_shape win = saved;
+    float win_ms = FLT_MAX;
+    constexpr int kReps = 3;
+    auto time_shape = [&](const nbvh_query_params* q) {
+        run(q);  // warm-up
+        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < kReps; r++) run(q);
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms / kReps;
+    };
+    try {
+        // Shapes in array order (unless the caller asked for the coherence order), then the
+        // winner in the coherence order where that would apply by default.
+        nbvh_query_params arr = *p;
+        if ((arr.flags & NBVH_QUERY_SORT_RAYS) == 0u) arr.flags |= NBVH_QUERY_ARRAY_ORDER;
+        for (uint32_t bpc : {0u, 1u, 2u, 3u})
+            for (uint32_t bands : {1u, 2u, 4u, 8u}) {
+                s->shape[p->precision] = nbvh_launch_shape{bpc, bands, 0u, 0u};
+                const float ms = time_shape(&arr);
+                if (ms < win_ms) {
+                    win_ms = ms;
+                    win    = s->shape[p->precision];
+                }
+            }
+        if (auto_sort_applies(p, n)) {
+            nbvh_query_params srt = *p;
+            srt.flags |= NBVH_QUERY_SORT_RAYS;
+            s->shape[p->precision] = win;
+            const float ms = time_shape(&srt);
+            if (ms < win_ms) {
+                win_ms        = ms;
+                win.ray_order = 1u;
+            }
+        }
+    } catch (...) {
+        s->shape[p->precision] = saved;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    s->shape[p->precision] = win;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best) *best = win;
+    if (best_ms) *best_ms = win_ms;
+}
+
+}  // namespace
+}  // namespace nbvh
+
 NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_params* p, const nbvh_ray* d_rays,
                                uint32_t n, nbvh_hit* d_hits, void* stream, nbvh_launch_shape* best, float* best_ms)
 {
@@ -1068,45 +1157,28 @@ NBVH_API int32_t nbvh_autotune(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_pa
         require(n > 0 && d_rays && d_hits, "empty ray batch");
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         hipStream_t st = (hipStream_t)stream;
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        if (hipEventCreate(&e1) != hipSuccess) {
-            (void)hipEventDestroy(e0);
-            fail(NBVH_ERR_HIP, "hipEventCreate failed");
-        }
-        const nbvh_launch_shape saved = s->shape[p->precision];
-        nbvh_launch_shape win = saved;
-        float win_ms = FLT_MAX;
-        constexpr int kReps = 3;
-        try {
-            for (uint32_t bpc : {0u, 1u, 2u, 3u})
-                for (uint32_t bands : {1u, 2u, 4u, 8u}) {
-                    s->shape[p->precision] = nbvh_launch_shape{bpc, bands};
-                    const RayBatch batch{d_rays, d_hits, nullptr, 0, n};
-                    run_query(ctx, s, p, batch, st);  // warm-up
-                    hip_check(hipEventRecord(e0, st), "hipEventRecord");
-                    for (int r = 0; r < kReps; r++) run_query(ctx, s, p, batch, st);
-                    hip_check(hipEventRecord(e1, st), "hipEventRecord");
-                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                    float ms = 0.f;
-                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                    ms /= kReps;
-                    if (ms < win_ms) {
-                        win_ms = ms;
-                        win    = s->shape[p->precision];
-                    }
-                }
-        } catch (...) {
-            s->shape[p->precision] = saved;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            throw;
-        }
-        s->shape[p->precision] = win;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (best) *best = win;
-        if (best_ms) *best_ms = win_ms;
+        const RayBatch batch{d_rays, d_hits, nullptr, 0, n};
+        autotune_batch(s, p, n, st, [&](const nbvh_query_params* q) { run_query(ctx, s, q, batch, st); }, best,
+                       best_ms);
+    });
+}
+
+NBVH_API int32_t nbvh_autotune_parts(nbvh_ctx* ctx, nbvh_scene* s, const nbvh_query_params* p,
+                                     const nbvh_query_part* parts, uint32_t n_parts, void* stream,
+                                     nbvh_launch_shape* best, float* best_ms)
+{
+    return guarded(ctx, [&] {
+        require(ctx && s && p && parts, "NULL argument");
+        require(s->ctx == ctx, "scene belongs to another context");
+        require(p->precision <= NBVH_PREC_FP8, "unknown precision");
+        require(n_parts >= 1 && n_parts <= kMaxBatchParts, "n_parts must be in 1..64");
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n_parts; i++) total += parts[i].n_rays;
+        require(total > 0 && total < (1ull << 32), "the parts hold no rays or more than 2^32 - 1");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        hipStream_t st = (hipStream_t)stream;
+        autotune_batch(s, p, (uint32_t)total, st,
+                       [&](const nbvh_query_params* q) { run_parts(ctx, s, q, parts, n_parts, st); }, best, best_ms);
     });
 }
 

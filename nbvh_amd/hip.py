@@ -55,6 +55,8 @@ def _runtime():
             "hipGraphLaunch": [vp, vp],
             "hipGraphExecDestroy": [vp],
             "hipGraphDestroy": [vp],
+            "hipHostMalloc": [ctypes.POINTER(vp), sz, ctypes.c_uint],
+            "hipHostFree": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -122,6 +124,32 @@ class Buffer:
     def free(self):
         if self.ptr:
             _runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostArray:
+    """Pinned (page-locked) host memory from hipHostMalloc, seen as a numpy array of `count`
+    elements of `dtype` (.array); copies from and to it run as asynchronous DMA."""
+
+    def __init__(self, dtype, count):
+        dtype = np.dtype(dtype)
+        self.nbytes = int(count) * dtype.itemsize
+        p = ctypes.c_void_p()
+        check(_runtime().hipHostMalloc(ctypes.byref(p), max(self.nbytes, 1), 0), "hipHostMalloc")
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype, int(count))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            _runtime().hipHostFree(self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -119,9 +119,10 @@ def test_sphere_field_fp16_mixed_launch(nb, orc, ctx):
     both = (kc == 1) & (nb.hit_kind(e_c) == 1) & (ref_c[sample]["node_id"] == e_c["node_id"])
     rel = np.abs(ref_c[sample]["t"][both] - e_c["t"][both]) / np.maximum(e_c["t"][both], 1e-3)
     assert both.sum() > 100 and np.percentile(rel, 99) < 1e-2
-    # The autotuner times launch shapes on the closest-hit rays and keeps the fastest for the scene.
-    shape, ms = mod.autotune(p, dc.ptr, n, hc.ptr, stream=s.handle)
-    assert ms > 0
+    # The autotuner times launch shapes and the ray order on the two-part launch itself and keeps
+    # the fastest for the scene; the records stay those of the first launch.
+    shape, ms = mod.autotune_parts(p, parts, stream=s.handle)
+    assert ms > 0 and mod.launch_shape(nb.PREC_FP16) == shape
     hc.zero()
     hs.zero()
     mod.query_parts_device(p, parts, stream=s.handle)

@@ -404,22 +404,28 @@ def test_scatter_diffuse_matches_oracle(nb, orc, bench, kind):
 
 
 def test_launch_shapes_and_autotune_keep_results(nb, orc, bench):
-    """Every launch shape (workgroups per CU x queue bands) gives identical hits; the autotuner
-    keeps one of the candidates."""
+    """Every launch shape (workgroups per CU x queue bands x ray order) gives identical hits; the
+    autotuner keeps one of the candidates; a 65,536-ray batch without an image order is sorted
+    where the shape says so, and NBVH_QUERY_ARRAY_ORDER keeps array order either way."""
     from nbvh_amd import hip
     mod = bench["module"]
-    _, rays = camera_rays(nb, orc, 128, 128)
+    _, rays = camera_rays(nb, orc, 256, 256)
+    rng = np.random.default_rng(4)
+    rays = rays[rng.permutation(len(rays))]  # incoherent order
     dr, dh = hip.Buffer.from_array(rays), hip.Buffer(len(rays) * 48)
-    ref = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    ref = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16, array_order=True)
     saved = mod.launch_shape(nb.PREC_FP16)
-    for bpc, bands in [(1, 1), (2, 8), (1, 4), (3, 2)]:
-        mod.set_launch_shape(nb.PREC_FP16, bpc, bands)
-        assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands)
+    for bpc, bands, order in [(1, 1, 0), (2, 8, 1), (1, 4, 0), (3, 2, 1), (0, 8, 1)]:
+        mod.set_launch_shape(nb.PREC_FP16, bpc, bands, order)
+        assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands, order)
         assert hits_equal(mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16), ref)
-    (bpc, bands), ms = mod.autotune(mod.params(precision=nb.PREC_FP16), dr.ptr, len(rays), dh.ptr)
-    assert bpc in (0, 1, 2, 3) and bands in (1, 2, 4, 8) and ms > 0
-    assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands)
+        assert hits_equal(mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16, array_order=True), ref)
+    (bpc, bands, order), ms = mod.autotune(mod.params(precision=nb.PREC_FP16), dr.ptr, len(rays), dh.ptr)
+    assert bpc in (0, 1, 2, 3) and bands in (1, 2, 4, 8) and order in (0, 1) and ms > 0
+    assert mod.launch_shape(nb.PREC_FP16) == (bpc, bands, order)
     assert hits_equal(dh.download(nb.HIT_DTYPE), ref)
+    with pytest.raises(nb.NbvhError):
+        mod.set_launch_shape(nb.PREC_FP16, 0, 0, 2)
     mod.set_launch_shape(nb.PREC_FP16, *saved)
 
 

@@ -143,16 +143,19 @@ def test_nbvh_cut_invariants(nb):
         assert set(cut["prim_count"][leaves].tolist()) <= {1, 2}
         assert np.all(cut["prim_count"][~leaves] == 0)
         assert np.array_equal(cut[0]["aabb_min"], nodes[0]["aabb_min"])
-        # Every triangle lies inside at least one leaf box.
-        c = v[t].mean(axis=1)
+        # Every triangle (all three vertices) lies inside one leaf box.
+        tmin, tmax = v[t].min(axis=1), v[t].max(axis=1)
         inside = np.zeros(len(t), bool)
         for lf in cut[leaves]:
-            inside |= np.all((c >= lf["aabb_min"]) & (c <= lf["aabb_max"]), axis=1)
+            inside |= np.all((tmin >= lf["aabb_min"]) & (tmax <= lf["aabb_max"]), axis=1)
         assert inside.all()
-        # Children adjacent, after the parent; boxes nested.
+        # Children adjacent, after the parent; each child's box inside its parent's.
         for i in np.nonzero(~leaves)[0]:
             l = cut[i]["left_or_prim_offset"]
             assert l > i and l + 1 < len(cut)
+            for c in (l, l + 1):
+                assert np.all(cut[c]["aabb_min"] >= cut[i]["aabb_min"])
+                assert np.all(cut[c]["aabb_max"] <= cut[i]["aabb_max"])
     # Asking for more leaves than the BVH has gives the whole tree (mesh leaves: LEAF_FINAL).
     tiny_v, tiny_t = nb.icosphere(0)
     tn, _ = nb.bvh_build(tiny_v, tiny_t)
