@@ -270,6 +270,28 @@ def train_bench(nb, hip, ctx, oracle, batch=1 << 16, steps=100, pool=1 << 20):
            "table_gradient": {"terms_per_step": adds, "gpu_atomic_adds_per_s": round(ceiling, 1),
                               "global_atomics_floor_ms": round(adds / ceiling * 1e3, 3),
                               "step_vs_that_floor": round(ms / (adds / ceiling * 1e3), 3)}}
+    # Step time with larger hash tables (same segments): up to 2^22 entries per level the slice
+    # scans, above that the hashed levels' global atomics.
+    big = {}
+    for log2 in (20, 21, 22, 23):
+        sc2 = nb.Scene(ctx, v, t)
+        sc2.build_nbvh(256)
+        sc2.init_network(nb.network_config(log2_hashmap_size=log2), seed=1, table_range=1e-2)
+        tr2 = nb.Trainer(ctx, sc2, max_batch=batch)
+        s2 = hip.Stream()
+        tr2.step_device(dx.ptr, dt.ptr, batch, stream=s2.handle, want_loss=False)
+        f0, f1 = hip.Event(), hip.Event()
+        f0.record(s2)
+        for k in range(1, 11):
+            tr2.step_device(dx.ptr + int(offs[k]) * 36, dt.ptr + int(offs[k]) * 32, batch, stream=s2.handle,
+                            want_loss=False)
+        f1.record(s2)
+        s2.synchronize()
+        big[f"2^{log2}"] = round(f0.elapsed_ms(f1) / 10, 3)
+        tr2.close()
+        s2.destroy()
+        sc2.close()
+    out["ms_per_step_by_table_size"] = dict({"2^19": out["ms_per_step"]}, **big)
     log(f"training: {out['ms_per_step']} ms per step of {batch} ({out['samples_per_s']} M samples/s), loss "
         f"{out['loss_first']} -> {out['loss_last']}, held-out visibility accuracy {acc0:.3f} -> {acc1:.3f}; "
         f"the {adds / 1e6:.1f} M table-gradient terms as global atomic adds ({ceiling / 1e9:.1f} G adds/s) "

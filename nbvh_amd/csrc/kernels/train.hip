@@ -329,6 +329,51 @@ __global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArg
     }
 }
 
+// Table gradient of the levels whose slices would be too many to scan the batch once per slice
+// (large hash tables): one thread per (sample, point) pair and such level adds w_c * dx[f] of its
+// 8 corners with global atomics. levels[0..count) lists them.
+struct AtomicLevels {
+    uint32_t count;
+    uint32_t level[kMaxLevels];
+};
+
+template <int P, int L, int F>
+__global__ __launch_bounds__(256) void train_scatter_atomic_kernel(TrainArgs t, AtomicLevels al,
+                                                                   const float* __restrict__ in,
+                                                                   const float* __restrict__ act,
+                                                                   float* __restrict__ gtab, uint32_t n)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pairs = (uint64_t)n * P;
+    if (g >= pairs * al.count) return;
+    const uint32_t k = (uint32_t)(g / pairs), jj = (uint32_t)(g - (uint64_t)k * pairs);
+    const uint32_t l = al.level[k], i = jj / P, p = jj - i * P;
+    const GridLevel& lv = t.net.level[l];
+    float pt[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) pt[c] = in[(size_t)i * (3 * P) + 3 * p + c];
+    uint32_t gc[3], idx[8];
+    float fr[3];
+    grid_cell(lv, pt[0], pt[1], pt[2], gc, fr);
+    if (!lv.dense)
+        hash_corners(lv, gc, idx);
+    else if (in_unit_cube<3>(pt))
+        grid_corners<false>(lv, gc, idx);
+    else
+        grid_corners<true>(lv, gc, idx);
+    const size_t S = t.act_stride;
+    float dx[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) dx[f] = act[(size_t)(kDX + (p * L + l) * F + f) * S + i];
+    float* base = gtab + (size_t)lv.offset * F;
+#pragma unroll
+    for (uint32_t c = 0; c < 8; c++) {
+        const float w = corner_weight(c, fr);
+#pragma unroll
+        for (int f = 0; f < F; f++) atomicAdd(base + (size_t)idx[c] * F + f, w * dx[f]);
+    }
+}
+
 // Partial dW[o][k] = sum over samples b in [b0, b1) of delta_b[o] * act_b[k] for one matrix
 // (O outputs, I inputs): tiles of 32 samples of the O delta rows and the I activation rows are
 // staged in LDS (sm, (O + I) x 33 floats), each thread accumulates an MO x MK block.
@@ -470,15 +515,31 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
     const dim3 grid((n + 63) / 64), block(64), grid_mlp((dev::kMlp + 255) / 256);
     if (a.n_levels > (uint32_t)kMaxLevels || a.n_features == 0) return hipErrorInvalidValue;
     // Parts per slice: about 32K corners per workgroup if the batch's corners spread evenly over
-    // the level (a skewed batch only makes some workgroups slower).
+    // the level (a skewed batch only makes some workgroups slower). A level of more than
+    // kMaxSlices slices (above 2^22 entries at 2 features, 2^21 at 4) scans the batch too often:
+    // its corners go to global atomics instead, and it gets no slice items. Measured per
+    // 65,536-sample step of the 16 x 2 grid, slices / atomics: 2^21 1.93 / 2.89 ms, 2^22 2.82 /
+    // 3.28 ms, 2^23 4.21 / 3.83 ms.
+#ifndef NBVH_MAX_SLICES
+#define NBVH_MAX_SLICES 256
+#endif
+    constexpr uint32_t kMaxSlices = NBVH_MAX_SLICES;
     dev::SliceMap map{};
+    dev::AtomicLevels atomic{};
     for (uint32_t l = 0; l < a.n_levels; l++) {
         const uint32_t E = dev::kSliceFloats / a.n_features, size = a.level[l].size;
         const uint32_t slices = (size + E - 1) / E;
         const double corners = (double)n * a.n_points * 8.0 * std::min(E, size) / size;
+        if (slices > kMaxSlices) {
+            map.parts[l]     = 1;
+            map.first[l + 1] = map.first[l];
+            atomic.level[atomic.count++] = l;
+            continue;
+        }
         map.parts[l]     = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(corners / 32768.0)));
         map.first[l + 1] = map.first[l] + slices * map.parts[l];
     }
+    const uint64_t atomic_threads = (uint64_t)n * a.n_points * atomic.count;
     bool shape = false;
 #define X(P_, L_, F_)                                                                                                \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                                                \
@@ -487,9 +548,12 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
         hipLaunchKernelGGL((dev::train_forward_kernel<P_, L_, F_>), grid, block, 0, stream, t, t.mlp, in, act, n);    \
         hipLaunchKernelGGL((dev::train_backward_kernel<P_, L_, F_>), grid, block, 0, stream, t, (const float*)wt, in, \
                            targets, act, n);                                                                         \
-        hipLaunchKernelGGL((dev::train_scatter_kernel<P_, L_, F_>), dim3(map.first[L_]), dim3(dev::kScatterThreads), 0, \
-                           stream, t, map,                                                                           \
-                           in, (const float*)act, t.grad_tables, n);                                                 \
+        if (map.first[L_] > 0)                                                                                       \
+            hipLaunchKernelGGL((dev::train_scatter_kernel<P_, L_, F_>), dim3(map.first[L_]), dim3(dev::kScatterThreads), \
+                               0, stream, t, map, in, (const float*)act, t.grad_tables, n);                          \
+        if (atomic_threads > 0)                                                                                      \
+            hipLaunchKernelGGL((dev::train_scatter_atomic_kernel<P_, L_, F_>), dim3((unsigned)((atomic_threads + 255) / 256)), \
+                               dim3(256), 0, stream, t, atomic, in, (const float*)act, t.grad_tables, n);            \
     }
     X(3, 16, 2) X(3, 8, 4)
 #undef X

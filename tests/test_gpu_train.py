@@ -107,6 +107,29 @@ def test_chess_shape_gradients_match_oracle(nb, orc, ctx, setup, n):
     scene.close()
 
 
+def test_large_table_gradients_match_oracle(nb, orc, ctx, setup):
+    """The chess shape with hash tables of 2^22 entries: each hashed level would be 512 slices of
+    128 KB, so its corners go to global atomics instead of slice scans (the dense levels keep the
+    slices); the gradients still match the FP64 restatement."""
+    cfg = nb.network_config(n_levels=8, n_features_per_level=4, log2_hashmap_size=22, base_resolution=8,
+                            per_level_scale=2.0)
+    v, t, _, _, _ = bench_scene(nb)
+    scene = nb.Scene(ctx, v, t)
+    scene.build_nbvh(256)
+    scene.init_network(cfg, seed=6, table_range=1e-2)
+    _, tables, mlp = scene.network()
+    lo, hi = scene.bbox()
+    net = orc.Net(cfg, tables, mlp, lo, hi)
+    rays, mint, maxt = nb.leaf_segments(scene.nbvh(), 1500, seed=5)
+    x = net.encode(rays, mint, maxt, mode=orc.FP32)
+    targets = nb.NeuralBVHModule(ctx, scene).segment_labels(rays, mint, maxt)
+    lv = net.levels()
+    hashed = lv["dense"] == 0
+    assert hashed.sum() == 3 and np.all(lv["size"][hashed] == 1 << 22)
+    _check_gradients(nb, ctx, scene, cfg, tables, mlp, lv, x, targets)
+    scene.close()
+
+
 def test_adam_step_matches_oracle(nb, orc, ctx, setup):
     from oracle import train
     x, t = setup["x"][:2048], setup["t"][:2048]
