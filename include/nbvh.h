@@ -120,6 +120,14 @@ enum nbvh_segment_distribution {
 /* Network: segment -> P points -> hash grid -> MLP (no biases, ReLU hidden, Sigmoid output).
  * include/cuda/neural/neural_bvh_module.h:79-131 (segment encoding) and
  * include/cuda/neural/utils/tcnn_config.h:228-264 (FullyFusedMLP). */
+/* How the depth output decodes into a hit distance (DepthLearningMode,
+ * include/cuda/neural/neural_bvh_module_common.cuh:23-150; the reference's Position and
+ * LocalPosition modes need a 3-wide depth output and are not provided). */
+enum nbvh_depth_mode {
+    NBVH_DEPTH_LOCAL_T = 0, /* along the segment: mint + u * (length + 1e-3) (the reference's default) */
+    NBVH_DEPTH_WORLD_T = 1  /* along the ray's span through the neural scene's box instead of the segment */
+};
+
 typedef struct nbvh_network_config {
     nbvh_grid_config grid;
     uint32_t n_points;             /* P segment points, 1..4 (reference: 3) */
@@ -127,6 +135,15 @@ typedef struct nbvh_network_config {
     uint32_t n_neurons;            /* hidden width, 64 */
     uint32_t n_hidden_layers;      /* 4 (=> 5 matrices) */
     uint32_t n_outputs;            /* 8: vis, depth, normal3, albedo3 */
+    /* Decoding (pt_target_const, include/cuda/neural/neural_bvh_module.h:243-276); all zero is the
+     * reference's default and the only form the trainer takes. */
+    uint32_t depth_mode;              /* nbvh_depth_mode */
+    uint32_t depth_linear_activation; /* output_depth_linear_activation: u = clamp((y - 0.45) * 10, 0, 1) */
+    uint32_t interpolate_depth;       /* interpolate_depth (closest hit): an estimate within
+                                       * depth_interpolation_eps of the current neural hit is averaged
+                                       * with it, normals summed and renormalised
+                                       * (src/cuda/neural/path/path_inference_kernels.cu:89-121) */
+    float    depth_interpolation_eps; /* 0 => 1e-3 */
 } nbvh_network_config;
 
 /* Arithmetic of the neural query / network inference. */

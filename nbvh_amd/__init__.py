@@ -15,7 +15,7 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY, HIT_NONE, HIT_NEURAL, HIT_TRIANGLE,
                    FLAG_STACK_OVERFLOW, FLAG_STEP_CAP, NO_NEURAL_CHILD, SEGMENT_EQUIDISTANT_EXTERIOR,
                    SEGMENT_EQUIDISTANT_INTERIOR, BACKGROUND_CONSTANT, BACKGROUND_SKY, TRAIN_TABLES, TRAIN_MLP,
-                   TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP)
+                   TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP, DEPTH_LOCAL_T, DEPTH_WORLD_T)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
 __all__ = ["Context", "Scene", "NeuralBVHModule", "Server", "Trainer", "Learner", "flip", "NbvhError", "network_config", "icosphere", "bvh_build",
@@ -37,11 +37,14 @@ def hit_segments(hits):
 
 def network_config(n_levels=16, n_features_per_level=2, log2_hashmap_size=19, base_resolution=16,
                    per_level_scale=1.5, n_points=3, segment_distribution=SEGMENT_EQUIDISTANT_EXTERIOR,
-                   n_neurons=64, n_hidden_layers=4, n_outputs=8):
-    """nbvh_network_config; defaults are the benchmark network (16 levels x 2 features, 2^19)."""
+                   n_neurons=64, n_hidden_layers=4, n_outputs=8, depth_mode=0, depth_linear_activation=False,
+                   interpolate_depth=False, depth_interpolation_eps=0.0):
+    """nbvh_network_config; defaults are the benchmark network (16 levels x 2 features, 2^19) with
+    the reference's default decoding (LocalT depth, sigmoid outputs, no depth interpolation)."""
     return NetworkConfig(GridConfig(n_levels, n_features_per_level, log2_hashmap_size, base_resolution,
                                     per_level_scale), n_points, segment_distribution, n_neurons, n_hidden_layers,
-                         n_outputs)
+                         n_outputs, depth_mode, 1 if depth_linear_activation else 0, 1 if interpolate_depth else 0,
+                         depth_interpolation_eps)
 
 
 # ------------------------------------------------------------------------------------------

@@ -36,6 +36,7 @@ QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_A
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
 SEGMENT_EQUIDISTANT_INTERIOR, SEGMENT_EQUIDISTANT_EXTERIOR = 0, 1
+DEPTH_LOCAL_T, DEPTH_WORLD_T = 0, 1
 NO_NEURAL_CHILD = 0xFFFFFFFF
 
 # ---- numpy record layouts (identical to the C structs) ----
@@ -56,7 +57,9 @@ class GridConfig(ctypes.Structure):
 class NetworkConfig(ctypes.Structure):
     _fields_ = [("grid", GridConfig), ("n_points", ctypes.c_uint32), ("segment_distribution", ctypes.c_uint32),
                 ("n_neurons", ctypes.c_uint32), ("n_hidden_layers", ctypes.c_uint32),
-                ("n_outputs", ctypes.c_uint32)]
+                ("n_outputs", ctypes.c_uint32), ("depth_mode", ctypes.c_uint32),
+                ("depth_linear_activation", ctypes.c_uint32), ("interpolate_depth", ctypes.c_uint32),
+                ("depth_interpolation_eps", ctypes.c_float)]
 
 
 class QueryParams(ctypes.Structure):

@@ -402,6 +402,19 @@ __device__ __forceinline__ float decode_depth(float u, float mint, float maxt)
     return u * (len + 1e-3f) + mint;
 }
 
+// WorldT decode (neural_bvh_module_common.cuh:58-84): the same on the ray's span [smin, smax]
+// through the neural scene's box; `inside` is the segment's negative mint.
+__device__ __forceinline__ float decode_span(float u, float smin, float smax, bool inside)
+{
+    const float len = smax - smin;
+    if (inside) {
+        const float inward = u * (len + 1e-3f);
+        if (inward < smax - len * kDepthEpsilon - kFixedDepthEpsilon) return smax - inward;
+        return FLT_MAX;
+    }
+    return u * (len + 1e-3f) + smin;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Watertight ray/triangle (Woop, Benthin, Wald, JCGT 2(1) 2013) with FP64 recomputation of the
 // edge functions when one is exactly zero; accepted interval [tmin, tmax) as

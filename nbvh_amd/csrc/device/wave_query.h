@@ -87,15 +87,32 @@ __device__ __forceinline__ nbvh_hit make_hit(const HitState& s, bool any_hit)
 // (neural_bvh_module_common.cuh:160-209,497-550, path_inference_kernels.cu:61-131): visibility
 // decision on the logit, LocalT depth decode, closest depth kept with its normal and albedo.
 // Returns true when the ray is finished (an any-hit query found an occluding segment).
-__device__ __forceinline__ bool apply_segment(const NetArgs& a, HitState& s, bool any_hit, float seg_mint,
-                                              float seg_maxt, uint32_t seg_node, const float z[8])
+// Decoded hit distance of a segment's depth output y (post-sigmoid): the network's depth mode
+// (LocalT on the segment, WorldT on the ray's span through the neural scene's box) and, with the
+// linear activation, u = clamp((y - 0.45) * 10, 0, 1) (decode_depth_estimate,
+// include/cuda/neural/neural_bvh_module_common.cuh:23-84). FLT_MAX = rejected.
+__device__ __forceinline__ float decode_segment_depth(const NetArgs& a, const Ray& r, float y, float seg_mint,
+                                                      float seg_maxt)
+{
+    const float u = a.depth_linear ? fminf(fmaxf((y - 0.45f) * 10.f, 0.f), 1.f) : y;
+    if (a.depth_mode == NBVH_DEPTH_WORLD_T) {
+        const Box b{a.bbox_min[0], a.bbox_min[1], a.bbox_min[2], a.bbox_max[0], a.bbox_max[1], a.bbox_max[2]};
+        float smin, smax;
+        slab(b, r, smin, smax);
+        return decode_span(u, smin, smax, seg_mint < 0.f);
+    }
+    return decode_depth(u, seg_mint, seg_maxt);
+}
+
+__device__ __forceinline__ bool apply_segment(const NetArgs& a, HitState& s, bool any_hit, const Ray& r,
+                                              float seg_mint, float seg_maxt, uint32_t seg_node, const float z[8])
 {
     s.segments++;
     if (!(z[0] > a.thr_logit)) return false;
     const bool inside = seg_mint < 0.f;
     if (any_hit) {
         bool h = true;
-        if (inside) h = decode_depth(sigmoidf_cw(z[1]), seg_mint, seg_maxt) != FLT_MAX;
+        if (inside) h = decode_segment_depth(a, r, sigmoidf_cw(z[1]), seg_mint, seg_maxt) != FLT_MAX;
         if (h) {
             s.kind      = NBVH_HIT_NEURAL;
             s.hit_node  = seg_node;
@@ -103,8 +120,14 @@ __device__ __forceinline__ bool apply_segment(const NetArgs& a, HitState& s, boo
         }
         return h;
     }
-    const float depth = decode_depth(sigmoidf_cw(z[1]), seg_mint, seg_maxt);
-    if (depth != FLT_MAX && depth < s.best_t) {
+    const float depth = decode_segment_depth(a, r, sigmoidf_cw(z[1]), seg_mint, seg_maxt);
+    if (depth == FLT_MAX) return false;
+    // interpolate_depth: an estimate within interp_eps of the current neural hit is averaged with it
+    // (process_neural_bvh_traversal_step, src/cuda/neural/path/path_inference_kernels.cu:89-121).
+    const bool near = a.interpolate_depth && s.kind == NBVH_HIT_NEURAL && fabsf(s.best_t - depth) <= a.interp_eps;
+    if (depth < s.best_t || near) {
+        const float prev_t = s.best_t;
+        const float prev_n[3] = {s.nrm[0], s.nrm[1], s.nrm[2]};
         s.best_t    = depth;
         s.kind      = NBVH_HIT_NEURAL;
         s.hit_node  = seg_node;
@@ -118,6 +141,15 @@ __device__ __forceinline__ bool apply_segment(const NetArgs& a, HitState& s, boo
         const float inv = 1.f / sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
 #pragma unroll
         for (int c = 0; c < 3; c++) s.nrm[c] = v[c] * inv;
+        if (near) {
+            s.best_t = 0.5f * (prev_t + depth);
+            float m[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) m[c] = s.nrm[c] + prev_n[c];
+            const float im = 1.f / sqrtf((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) s.nrm[c] = m[c] * im;
+        }
     }
     return false;
 }
@@ -282,7 +314,7 @@ template <bool PARTS>
 __device__ __forceinline__ void consume_segment(const NetArgs& a, Slot& s, const float z[8], const RayBatch& b)
 {
     s.state = SLOT_RAY;
-    if (apply_segment(a, s.h, s.any_hit, s.seg_mint, s.seg_maxt, s.seg_node, z)) finish_ray<PARTS>(a, s, b);
+    if (apply_segment(a, s.h, s.any_hit, s.ray, s.seg_mint, s.seg_maxt, s.seg_node, z)) finish_ray<PARTS>(a, s, b);
 }
 
 }  // namespace dev

@@ -112,6 +112,8 @@ void refresh_args(nbvh_scene* s)
     for (int c = 0; c < 3; c++) {
         a.enc_center[c] = (s->bbox_max[c] + s->bbox_min[c]) * 0.5f;
         a.enc_extent[c] = (s->bbox_max[c] - s->bbox_min[c]) * dil;
+        a.bbox_min[c]   = s->bbox_min[c];
+        a.bbox_max[c]   = s->bbox_max[c];
     }
     a.thr_logit = 0.f;
     a.max_steps = kDefaultMaxSteps;
@@ -124,6 +126,10 @@ void refresh_args(nbvh_scene* s)
     a.n_neurons       = cfg.n_neurons;
     a.n_hidden_layers = cfg.n_hidden_layers;
     a.n_outputs       = cfg.n_outputs;
+    a.depth_mode        = cfg.depth_mode;
+    a.depth_linear      = cfg.depth_linear_activation;
+    a.interpolate_depth = cfg.interpolate_depth;
+    a.interp_eps        = cfg.depth_interpolation_eps > 0.f ? cfg.depth_interpolation_eps : 1e-3f;
     const uint32_t P  = cfg.n_points;
     float delta, offset;
     if (cfg.segment_distribution == NBVH_SEGMENT_EQUIDISTANT_EXTERIOR) {

@@ -31,6 +31,11 @@ void network_validate(const nbvh_network_config& c)
     if (c.n_neurons != 64) fail(NBVH_ERR_UNSUPPORTED, "network: n_neurons must be 64");
     if (c.n_hidden_layers < 1 || c.n_hidden_layers > 8) fail(NBVH_ERR_UNSUPPORTED, "network: n_hidden_layers must be 1..8");
     if (c.n_outputs != 8) fail(NBVH_ERR_UNSUPPORTED, "network: n_outputs must be 8 (vis, depth, normal3, albedo3)");
+    if (c.depth_mode > NBVH_DEPTH_WORLD_T) fail(NBVH_ERR_UNSUPPORTED, "network: depth_mode must be LocalT or WorldT");
+    if (c.depth_linear_activation > 1 || c.interpolate_depth > 1)
+        fail(NBVH_ERR_INVALID_ARGUMENT, "network: depth_linear_activation and interpolate_depth are 0 or 1");
+    if (!(c.depth_interpolation_eps >= 0.f) || !std::isfinite(c.depth_interpolation_eps))
+        fail(NBVH_ERR_INVALID_ARGUMENT, "network: depth_interpolation_eps must be finite and >= 0");
 }
 
 std::vector<GridLevel> grid_levels(const nbvh_grid_config& g)

@@ -30,7 +30,9 @@ bool trainable(const nbvh_network_config& c)
 {
     const bool shape = (c.n_points == 3 && c.grid.n_levels == 16 && c.grid.n_features_per_level == 2) ||
                        (c.n_points == 3 && c.grid.n_levels == 8 && c.grid.n_features_per_level == 4);
-    return shape && c.n_neurons == 64 && c.n_hidden_layers == 4 && c.n_outputs == 8;
+    // The loss trains the LocalT depth through the sigmoid output (the reference's defaults).
+    return shape && c.n_neurons == 64 && c.n_hidden_layers == 4 && c.n_outputs == 8 &&
+           c.depth_mode == NBVH_DEPTH_LOCAL_T && c.depth_linear_activation == 0;
 }
 
 // Learning-rate factor of update number `step` (from 1) under the exponential decay: decay_base to

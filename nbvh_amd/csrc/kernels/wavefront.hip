@@ -155,7 +155,9 @@ __global__ __launch_bounds__(256) void wf_consume_kernel(NetArgs a, RayBatch b, 
 #pragma unroll
     for (int k = 0; k < 8; k++) z[k] = logits[(size_t)j * 8 + k];
     HitState h = st[g].h;
-    if (apply_segment(a, h, any, sg.mint, sg.maxt, sg.node, z)) {
+    Ray r{};
+    if (a.depth_mode == NBVH_DEPTH_WORLD_T) r = load_ray(b.parts ? b.parts[part].rays : b.rays, local);
+    if (apply_segment(a, h, any, r, sg.mint, sg.maxt, sg.node, z)) {
         store_hit(b.parts ? b.parts[part].hits : b.hits, local, make_hit(h, any));
         return;
     }

@@ -54,6 +54,11 @@ struct NetArgs {
     float    fp8_wscale[8];            // per-layer weight dequantisation scales (FP8 variant)
     uint32_t n_dense;                  // levels [n_dense, n_levels) are all hashed (grid resolution
                                        // grows with the level, so the dense levels are a prefix)
+    uint32_t depth_mode;               // nbvh_depth_mode
+    uint32_t depth_linear;             // depth output read as clamp((y - 0.45) * 10, 0, 1)
+    uint32_t interpolate_depth;        // closest hit: average estimates within interp_eps
+    float    interp_eps;
+    float    bbox_min[3], bbox_max[3]; // neural scene box (WorldT decode)
     GridLevel level[kMaxLevels];
 };
 

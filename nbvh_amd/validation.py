@@ -165,6 +165,9 @@ def settings(config):
                                                                             "EquidistantExterior"))
     if dist is None:
         raise ValueError(f"segment_distribution {bm.get('segment_distribution')} is not supported")
+    depth_mode = {"LocalT": 0, "WorldT": 1}.get(pt.get("depth_learning_mode", "LocalT"))
+    if depth_mode is None:
+        raise ValueError(f"depth_learning_mode {pt.get('depth_learning_mode')} is not supported")
     return dict(
         mode=r.get("validation_mode", "Ignore"),
         base=r.get("base_config_name", "config"), variant=r.get("variant_config_name", "variant"),
@@ -181,7 +184,10 @@ def settings(config):
                                   per_level_scale=float(grid.get("per_level_scale", 2.0)),
                                   n_points=int(bm.get("input_segment_points", 3)), segment_distribution=dist,
                                   n_neurons=int(net.get("n_neurons", 64)),
-                                  n_hidden_layers=int(net.get("n_hidden_layers", 4))),
+                                  n_hidden_layers=int(net.get("n_hidden_layers", 4)), depth_mode=depth_mode,
+                                  depth_linear_activation=bool(pt.get("output_depth_linear_activation", False)),
+                                  interpolate_depth=bool(pt.get("interpolate_depth", False)),
+                                  depth_interpolation_eps=float(pt.get("depth_interpolation_eps", 1e-3))),
         max_depth=int(nm.get("max_depth", 3)),
         primary_threshold=float(pt.get("primary_visibility_threshold", 0.5)),
         secondary_threshold=float(pt.get("secondary_visibility_threshold", 0.5)),

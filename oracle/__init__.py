@@ -30,7 +30,9 @@ class OracleNet(ctypes.Structure):
                 ("distribution", ctypes.c_uint32), ("n_neurons", ctypes.c_uint32),
                 ("n_hidden_layers", ctypes.c_uint32), ("n_outputs", ctypes.c_uint32),
                 ("tables", ctypes.c_void_p), ("mlp", ctypes.c_void_p),
-                ("bbox_min", ctypes.c_float * 3), ("bbox_max", ctypes.c_float * 3)]
+                ("bbox_min", ctypes.c_float * 3), ("bbox_max", ctypes.c_float * 3),
+                ("depth_mode", ctypes.c_uint32), ("depth_linear", ctypes.c_uint32),
+                ("interpolate_depth", ctypes.c_uint32), ("interp_eps", ctypes.c_float)]
 
 
 def build():
@@ -92,7 +94,10 @@ class Net:
         self.s = OracleNet(g.n_levels, g.n_features_per_level, g.log2_hashmap_size, g.base_resolution,
                            g.per_level_scale, cfg.n_points, cfg.segment_distribution, cfg.n_neurons,
                            cfg.n_hidden_layers, cfg.n_outputs, self.tables.ctypes.data, self.mlp.ctypes.data,
-                           (ctypes.c_float * 3)(*bbox_min), (ctypes.c_float * 3)(*bbox_max))
+                           (ctypes.c_float * 3)(*bbox_min), (ctypes.c_float * 3)(*bbox_max),
+                           getattr(cfg, "depth_mode", 0), getattr(cfg, "depth_linear_activation", 0),
+                           getattr(cfg, "interpolate_depth", 0),
+                           getattr(cfg, "depth_interpolation_eps", 0.0) or 1e-3)
         self.cfg = cfg
 
     @property

@@ -294,6 +294,10 @@ typedef struct oracle_net {
     const float *tables;
     const float *mlp;
     float        bbox_min[3], bbox_max[3];
+    // Decoding (pt_target_const, include/cuda/neural/neural_bvh_module.h:243-276): depth mode
+    // (0 LocalT, 1 WorldT), linear depth activation, depth interpolation and its epsilon.
+    uint32_t     depth_mode, depth_linear, interpolate_depth;
+    float        interp_eps;
 } oracle_net;
 }
 
@@ -498,6 +502,40 @@ inline float decode_depth(double u, float mint, float maxt)
     return (float)(u * ((double)len + 1e-3) + (double)mint);
 }
 
+// WorldT (neural_bvh_module_common.cuh:58-84): the LocalT formulas on the ray's span [smin, smax]
+// through the neural scene's box (intersect_bbox_min_max without the ray's own bounds,
+// include/cuda/base/ray.cuh:116-132); `inside` is the segment's negative mint.
+template <typename R>
+float decode_span(R u, float smin, float smax, bool inside)
+{
+    const float len = smax - smin;
+    if (inside) {
+        const float inward = sizeof(R) == 4 ? (float)u * (len + 1e-3f) : (float)((double)u * ((double)len + 1e-3));
+        if (inward < smax - len * DEPTH_EPSILON - FIXED_DEPTH_EPSILON) return smax - inward;
+        return FLT_MAX;
+    }
+    return sizeof(R) == 4 ? (float)u * (len + 1e-3f) + smin : (float)((double)u * ((double)len + 1e-3) + (double)smin);
+}
+
+// The hit distance a segment's depth output y (post-sigmoid) decodes to under the network's
+// depth mode and activation (decode_depth_estimate, neural_bvh_module_common.cuh:23-84).
+template <typename R>
+float decode_segment(const oracle_net &net, const Ray &ray, R y, float mint, float maxt)
+{
+    R u = y;
+    if (net.depth_linear) {
+        u = (y - (R)0.45f) * (R)10.f;
+        u = u > (R)0 ? u : (R)0;
+        u = u < (R)1 ? u : (R)1;
+    }
+    if (net.depth_mode == 1) {
+        float smin, smax;
+        slab(net.bbox_min, net.bbox_max, ray, smin, smax);
+        return decode_span<R>(u, smin, smax, mint < 0.f);
+    }
+    return decode_depth(u, mint, maxt);
+}
+
 struct SegRecord {
     uint32_t ray;
     uint32_t node;
@@ -514,6 +552,7 @@ void neural_ray(const oracle_net &net, const std::vector<Level> &lv, const Node 
 {
     hit = Hit{FLT_MAX, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, {0, 0, 0}, 0.f, {0, 0, 0}, 0.f};
     float    best_t = ray.tmax;
+    R        cur_n[3] = {0, 0, 0};  // the current hit's normal in the evaluation precision
     uint32_t stack[NEURAL_STACK_SIZE];
     int      ss    = 1;
     stack[0]       = 0;
@@ -542,7 +581,7 @@ void neural_ray(const oracle_net &net, const std::vector<Level> &lv, const Node 
             const bool inside = mint < 0.f;
             if (any) {
                 bool h = true;
-                if (inside) h = decode_depth(sig(z[1]), mint, maxt) != FLT_MAX;
+                if (inside) h = decode_segment<R>(net, ray, sig(z[1]), mint, maxt) != FLT_MAX;
                 if (h) {
                     kind        = 1;
                     hit.node_id = node;
@@ -550,8 +589,13 @@ void neural_ray(const oracle_net &net, const std::vector<Level> &lv, const Node 
                     break;
                 }
             } else {
-                const float depth = decode_depth(sig(z[1]), mint, maxt);
-                if (depth != FLT_MAX && depth < best_t) {
+                const float depth = decode_segment<R>(net, ray, sig(z[1]), mint, maxt);
+                // interpolate_depth (path_inference_kernels.cu:89-121): an estimate within the
+                // epsilon of the current neural hit is averaged with it, normals summed.
+                const bool near = depth != FLT_MAX && net.interpolate_depth && kind == 1 &&
+                                  std::fabs(best_t - depth) <= net.interp_eps;
+                if (depth != FLT_MAX && (depth < best_t || near)) {
+                    const float prev_t = best_t;
                     best_t      = depth;
                     kind        = 1;
                     hit.node_id = node;
@@ -562,9 +606,21 @@ void neural_ray(const oracle_net &net, const std::vector<Level> &lv, const Node 
                         a[c] = sig(z[5 + c]);
                     }
                     const R inv = (R)1 / std::sqrt((s[0] * s[0] + s[1] * s[1]) + s[2] * s[2]);
+                    R n[3];
                     for (int c = 0; c < 3; c++) {
-                        hit.normal_or_uv[c] = (float)(s[c] * inv);
-                        hit.albedo[c]       = (float)a[c];
+                        n[c]          = s[c] * inv;
+                        hit.albedo[c] = (float)a[c];
+                    }
+                    if (near) {
+                        best_t = 0.5f * (prev_t + depth);
+                        R m[3];
+                        for (int c = 0; c < 3; c++) m[c] = n[c] + cur_n[c];
+                        const R im = (R)1 / std::sqrt((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]);
+                        for (int c = 0; c < 3; c++) n[c] = m[c] * im;
+                    }
+                    for (int c = 0; c < 3; c++) {
+                        cur_n[c]            = n[c];
+                        hit.normal_or_uv[c] = (float)n[c];
                     }
                 }
             }

@@ -125,6 +125,42 @@ def test_neural_query_fp32_bit_exact(nb, orc, bench, any_hit):
     assert hits_equal(got, exp)
 
 
+@pytest.mark.parametrize("world_t,linear,interp", [(True, False, False), (False, True, True), (True, True, True)])
+def test_decode_options_bit_exact(nb, orc, ctx, bench, world_t, linear, interp):
+    """The network's decode options (WorldT depth, linear depth activation, depth interpolation
+    within 2e-2) on the benchmark scene: FP32 closest and any hit bit-exact with the oracle,
+    the wavefront-round schedule identical to the persistent kernel, FP16 hit/miss within 0.5 %
+    of the FP64 oracle."""
+    cfg = nb.network_config(depth_mode=nb.DEPTH_WORLD_T if world_t else nb.DEPTH_LOCAL_T,
+                            depth_linear_activation=linear, interpolate_depth=interp, depth_interpolation_eps=2e-2)
+    scene = nb.Scene(ctx, bench["v"], bench["t"])
+    scene.build_nbvh(256)
+    scene.init_network(cfg, seed=3, table_range=1e-1)
+    cfg2, tables, mlp = scene.network()
+    assert (cfg2.depth_mode, cfg2.depth_linear_activation, cfg2.interpolate_depth) == (int(world_t), int(linear),
+                                                                                       int(interp))
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg2, tables, mlp, bmin, bmax)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    _, rays = camera_rays(nb, orc, 96, 96)
+    rays = np.concatenate([rays, random_rays(5000, 17, inside_frac=0.4)])
+    for any_hit in (False, True):
+        kind = nb.QUERY_NEURAL_ANY if any_hit else nb.QUERY_NEURAL_CLOSEST
+        got = mod.neural_bvh_traversal(rays, kind=kind, precision=nb.PREC_FP32)
+        exp = net.neural_query(scene.nbvh(), rays, any_hit=any_hit, mode=orc.FP32)
+        assert 0.1 < (nb.hit_kind(exp) == nb.HIT_NEURAL).mean() < 0.95
+        assert hits_equal(got, exp), any_hit
+        wf = mod.neural_bvh_traversal(rays, kind=kind, precision=nb.PREC_FP32, wavefront=True)
+        assert hits_equal(wf, got), any_hit
+    g16 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16)
+    e64 = net.neural_query(scene.nbvh(), rays, mode=orc.FP64)
+    assert (nb.hit_kind(g16) != nb.hit_kind(e64)).mean() <= 0.005
+    if world_t or linear:  # the trainer's loss is for the default decoding only
+        with pytest.raises(nb.NbvhError):
+            nb.Trainer(ctx, scene, max_batch=64)
+    scene.close()
+
+
 def test_neural_query_step_cap_bit_exact(nb, orc, bench):
     rays = random_rays(4000, 8, inside_frac=0.5)
     got = bench["module"].neural_bvh_traversal(rays, max_traversal_steps=1)
