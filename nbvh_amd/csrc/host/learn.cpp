@@ -12,11 +12,25 @@
 #include <vector>
 
 #include "../kernels/launch.h"
+#include <climits>
+
 #include "context.h"
 
 using namespace nbvh;
 
 namespace {
+
+// The scheduler's int arithmetic as the reference's compiled code performs it: its expected node
+// count and split totals overflow int within its own shipped schedules (e.g. interval 40, last
+// 3000, scaling 2.5), which wraps in two's complement; int *= float truncates like x86's
+// cvttss2si, INT_MIN when the product is out of range.
+int wrap_add(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+int wrap_mul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
+int mul_trunc(int a, float f)
+{
+    const float p = (float)a * f;
+    return (p >= -2147483648.f && p < 2147483648.f) ? (int)p : INT_MIN;
+}
 
 // BvhSplitScheduler (include/cuda/neural/utils/bvh_split_scheduler.h:13-144). The products with
 // the float scalings truncate to int, as the reference's int *= float does.
@@ -39,12 +53,12 @@ struct SplitScheduler {
         if (it > last_iteration) return 0;
         int n = 0;
         if (it % interval == 0) {
-            n = cur_num;
-            cur_total += n;
+            n         = cur_num;
+            cur_total = wrap_add(cur_total, n);
         }
         if (it % cur_interval == 0) {
-            cur_num      = (int)((float)cur_num * scaling);
-            cur_interval = (int)((float)cur_interval * interval_scaling);
+            cur_num      = mul_trunc(cur_num, scaling);
+            cur_interval = mul_trunc(cur_interval, interval_scaling);
         }
         return n;
     }
@@ -54,12 +68,12 @@ struct SplitScheduler {
         stats_total_splits = 0;
         int expected_nodes = start_nodes;
         for (int i = 0; i < num_iterations; i++) {
-            const int leaves = (expected_nodes + 1) / 2;
+            const int leaves = wrap_add(expected_nodes, 1) / 2;
             int s = splits(i);
             if (s > 0) {
                 if (s > leaves) s = leaves;
-                stats_total_splits += s;
-                expected_nodes += 2 * s;
+                stats_total_splits = wrap_add(stats_total_splits, s);
+                expected_nodes     = wrap_add(expected_nodes, wrap_mul(2, s));
             }
         }
         reset();
@@ -68,9 +82,9 @@ struct SplitScheduler {
     {
         bool assign = false;
         if (lod_decision == 0)
-            assign = (max_lods - current_lod) * last_iteration / num_lods_learned <= it;
+            assign = wrap_mul(max_lods - current_lod, last_iteration) / num_lods_learned <= it;
         else
-            assign = (max_lods - current_lod) * stats_total_splits / num_lods_learned <= cur_total;
+            assign = wrap_mul(max_lods - current_lod, stats_total_splits) / num_lods_learned <= cur_total;
         return assign || it >= last_iteration;
     }
 };
@@ -333,7 +347,10 @@ void check_config(const nbvh_learner_config& c)
             "sampler parameters out of range");
     require(c.num_lods_learned >= 1 && c.num_lods_learned <= NBVH_NUM_LODS, "num_lods_learned must be 1..7");
     require(c.lod_decision <= 1, "lod_decision must be 0 or 1");
-    require(c.splitting_interval > 0 && c.split_scaling > 0.f && c.split_interval_scaling > 0.f && c.start_split_num >= 0,
+    // An interval scaling below 1 can truncate the schedule's interval to 0 (a modulo by zero in
+    // the reference's scheduler).
+    require(c.splitting_interval > 0 && c.split_scaling > 0.f && c.split_interval_scaling >= 1.f &&
+                c.start_split_num >= 0 && c.last_split_iteration >= 0,
             "split schedule parameters out of range");
     if (c.max_training_bounces > 1)
         fail(NBVH_ERR_UNSUPPORTED, "training bounces (max_training_bounces > 1) are not supported");

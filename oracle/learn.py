@@ -25,8 +25,26 @@ def leaf_lod(lod):
     return 2 + lod
 
 
+def w32(x):
+    """x wrapped to int32 (two's complement), as the reference's compiled int arithmetic."""
+    return ((int(x) + (1 << 31)) % (1 << 32)) - (1 << 31)
+
+
+def mul_trunc(a, f):
+    """int *= float as x86's cvttss2si: truncation toward zero, INT_MIN out of range."""
+    p = np.float32(a) * np.float32(f)
+    return int(p) if -2147483648.0 <= p < 2147483648.0 else -(1 << 31)
+
+
+def c_div(a, b):
+    """C integer division (truncation toward zero)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
 class SplitSchedule:
-    """BvhSplitScheduler with int truncation of the float scalings (int *= float)."""
+    """BvhSplitScheduler with int truncation of the float scalings (int *= float) and its int
+    arithmetic wrapping in two's complement (its node counts overflow within shipped schedules)."""
 
     def __init__(self, splitting_interval=50, last_split_iteration=2000, start_split_num=1, split_scaling=2.0,
                  split_interval_scaling=2.0, lod_decision=0):
@@ -47,10 +65,10 @@ class SplitSchedule:
         n = 0
         if it % self.interval == 0:
             n = self.num
-            self.total += n
+            self.total = w32(self.total + n)
         if it % self.cur_interval == 0:
-            self.num = int(np.float32(self.num) * self.scaling)
-            self.cur_interval = int(np.float32(self.cur_interval) * self.iscaling)
+            self.num = mul_trunc(self.num, self.scaling)
+            self.cur_interval = mul_trunc(self.cur_interval, self.iscaling)
         return n
 
     def update_stats_and_reset(self, start_nodes, num_iterations):
@@ -58,19 +76,19 @@ class SplitSchedule:
         self.stats_total = 0
         nodes = start_nodes
         for i in range(num_iterations):
-            leaves = (nodes + 1) // 2
+            leaves = c_div(w32(nodes + 1), 2)
             s = self.splits(i)
             if s > 0:
                 s = min(s, leaves)
-                self.stats_total += s
-                nodes += 2 * s
+                self.stats_total = w32(self.stats_total + s)
+                nodes = w32(nodes + w32(2 * s))
         self.reset()
 
     def should_assign_new_lod(self, it, current_lod, num_lods_learned, max_lods=NUM_LODS):
         if self.lod_decision == 0:
-            assign = (max_lods - current_lod) * self.last // num_lods_learned <= it
+            assign = c_div(w32((max_lods - current_lod) * self.last), num_lods_learned) <= it
         else:
-            assign = (max_lods - current_lod) * self.stats_total // num_lods_learned <= self.total
+            assign = c_div(w32((max_lods - current_lod) * self.stats_total), num_lods_learned) <= self.total
         return assign or it >= self.last
 
 

@@ -186,3 +186,21 @@ def test_candidates_ties_and_lod(nb, scene):
     assert len(tc.nodes) > n_before
     new = tc.nodes[n_before:]
     assert set(new["prim_count"][new["left_or_prim_offset"] == L.NO_CHILD]) <= {L.LEAF_FINAL, L.leaf_lod(5)}
+
+
+def test_split_schedule_int_arithmetic():
+    """The scheduler's int arithmetic as compiled code performs it: split counts doubling every
+    iteration reach 2^30, then int *= float leaves the int range (INT_MIN, as x86's truncating
+    conversion gives), and the running total wraps in two's complement."""
+    from oracle.learn import SplitSchedule
+    s = SplitSchedule(splitting_interval=1, last_split_iteration=40, start_split_num=1, split_scaling=2.0,
+                      split_interval_scaling=1.0)
+    got = [s.splits(i) for i in range(36)]
+    assert got[:31] == [1 << i for i in range(31)]
+    assert got[31:] == [-(1 << 31)] * 5
+    assert s.total == ((sum(got) + (1 << 31)) % (1 << 32)) - (1 << 31)
+    # Expected nodes from 1: split i adds 2^i (the leaf cap equals it), so after i = 29 the count is
+    # 2^31 - 1; at i = 30, (nodes + 1) / 2 wraps to INT_MIN / 2, the cap turns the 2^30 splits into
+    # -2^30, and the total 2^30 - 1 becomes -1; later counts are negative (no splits).
+    s.update_stats_and_reset(1, 36)
+    assert s.stats_total == -1
