@@ -41,14 +41,17 @@ def oracle_sample(orc, learner, scene, v, t, step, tc, prev, max_loss, n_sub_bat
                             current_max_loss=max_loss if step > 0 else 0.0)
 
 
-@pytest.mark.parametrize("n_sub_batch_log2", [0, 2])
-def test_sampler_and_tree_cut_match_the_restatements(nb, orc, ctx, n_sub_batch_log2):
+@pytest.mark.parametrize("n_sub_batch_log2,lod_decision", [(0, 0), (2, 0), (0, 1)])
+def test_sampler_and_tree_cut_match_the_restatements(nb, orc, ctx, n_sub_batch_log2, lod_decision):
+    """The schedule grows its split count 1.5x per step, so its expected node count leaves the int
+    range within the 60 planned steps: the LoD decisions by equal splits (lod_decision 1) then rest
+    on the wrapped totals, on both sides."""
     from nbvh_amd import _ffi
     from oracle import learn as OL
     v, t, scene = make_scene(nb, ctx)
     batch = 1 << 15
     sched = dict(splitting_interval=1, last_split_iteration=50, start_split_num=3, split_scaling=1.5,
-                 split_interval_scaling=1.0)
+                 split_interval_scaling=1.0, lod_decision=lod_decision)
     lr = nb.Learner(ctx, scene, batch_size=batch, neural_bvh_min_depth=4, log2_bvh_max_traversals=n_sub_batch_log2,
                     num_lods_learned=2, max_training_steps=60, **sched)
     nodes, _ = scene.bvh()
