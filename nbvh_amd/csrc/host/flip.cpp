@@ -16,24 +16,16 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// linear sRGB (D65) -> XYZ.
-constexpr double kRgb2Xyz[9] = {0.4124564, 0.3575761, 0.1804375, 0.2126729, 0.7151522,
-                                0.0721750, 0.0193339, 0.1191920, 0.9503041};
-
-void invert3(const double m[9], double out[9])
-{
-    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
-                       m[2] * (m[3] * m[7] - m[4] * m[6]);
-    out[0] = (m[4] * m[8] - m[5] * m[7]) / det;
-    out[1] = (m[2] * m[7] - m[1] * m[8]) / det;
-    out[2] = (m[1] * m[5] - m[2] * m[4]) / det;
-    out[3] = (m[5] * m[6] - m[3] * m[8]) / det;
-    out[4] = (m[0] * m[8] - m[2] * m[6]) / det;
-    out[5] = (m[2] * m[3] - m[0] * m[5]) / det;
-    out[6] = (m[3] * m[7] - m[4] * m[6]) / det;
-    out[7] = (m[1] * m[6] - m[0] * m[7]) / det;
-    out[8] = (m[0] * m[4] - m[1] * m[3]) / det;
-}
+// The colour tables of the method's reference implementation (figures/flip/flip_api.py:155-205):
+// linear sRGB (D65) -> XYZ as exact ratios, its tabulated inverse, the D65 illuminant and its
+// tabulated reciprocal (XYZ is normalised by multiplying with the reciprocal).
+constexpr double kRgb2Xyz[9] = {10135552.0 / 24577794, 8788810.0 / 24577794, 4435075.0 / 24577794,
+                                2613072.0 / 12288897,  8788810.0 / 12288897, 887015.0 / 12288897,
+                                1425312.0 / 73733382,  8788810.0 / 73733382, 70074185.0 / 73733382};
+constexpr double kXyz2Rgb[9] = {3.241003275,  -1.537398934, -0.498615861, -0.969224334, 1.875930071,
+                                0.041554224,  0.055639423,  -0.204011202, 1.057148933};
+constexpr double kWhite[3] = {0.950428545, 1.0, 1.088900371};
+constexpr double kInvWhite[3] = {1.052156925, 1.0, 0.918357670};
 
 double lab_f(double t)
 {
@@ -44,11 +36,9 @@ double lab_f(double t)
 // Hunt-adjusted CIELab of a linear RGB colour.
 void hunt_lab(const double rgb[3], double lab[3])
 {
-    double white[3], n[3];
-    for (int r = 0; r < 3; r++) {
-        white[r] = kRgb2Xyz[3 * r] + kRgb2Xyz[3 * r + 1] + kRgb2Xyz[3 * r + 2];
-        n[r] = (kRgb2Xyz[3 * r] * rgb[0] + kRgb2Xyz[3 * r + 1] * rgb[1] + kRgb2Xyz[3 * r + 2] * rgb[2]) / white[r];
-    }
+    double n[3];
+    for (int r = 0; r < 3; r++)
+        n[r] = (kRgb2Xyz[3 * r] * rgb[0] + kRgb2Xyz[3 * r + 1] * rgb[1] + kRgb2Xyz[3 * r + 2] * rgb[2]) * kInvWhite[r];
     const double L = 116 * lab_f(n[1]) - 16;
     lab[0] = L;
     lab[1] = 0.01 * L * 500 * (lab_f(n[0]) - lab_f(n[1]));
@@ -126,13 +116,14 @@ NBVH_API int32_t nbvh_flip(nbvh_ctx* ctx, const float* d_reference, const float*
             a.edge = a.csf + 3 * (size_t)nc * nc;
             a.point = a.edge + (size_t)nf * nf;
             a.cmax = (float)cmax;
-            double inv[9];
-            invert3(kRgb2Xyz, inv);
             for (int i = 0; i < 9; i++) {
                 a.rgb2xyz[i] = (float)kRgb2Xyz[i];
-                a.xyz2rgb[i] = (float)inv[i];
+                a.xyz2rgb[i] = (float)kXyz2Rgb[i];
             }
-            for (int r = 0; r < 3; r++) a.white[r] = (float)(kRgb2Xyz[3 * r] + kRgb2Xyz[3 * r + 1] + kRgb2Xyz[3 * r + 2]);
+            for (int r = 0; r < 3; r++) {
+                a.white[r] = (float)kWhite[r];
+                a.inv_white[r] = (float)kInvWhite[r];
+            }
             a.prep_ref = reinterpret_cast<float*>(mem + o_prep);
             a.prep_test = a.prep_ref + 4 * n;
             a.map = d_error_map;

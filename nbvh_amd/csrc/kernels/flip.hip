@@ -1,7 +1,8 @@
 // flip.hip -- LDR-FLIP image error of a rendering against a reference rendering, the metric of the
 // reference's validation mode (compute_and_display_flip_error,
 // src/cuda/neural/neural_bvh_renderer.cu:40-92), written from the published method (Andersson et
-// al., "FLIP: A Difference Evaluator for Alternating Images", HPG 2020):
+// al., "FLIP: A Difference Evaluator for Alternating Images", HPG 2020) with the steps and
+// constants of the method's reference implementation (figures/flip/flip_api.py:439-497):
 //
 //   flip_prepare_kernel  per pixel of both accumulation buffers (rgb sum, sample count): mean
 //                        linear RGB -> sRGB clamped to [0, 1] -> linear RGB -> YyCxCz and Y / Yn;
@@ -31,11 +32,11 @@ __device__ __forceinline__ float srgb_decode(float c)
     return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
 }
 
-// XYZ / white of linear sRGB (D65 primaries).
+// XYZ of linear sRGB (D65 primaries) normalised by the illuminant (times its reciprocal).
 __device__ __forceinline__ void rgb_to_xyz_n(const FlipArgs& a, const float rgb[3], float xyz[3])
 {
     for (int r = 0; r < 3; r++)
-        xyz[r] = (a.rgb2xyz[3 * r] * rgb[0] + a.rgb2xyz[3 * r + 1] * rgb[1] + a.rgb2xyz[3 * r + 2] * rgb[2]) / a.white[r];
+        xyz[r] = (a.rgb2xyz[3 * r] * rgb[0] + a.rgb2xyz[3 * r + 1] * rgb[1] + a.rgb2xyz[3 * r + 2] * rgb[2]) * a.inv_white[r];
 }
 
 __device__ __forceinline__ float lab_f(float t)

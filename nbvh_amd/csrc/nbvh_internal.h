@@ -167,7 +167,7 @@ struct FlipArgs {
     const float* edge;    // x kernel of the edge detector
     const float* point;   // x kernel of the point detector
     float        cmax;    // HyAB^0.7 of Hunt-adjusted green against blue
-    float        rgb2xyz[9], xyz2rgb[9], white[3];
+    float        rgb2xyz[9], xyz2rgb[9], white[3], inv_white[3];
     float*       prep_ref;   // w*h float4
     float*       prep_test;
     float*       map;        // optional w*h error map

@@ -7,9 +7,13 @@ of a 0.7 m wide 3840-pixel monitor seen from 0.7 m, mean of the error map). Its 
 flip-cuda (an empty submodule here) and the Python tool under figures/flip needs OpenCV and
 OpenEXR, which are not importable, so this follows the published method (Andersson, Nilsson,
 Akenine-Moeller, Oskarsson, Astrom, Fairchild: "FLIP: A Difference Evaluator for Alternating
-Images", HPG 2020), in FP64:
+Images", HPG 2020) with every constant and step as the reference's Python tool has it
+(figures/flip/flip_api.py:134-205 colour tables, :275-323 contrast sensitivity kernels, :326-348
+filtering and clamp, :350-398 Hunt, HyAB, remap, :400-437 feature kernels, :439-497 LDR-FLIP),
+in FP64 where that tool computes in FP32:
 
-  colour  sRGB -> linear RGB -> XYZ (sRGB primaries, D65) -> YyCxCz; each channel filtered with
+  colour  sRGB -> linear RGB -> XYZ (sRGB primaries, D65; the tables of the method's own
+          implementation, figures/flip/flip_api.py:134-205) -> YyCxCz; each channel filtered with
           its contrast sensitivity kernel, the sum of Gaussians a1 sqrt(pi/b1) exp(-pi^2 d^2/b1) +
           a2 sqrt(pi/b2) exp(-pi^2 d^2/b2) of the distance d in degrees (achromatic (1, 0.0047,
           0, 1e-5), red-green (1, 0.0053, 0, 1e-5), blue-yellow (34.1, 0.04, 13.5, 0.025)),
@@ -24,8 +28,9 @@ Images", HPG 2020), in FP64:
   error   dEc^(1 - dEf) per pixel; the mean over pixels.
 
 Borders replicate the edge pixel. Inputs: linear RGB, clamped to [0, 1] after conversion to sRGB
-(LDR-FLIP's input range). Parity unpinned: the reference's FLIP implementation itself (not
-runnable here); the oracle is pinned by the method's closed forms (tests/test_oracle_flip.py).
+(LDR-FLIP's input range). The reference's FLIP code is not runnable here (flip-cuda absent, the
+Python tool needs OpenCV), so no output of it pins this; the oracle is pinned by the method's
+closed forms and the colour tables' own identities (tests/test_oracle_flip.py).
 """
 import numpy as np
 from scipy import ndimage
@@ -35,11 +40,17 @@ PPD_REFERENCE = 0.7 * (3840.0 / 0.7) * (np.pi / 180.0)  # calculatePPD(0.7 m, 38
 CSF = {"A": (1.0, 0.0047, 0.0, 1e-5), "RG": (1.0, 0.0053, 0.0, 1e-5), "BY": (34.1, 0.04, 13.5, 0.025)}
 QC, QF, PC, PT, GW = 0.7, 0.5, 0.4, 0.95, 0.082
 
-M_RGB2XYZ = np.array([[0.4124564, 0.3575761, 0.1804375],
-                      [0.2126729, 0.7151522, 0.0721750],
-                      [0.0193339, 0.1191920, 0.9503041]])
-M_XYZ2RGB = np.linalg.inv(M_RGB2XYZ)
-WHITE = M_RGB2XYZ @ np.ones(3)
+# The colour tables of the method's reference implementation (figures/flip/flip_api.py:155-205,
+# color_space_transform): linear sRGB -> XYZ as exact ratios, its tabulated inverse, the D65
+# illuminant and its tabulated reciprocal (XYZ is normalised by multiplying with the reciprocal).
+M_RGB2XYZ = np.array([[10135552 / 24577794, 8788810 / 24577794, 4435075 / 24577794],
+                      [2613072 / 12288897, 8788810 / 12288897, 887015 / 12288897],
+                      [1425312 / 73733382, 8788810 / 73733382, 70074185 / 73733382]])
+M_XYZ2RGB = np.array([[3.241003275, -1.537398934, -0.498615861],
+                      [-0.969224334, 1.875930071, 0.041554224],
+                      [0.055639423, -0.204011202, 1.057148933]])
+WHITE = np.array([0.950428545, 1.0, 1.088900371])
+INV_WHITE = np.array([1.052156925, 1.0, 0.918357670])
 
 
 def linear_to_srgb(c):
@@ -53,7 +64,7 @@ def srgb_to_linear(c):
 
 
 def rgb_to_ycxcz(rgb):
-    xyz = (rgb @ M_RGB2XYZ.T) / WHITE
+    xyz = (rgb @ M_RGB2XYZ.T) * INV_WHITE
     return np.stack([116 * xyz[..., 1] - 16, 500 * (xyz[..., 0] - xyz[..., 1]), 200 * (xyz[..., 1] - xyz[..., 2])],
                     axis=-1)
 
@@ -71,7 +82,7 @@ def _lab_f(t):
 
 
 def rgb_to_lab(rgb):
-    xyz = (rgb @ M_RGB2XYZ.T) / WHITE
+    xyz = (rgb @ M_RGB2XYZ.T) * INV_WHITE
     fx, fy, fz = _lab_f(xyz[..., 0]), _lab_f(xyz[..., 1]), _lab_f(xyz[..., 2])
     return np.stack([116 * fy - 16, 500 * (fx - fy), 200 * (fy - fz)], axis=-1)
 
