@@ -270,8 +270,8 @@ def train_bench(nb, hip, ctx, oracle, batch=1 << 16, steps=100, pool=1 << 20):
            "table_gradient": {"terms_per_step": adds, "gpu_atomic_adds_per_s": round(ceiling, 1),
                               "global_atomics_floor_ms": round(adds / ceiling * 1e3, 3),
                               "step_vs_that_floor": round(ms / (adds / ceiling * 1e3), 3)}}
-    # Step time with larger hash tables (same segments): up to 2^22 entries per level the slice
-    # scans, above that the hashed levels' global atomics.
+    # Step time with larger hash tables (same segments): from 2^21 entries per level the hashed
+    # levels' table gradient is binned, below that scanned once per slice.
     big = {}
     for log2 in (20, 21, 22, 23):
         sc2 = nb.Scene(ctx, v, t)

@@ -533,8 +533,9 @@ typedef struct nbvh_train_config {
     float    beta2;         /* 0.999 */
     float    epsilon;       /* 1e-8 */
     float    l2_reg;        /* 1e-8 */
-    uint32_t max_batch;     /* samples per step, at most 2^24 (720 floats of device scratch each,
-                               plus about 19 MB per trainer) */
+    uint32_t max_batch;     /* samples per step, at most 2^24 (device scratch per sample: 720
+                               floats, plus n_points * n_levels * 8 corner terms of 1 + F floats,
+                               4.5 KB at 3 x 16 x 2; plus about 20 MB per trainer) */
     uint32_t decay_start;   /* 10000 in the reference; 0 with decay_interval 0: no decay */
     uint32_t decay_interval;/* 2000 in the reference; 0: no decay */
     float    decay_base;    /* 0.33 in the reference */

@@ -19,7 +19,7 @@ struct nbvh_trainer {
     uint32_t step    = 0;
     uint64_t net_generation = 0;  // the scene's network generation the buffers were sized for
     NetArgs net{};                // the scene's network arguments at creation (shape and levels)
-    DevBuf tables, mlp, g_tables, g_mlp, m_tables, v_tables, m_mlp, v_mlp, n_tables, n_mlp_steps, act, loss;
+    DevBuf tables, mlp, g_tables, g_mlp, m_tables, v_tables, m_mlp, v_mlp, n_tables, n_mlp_steps, act, bins, loss;
 };
 
 namespace {
@@ -65,6 +65,7 @@ TrainArgs train_args(nbvh_trainer* t)
     a.grad_tables = t->g_tables.as<float>();
     a.grad_mlp    = t->g_mlp.as<float>();
     a.loss        = t->loss.as<float>();
+    a.bins        = t->bins.ptr;
     return a;
 }
 
@@ -101,6 +102,8 @@ NBVH_API int32_t nbvh_trainer_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbv
         t->n_tables.alloc(ctx, t->n_table * sizeof(uint32_t));
         t->n_mlp_steps.alloc(ctx, t->n_mlp * sizeof(uint32_t));
         t->act.alloc(ctx, train_act_bytes(cfg->max_batch));
+        t->bins.alloc(ctx, train_bin_bytes(cfg->max_batch, scene->cfg.n_points, scene->cfg.grid.n_levels,
+                                           scene->cfg.grid.n_features_per_level));
         t->loss.alloc(ctx, 256);
         hip_check(train_zero(t->m_tables.as<float>(), t->n_table, nullptr), "zero");
         hip_check(train_zero(t->v_tables.as<float>(), t->n_table, nullptr), "zero");
@@ -135,7 +138,6 @@ void trainer_step(nbvh_trainer* t, const float* d_inputs, const float* d_targets
     if (n > 0) require(d_inputs && d_targets, "NULL buffer");
     check_generation(t);
     hip_check(hipSetDevice(t->ctx->device), "hipSetDevice");
-    hip_check(train_zero(t->g_tables.as<float>(), t->n_table, st), "zero");
     hip_check(train_zero(t->g_mlp.as<float>(), t->n_mlp, st), "zero");
     hip_check(train_zero(t->loss.as<float>(), 1, st), "zero");
     bool supported = false;

@@ -61,6 +61,7 @@ hipError_t wf_consume(const NetArgs& a, const RayBatch& b, const WfSeg* segs, co
 // encoded inputs (9 floats) and targets (8 floats) for the supported shapes (*supported), Adam.
 uint32_t train_act_stride(uint32_t max_batch);
 size_t   train_act_bytes(uint32_t max_batch);
+size_t   train_bin_bytes(uint32_t max_batch, uint32_t n_points, uint32_t n_levels, uint32_t n_features);
 hipError_t train_zero(float* p, uint64_t n, hipStream_t stream);
 hipError_t train_gradients(const TrainArgs& t, const float* in, const float* targets, uint32_t n, hipStream_t stream,
                            bool* supported);

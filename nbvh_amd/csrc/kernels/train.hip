@@ -15,9 +15,12 @@
 //                    (pt_composite_loss.cuh:19-123, neural_bvh_module.h:250-266), its gradient
 //                    through the Sigmoid outputs and the ReLU layers (rows of W^T), stored per
 //                    layer, down to the feature gradients;
-//   train_scatter    the table gradient, one workgroup per 128 KB slice of a level: the slice is
-//                    summed in LDS from every (sample, point) whose corners fall in it and
-//                    stored whole, so no global atomics are needed;
+//   train_scatter    the table gradient of levels of fewer than 128 slices of 128 KB, one
+//                    workgroup per slice: the slice is summed in LDS from every (sample, point)
+//                    whose corners fall in it and stored whole;
+//   train_bin_*      the table gradient of the other levels: the corner terms binned by slice
+//                    (count, scan, scatter), then one workgroup per slice sums its bin in LDS
+//                    and stores the slice whole; no global atomics on the gradient;
 //   train_wgrad      the matrix gradients dW[o][i] = sum over samples of delta[o] * act[i]: each
 //                    workgroup reduces one chunk of samples for one matrix through LDS tiles of 32
 //                    samples and writes its partial; train_wgrad_reduce adds the partials in
@@ -329,49 +332,224 @@ __global__ __launch_bounds__(kScatterThreads) void train_scatter_kernel(TrainArg
     }
 }
 
-// Table gradient of the levels whose slices would be too many to scan the batch once per slice
-// (large hash tables): one thread per (sample, point) pair and such level adds w_c * dx[f] of its
-// 8 corners with global atomics. levels[0..count) lists them.
-struct AtomicLevels {
-    uint32_t count;
-    uint32_t level[kMaxLevels];
+// Binned table gradient, for levels of 128 or more 128 KB slices: the level is cut into bins of
+// kBinFloats gradient floats; every corner term (entry, w_c * dx[f]) of the level is written to
+// its bin, then one workgroup per bin adds the bin's terms in LDS and stores the bin whole. Four
+// passes:
+//   train_bin_count_kernel    per workgroup a histogram of its pairs' corner bins in LDS, added to
+//                             the bin counts (one global atomic per workgroup and bin touched);
+//   train_bin_scan_kernel     exclusive scan of the counts: bin b's terms at [offsets[b],
+//                             offsets[b + 1]); the cursors start at the offsets;
+//   train_bin_scatter_kernel  the same pairs again: each workgroup reserves its range of every
+//                             bin (one global atomic per bin touched) and writes its terms;
+//   train_bin_accumulate_kernel  one workgroup per bin: LDS sum, the bin stored whole.
+// The batch is read three times and each term twice, instead of one global atomic add per
+// (pair, corner, feature) or one scan of the batch per slice.
+struct BinMap {
+    uint32_t count;                    // binned levels
+    uint32_t level[kMaxLevels];        // their indices
+    uint32_t first[kMaxLevels + 1];    // first bin of binned level k; first[count] = bins
 };
+#ifndef NBVH_BIN_THREADS
+#define NBVH_BIN_THREADS 256
+#endif
+// 8192 pairs: a workgroup's terms reach each bin in runs long enough to write coalesced. Measured
+// per step at 2^21 / 2^22 / 2^23 entries (256 threads): 2048 pairs 1.74 / 2.14 / 2.68 ms, 8192
+// pairs 1.61 / 2.02 / 2.56 ms (1024 threads: 1.59 / 2.07 / 2.62).
+#ifndef NBVH_BIN_PAIRS
+#define NBVH_BIN_PAIRS 8192
+#endif
+constexpr uint32_t kBinThreads = NBVH_BIN_THREADS;
+constexpr uint32_t kBinPairs   = NBVH_BIN_PAIRS;  // (sample, point) pairs per workgroup and level
+constexpr uint32_t kBinFloats  = 16384;   // gradient floats per bin (64 KB of LDS: 2 workgroups per CU)
+constexpr uint32_t kBinWindow  = 8192;    // bins of one level a count / scatter workgroup handles
+constexpr uint32_t kBinUnroll  = 4;       // terms per thread whose loads are issued together
 
-template <int P, int L, int F>
-__global__ __launch_bounds__(256) void train_scatter_atomic_kernel(TrainArgs t, AtomicLevels al,
-                                                                   const float* __restrict__ in,
-                                                                   const float* __restrict__ act,
-                                                                   float* __restrict__ gtab, uint32_t n)
+// Corners and fractional position of pair jj = (sample i, point p) at level lv.
+template <int P>
+__device__ __forceinline__ void pair_corners(const GridLevel& lv, const float* __restrict__ in, uint32_t i, uint32_t p,
+                                             uint32_t idx[8], float fr[3])
 {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t pairs = (uint64_t)n * P;
-    if (g >= pairs * al.count) return;
-    const uint32_t k = (uint32_t)(g / pairs), jj = (uint32_t)(g - (uint64_t)k * pairs);
-    const uint32_t l = al.level[k], i = jj / P, p = jj - i * P;
-    const GridLevel& lv = t.net.level[l];
     float pt[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) pt[c] = in[(size_t)i * (3 * P) + 3 * p + c];
-    uint32_t gc[3], idx[8];
-    float fr[3];
-    grid_cell(lv, pt[0], pt[1], pt[2], gc, fr);
+    uint32_t g[3];
+    grid_cell(lv, pt[0], pt[1], pt[2], g, fr);
     if (!lv.dense)
-        hash_corners(lv, gc, idx);
+        hash_corners(lv, g, idx);
     else if (in_unit_cube<3>(pt))
-        grid_corners<false>(lv, gc, idx);
+        grid_corners<false>(lv, g, idx);
     else
-        grid_corners<true>(lv, gc, idx);
-    const size_t S = t.act_stride;
-    float dx[F];
+        grid_corners<true>(lv, g, idx);
+}
+
+// Workgroup (x, k, w): pairs [x * kBinPairs, ...) of binned level k, the corners in the level's
+// bins [w * kBinWindow, (w + 1) * kBinWindow); hist: one counter per bin of the window (dynamic
+// LDS).
+template <int P, int F>
+__global__ __launch_bounds__(kBinThreads) void train_bin_count_kernel(TrainArgs t, BinMap bm,
+                                                                      const float* __restrict__ in, uint32_t n,
+                                                                      uint32_t* __restrict__ counts)
+{
+    extern __shared__ uint32_t hist[];
+    constexpr uint32_t E = kBinFloats / F;
+    const uint32_t k = blockIdx.y, w0 = blockIdx.z * kBinWindow, level_bins = bm.first[k + 1] - bm.first[k];
+    if (w0 >= level_bins) return;
+    const uint32_t first = bm.first[k] + w0, bins = level_bins - w0 < kBinWindow ? level_bins - w0 : kBinWindow;
+    const GridLevel& lv = t.net.level[bm.level[k]];
+    for (uint32_t j = threadIdx.x; j < bins; j += kBinThreads) hist[j] = 0u;
+    __syncthreads();
+    const uint32_t total = n * (uint32_t)P, j0 = blockIdx.x * kBinPairs;
+    const uint32_t j1 = total - j0 < kBinPairs ? total : j0 + kBinPairs;
+    for (uint32_t jj = j0 + threadIdx.x; jj < j1; jj += kBinThreads) {
+        const uint32_t i = jj / P, p = jj - i * P;
+        uint32_t idx[8];
+        float fr[3];
+        pair_corners<P>(lv, in, i, p, idx, fr);
 #pragma unroll
-    for (int f = 0; f < F; f++) dx[f] = act[(size_t)(kDX + (p * L + l) * F + f) * S + i];
-    float* base = gtab + (size_t)lv.offset * F;
-#pragma unroll
-    for (uint32_t c = 0; c < 8; c++) {
-        const float w = corner_weight(c, fr);
-#pragma unroll
-        for (int f = 0; f < F; f++) atomicAdd(base + (size_t)idx[c] * F + f, w * dx[f]);
+        for (int c = 0; c < 8; c++) {
+            const uint32_t r = idx[c] / E - w0;
+            if (r < bins) atomicAdd(&hist[r], 1u);
+        }
     }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < bins; j += kBinThreads)
+        if (hist[j] != 0u) atomicAdd(&counts[first + j], hist[j]);
+}
+
+// One workgroup: offsets[b] = counts[0] + ... + counts[b - 1] for b <= bins, cursors = offsets.
+__global__ __launch_bounds__(1024) void train_bin_scan_kernel(const uint32_t* __restrict__ counts, uint32_t bins,
+                                                              uint32_t* __restrict__ offsets,
+                                                              uint32_t* __restrict__ cursors)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t tid = threadIdx.x, per = (bins + 1023u) / 1024u;
+    const uint32_t b0 = tid * per < bins ? tid * per : bins, b1 = bins - b0 < per ? bins : b0 + per;
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; b++) sum += counts[b];
+    part[tid] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid > 0 ? part[tid - 1] : 0u;
+    for (uint32_t b = b0; b < b1; b++) {
+        offsets[b] = run;
+        cursors[b] = run;
+        run += counts[b];
+    }
+    if (tid == 1023u) offsets[bins] = part[1023];
+}
+
+// The count kernel's workgroups again: local counts, a range of each touched bin reserved with one
+// global atomic, then each term (entry within its bin, F values) written at the range's next
+// position (LDS counter).
+template <int P, int L, int F>
+__global__ __launch_bounds__(kBinThreads) void train_bin_scatter_kernel(TrainArgs t, BinMap bm,
+                                                                        const float* __restrict__ in,
+                                                                        const float* __restrict__ act, uint32_t n,
+                                                                        uint32_t* __restrict__ cursors,
+                                                                        uint32_t* __restrict__ term_r,
+                                                                        float* __restrict__ term_v)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t E = kBinFloats / F;
+    const uint32_t k = blockIdx.y, l = bm.level[k], w0 = blockIdx.z * kBinWindow;
+    const uint32_t level_bins = bm.first[k + 1] - bm.first[k];
+    if (w0 >= level_bins) return;
+    const uint32_t first = bm.first[k] + w0, bins = level_bins - w0 < kBinWindow ? level_bins - w0 : kBinWindow;
+    uint32_t* hist = lds;
+    uint32_t* base = lds + bins;
+    const GridLevel& lv = t.net.level[l];
+    for (uint32_t j = threadIdx.x; j < bins; j += kBinThreads) hist[j] = 0u;
+    __syncthreads();
+    const uint32_t total = n * (uint32_t)P, j0 = blockIdx.x * kBinPairs;
+    const uint32_t j1 = total - j0 < kBinPairs ? total : j0 + kBinPairs;
+    for (uint32_t jj = j0 + threadIdx.x; jj < j1; jj += kBinThreads) {
+        const uint32_t i = jj / P, p = jj - i * P;
+        uint32_t idx[8];
+        float fr[3];
+        pair_corners<P>(lv, in, i, p, idx, fr);
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const uint32_t r = idx[c] / E - w0;
+            if (r < bins) atomicAdd(&hist[r], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < bins; j += kBinThreads) {
+        const uint32_t c = hist[j];
+        base[j] = c != 0u ? atomicAdd(&cursors[first + j], c) : 0u;
+        hist[j] = 0u;
+    }
+    __syncthreads();
+    const size_t S = t.act_stride;
+    for (uint32_t jj = j0 + threadIdx.x; jj < j1; jj += kBinThreads) {
+        const uint32_t i = jj / P, p = jj - i * P;
+        uint32_t idx[8];
+        float fr[3];
+        pair_corners<P>(lv, in, i, p, idx, fr);
+        float dx[F];
+#pragma unroll
+        for (int f = 0; f < F; f++) dx[f] = act[(size_t)(kDX + (p * L + l) * F + f) * S + i];
+#pragma unroll
+        for (uint32_t c = 0; c < 8; c++) {
+            const uint32_t sl = idx[c] / E, r = sl - w0;
+            if (r >= bins) continue;
+            const uint32_t pos = base[r] + atomicAdd(&hist[r], 1u);
+            const float w = corner_weight(c, fr);
+            term_r[pos] = idx[c] - sl * E;
+#pragma unroll
+            for (int f = 0; f < F; f++) term_v[(size_t)pos * F + f] = w * dx[f];
+        }
+    }
+}
+
+// One workgroup per bin: the bin's terms added in LDS (kBinUnroll terms per thread loaded before
+// their adds), the bin's entries stored whole (zeros included).
+template <int F>
+__global__ __launch_bounds__(kScatterThreads) void train_bin_accumulate_kernel(TrainArgs t, BinMap bm,
+                                                                               const uint32_t* __restrict__ offsets,
+                                                                               const uint32_t* __restrict__ term_r,
+                                                                               const float* __restrict__ term_v,
+                                                                               float* __restrict__ gtab)
+{
+    constexpr uint32_t E = kBinFloats / F;
+    __shared__ float4 acc4[kBinFloats / 4];
+    float* acc = reinterpret_cast<float*>(acc4);
+    const uint32_t b = blockIdx.x;
+    uint32_t k = 0;
+    while (k + 1 < bm.count && b >= bm.first[k + 1]) k++;
+    const GridLevel& lv = t.net.level[bm.level[k]];
+    const uint32_t e0 = (b - bm.first[k]) * E;
+    const uint32_t len = lv.size - e0 < E ? lv.size - e0 : E;  // a multiple of 8 entries
+    for (uint32_t j = threadIdx.x; j < len * F / 4; j += kScatterThreads) acc4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const uint32_t q0 = offsets[b], q1 = offsets[b + 1];
+    constexpr uint32_t U = kBinUnroll, B = kScatterThreads;
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += U * B) {
+        uint32_t r[U];
+        float v[U][F];
+#pragma unroll
+        for (uint32_t h = 0; h < U; h++) {
+            const uint32_t qq = q + h * B < q1 ? q + h * B : q;
+            r[h] = term_r[qq];
+#pragma unroll
+            for (int f = 0; f < F; f++) v[h][f] = term_v[(size_t)qq * F + f];
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < U; h++) {
+            if (q + h * B >= q1) break;
+#pragma unroll
+            for (int f = 0; f < F; f++) atomicAdd(&acc[r[h] * F + f], v[h][f]);
+        }
+    }
+    __syncthreads();
+    float4* out = reinterpret_cast<float4*>(gtab + ((size_t)lv.offset + e0) * F);
+    for (uint32_t j = threadIdx.x; j < len * F / 4; j += kScatterThreads) out[j] = acc4[j];
 }
 
 // Partial dW[o][k] = sum over samples b in [b0, b1) of delta_b[o] * act_b[k] for one matrix
@@ -500,6 +678,16 @@ hipError_t train_zero(float* p, uint64_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
+// Bin workspace: the bin counts, offsets and cursors, then the corner terms of a max batch
+// (P * L * 8 per sample: entry within its slice, and F values).
+constexpr uint32_t kMaxBins = 131072;  // bins of all binned levels (2^26 entries x 16 levels at 2 features)
+
+size_t train_bin_bytes(uint32_t max_batch, uint32_t n_points, uint32_t n_levels, uint32_t n_features)
+{
+    const size_t terms = (size_t)train_act_stride(max_batch) * n_points * n_levels * 8u;
+    return (size_t)3 * (kMaxBins + 1) * sizeof(uint32_t) + terms * (1 + n_features) * sizeof(float);
+}
+
 // The caller guarantees n <= act_stride (the trainer's max batch).
 hipError_t train_gradients(const TrainArgs& t, const float* in, const float* targets, uint32_t n, hipStream_t stream,
                            bool* supported)
@@ -514,32 +702,59 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
     float* const   part = wt + dev::kMlp;
     const dim3 grid((n + 63) / 64), block(64), grid_mlp((dev::kMlp + 255) / 256);
     if (a.n_levels > (uint32_t)kMaxLevels || a.n_features == 0) return hipErrorInvalidValue;
-    // Parts per slice: about 32K corners per workgroup if the batch's corners spread evenly over
-    // the level (a skewed batch only makes some workgroups slower). A level of more than
-    // kMaxSlices slices (above 2^22 entries at 2 features, 2^21 at 4) scans the batch too often:
-    // its corners go to global atomics instead, and it gets no slice items. Measured per
-    // 65,536-sample step of the 16 x 2 grid, slices / atomics: 2^21 1.93 / 2.89 ms, 2^22 2.82 /
-    // 3.28 ms, 2^23 4.21 / 3.83 ms.
-#ifndef NBVH_MAX_SLICES
-#define NBVH_MAX_SLICES 256
+    // Levels of at least NBVH_BIN_MIN_SLICES slices (16 MB of gradient) take the binned path; the
+    // others are scanned once per slice, split into parts so that each workgroup adds about 32K
+    // corners if the batch's corners spread evenly over the level (a skewed batch only makes some
+    // workgroups slower). Measured per 65,536-sample step of the 16 x 2 grid, binning levels of
+    // >= 16 / 64 / 128 slices and slice scans only (2048 pairs per binning workgroup): 2^19 1.14 /
+    // 1.06 / 1.06 / 1.06 ms, 2^20 1.37 / 1.40 / 1.32 / 1.32, 2^21 1.66 / 1.69 / 1.69 / 1.89, 2^22
+    // 2.06 / 2.09 / 2.09 / 2.74, 2^23 2.60 / 2.62 / 2.62 / 4.03 (global atomics on the gradient,
+    // the earlier form for the largest tables: 2^22 2.82, 2^23 3.83).
+#ifndef NBVH_BIN_MIN_SLICES
+#define NBVH_BIN_MIN_SLICES 128
 #endif
-    constexpr uint32_t kMaxSlices = NBVH_MAX_SLICES;
     dev::SliceMap map{};
-    dev::AtomicLevels atomic{};
+    dev::BinMap bins{};
+    uint32_t max_bins = 0;
     for (uint32_t l = 0; l < a.n_levels; l++) {
         const uint32_t E = dev::kSliceFloats / a.n_features, size = a.level[l].size;
         const uint32_t slices = (size + E - 1) / E;
         const double corners = (double)n * a.n_points * 8.0 * std::min(E, size) / size;
-        if (slices > kMaxSlices) {
+        if (slices >= NBVH_BIN_MIN_SLICES) {
+            const uint32_t EB = dev::kBinFloats / a.n_features, level_bins = (size + EB - 1) / EB;
             map.parts[l]     = 1;
             map.first[l + 1] = map.first[l];
-            atomic.level[atomic.count++] = l;
+            bins.level[bins.count]     = l;
+            bins.first[bins.count + 1] = bins.first[bins.count] + level_bins;
+            bins.count++;
+            max_bins = std::max(max_bins, level_bins);
             continue;
         }
         map.parts[l]     = (uint32_t)std::min(64.0, std::max(1.0, std::ceil(corners / 32768.0)));
         map.first[l + 1] = map.first[l] + slices * map.parts[l];
     }
-    const uint64_t atomic_threads = (uint64_t)n * a.n_points * atomic.count;
+    const uint32_t n_bins = bins.first[bins.count];
+    if (n_bins > kMaxBins) return hipErrorInvalidValue;
+    const uint32_t windows = (max_bins + dev::kBinWindow - 1) / dev::kBinWindow;
+    const uint32_t window_bins = std::min(max_bins, dev::kBinWindow);
+    uint32_t* const counts  = reinterpret_cast<uint32_t*>(t.bins);
+    uint32_t* const offsets = counts + (kMaxBins + 1);
+    uint32_t* const cursors = offsets + (kMaxBins + 1);
+    uint32_t* const term_r  = cursors + (kMaxBins + 1);
+    float* const    term_v  = reinterpret_cast<float*>(term_r + (size_t)S * a.n_points * a.n_levels * 8u);
+    const dim3 bin_grid((n * a.n_points + dev::kBinPairs - 1) / dev::kBinPairs, bins.count, windows);
+    // Split slices add into the gradient with global atomics: their levels start from zero. Every
+    // other entry is stored by its slice.
+    for (uint32_t l = 0; l < a.n_levels; l++)
+        if (map.parts[l] > 1) {
+            const hipError_t e = train_zero(t.grad_tables + (size_t)a.level[l].offset * a.n_features,
+                                            (uint64_t)a.level[l].size * a.n_features, stream);
+            if (e != hipSuccess) return e;
+        }
+    if (n_bins > 0) {
+        const hipError_t e = train_zero(reinterpret_cast<float*>(counts), n_bins, stream);
+        if (e != hipSuccess) return e;
+    }
     bool shape = false;
 #define X(P_, L_, F_)                                                                                                \
     if (a.n_points == P_ && a.n_levels == L_ && a.n_features == F_) {                                                \
@@ -551,9 +766,17 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
         if (map.first[L_] > 0)                                                                                       \
             hipLaunchKernelGGL((dev::train_scatter_kernel<P_, L_, F_>), dim3(map.first[L_]), dim3(dev::kScatterThreads), \
                                0, stream, t, map, in, (const float*)act, t.grad_tables, n);                          \
-        if (atomic_threads > 0)                                                                                      \
-            hipLaunchKernelGGL((dev::train_scatter_atomic_kernel<P_, L_, F_>), dim3((unsigned)((atomic_threads + 255) / 256)), \
-                               dim3(256), 0, stream, t, atomic, in, (const float*)act, t.grad_tables, n);            \
+        if (n_bins > 0) {                                                                                            \
+            hipLaunchKernelGGL((dev::train_bin_count_kernel<P_, F_>), bin_grid, dim3(dev::kBinThreads),                \
+                               window_bins * 4, stream, t, bins, in, n, counts);                                     \
+            hipLaunchKernelGGL(dev::train_bin_scan_kernel, dim3(1), dim3(1024), 0, stream, (const uint32_t*)counts,  \
+                               n_bins, offsets, cursors);                                                           \
+            hipLaunchKernelGGL((dev::train_bin_scatter_kernel<P_, L_, F_>), bin_grid, dim3(dev::kBinThreads),          \
+                               window_bins * 8, stream, t, bins, in, (const float*)act, n, cursors, term_r, term_v);  \
+            hipLaunchKernelGGL((dev::train_bin_accumulate_kernel<F_>), dim3(n_bins), dim3(dev::kScatterThreads), 0,   \
+                               stream, t, bins, (const uint32_t*)offsets, (const uint32_t*)term_r,                  \
+                               (const float*)term_v, t.grad_tables);                                                \
+        }                                                                                                            \
     }
     X(3, 16, 2) X(3, 8, 4)
 #undef X

@@ -121,6 +121,7 @@ struct TrainArgs {
     float*       grad_mlp;
     float*       loss;
     float*       sample_loss;  // optional: per-sample sum of the 8 element loss values
+    void*        bins;         // workspace of the binned table gradient (train_bin_bytes)
 };
 
 // Tree-cut learning data of one N-BVH node (NeuralTreeCutData,
