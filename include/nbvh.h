@@ -535,7 +535,9 @@ typedef struct nbvh_train_config {
     float    l2_reg;        /* 1e-8 */
     uint32_t max_batch;     /* samples per step, at most 2^24 (device scratch per sample: 720
                                floats, plus n_points * n_levels * 8 corner terms of 1 + F floats,
-                               4.5 KB at 3 x 16 x 2; plus about 20 MB per trainer) */
+                               4.5 KB at 3 x 16 x 2; plus about 20 MB per trainer); and
+                               max_batch * n_points * n_levels * 8 below 2^32 (11.1 M samples at
+                               3 x 16 levels) */
     uint32_t decay_start;   /* 10000 in the reference; 0 with decay_interval 0: no decay */
     uint32_t decay_interval;/* 2000 in the reference; 0: no decay */
     float    decay_base;    /* 0.33 in the reference */

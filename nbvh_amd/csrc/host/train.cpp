@@ -81,6 +81,10 @@ NBVH_API int32_t nbvh_trainer_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbv
         if (!scene->has_net) fail(NBVH_ERR_STATE, "training a scene without network weights");
         if (!trainable(scene->cfg)) fail(NBVH_ERR_UNSUPPORTED, "no training kernels for this network shape");
         require(cfg->max_batch >= 1 && cfg->max_batch <= (1u << 24), "max_batch must be 1..2^24");
+        // the table gradient's corner terms of a step are indexed with 32 bits
+        require((uint64_t)train_act_stride(cfg->max_batch) * scene->cfg.n_points * scene->cfg.grid.n_levels * 8u <
+                    (1ull << 32),
+                "max_batch too large for this network: max_batch * n_points * n_levels * 8 must be below 2^32");
         require(cfg->decay_interval == 0 || (std::isfinite(cfg->decay_base) && cfg->decay_base > 0.f),
                 "decay_base must be finite and > 0");
         require(cfg->learning_rate > 0.f && cfg->beta1 >= 0.f && cfg->beta1 < 1.f && cfg->beta2 >= 0.f &&

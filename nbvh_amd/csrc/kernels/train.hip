@@ -735,6 +735,8 @@ hipError_t train_gradients(const TrainArgs& t, const float* in, const float* tar
     }
     const uint32_t n_bins = bins.first[bins.count];
     if (n_bins > kMaxBins) return hipErrorInvalidValue;
+    // Term positions are 32-bit: a step bins fewer than 2^32 corner terms.
+    if ((uint64_t)n * a.n_points * 8u * bins.count > 0xFFFFFFFFull) return hipErrorInvalidValue;
     const uint32_t windows = (max_bins + dev::kBinWindow - 1) / dev::kBinWindow;
     const uint32_t window_bins = std::min(max_bins, dev::kBinWindow);
     uint32_t* const counts  = reinterpret_cast<uint32_t*>(t.bins);

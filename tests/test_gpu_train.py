@@ -235,3 +235,12 @@ def test_trainer_refuses_a_replaced_network(nb, orc, ctx, setup):
     tr2.commit()
     tr2.close()
     scene.close()
+
+
+def test_trainer_refuses_batches_beyond_32bit_term_indices(nb, orc, ctx, setup):
+    """The table gradient indexes a step's corner terms with 32 bits: a max batch whose
+    n_points * n_levels * 8 terms per sample reach 2^32 (2^24 samples at 3 points x 16 levels:
+    6.4e9 terms) is refused at creation with NBVH_ERR_INVALID_ARGUMENT, before any allocation."""
+    with pytest.raises(nb.NbvhError) as e:
+        nb.Trainer(ctx, setup["scene"], max_batch=1 << 24)
+    assert e.value.status == nb._ffi.ERR_INVALID_ARGUMENT
