@@ -114,7 +114,12 @@ typedef struct nbvh_grid_config {
 
 enum nbvh_segment_distribution {
     NBVH_SEGMENT_EQUIDISTANT_INTERIOR = 0, /* t_k = (k + 0.5) / P */
-    NBVH_SEGMENT_EQUIDISTANT_EXTERIOR = 1  /* t_k = k / (P - 1) (every shipped config) */
+    NBVH_SEGMENT_EQUIDISTANT_EXTERIOR = 1, /* t_k = k / (P - 1) (every shipped config) */
+    /* t_k drawn for every evaluated segment from the ray's LCG stream (lcg_randomf), the stream
+     * seeded as get_lcg_rng(segment_seed, 0, i) for ray i of its array or part (Uniform,
+     * src/cuda/neural/neural_bvh_module_kernels.cu:53-67). Queries only: path tracing, wavefront
+     * rounds and the learner refuse it (NBVH_ERR_UNSUPPORTED). */
+    NBVH_SEGMENT_UNIFORM = 2
 };
 
 /* Network: segment -> P points -> hash grid -> MLP (no biases, ReLU hidden, Sigmoid output).
@@ -172,7 +177,8 @@ typedef struct nbvh_query_params {
                                      traced in 8x8 pixel tiles. 0 => array order. Results do not
                                      depend on it. */
     uint32_t flags;               /* nbvh_query_flags */
-    uint32_t reserved;
+    uint32_t segment_seed;        /* seed of the rays' LCG streams under NBVH_SEGMENT_UNIFORM
+                                     (ignored otherwise) */
 } nbvh_query_params;
 
 enum nbvh_query_flags {

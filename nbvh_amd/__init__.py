@@ -14,7 +14,7 @@ from ._ffi import (HIT_DTYPE, NODE_DTYPE, RAY_DTYPE, NbvhError, NetworkConfig, G
                    PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8, QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY,
                    QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY, HIT_NONE, HIT_NEURAL, HIT_TRIANGLE,
                    FLAG_STACK_OVERFLOW, FLAG_STEP_CAP, NO_NEURAL_CHILD, SEGMENT_EQUIDISTANT_EXTERIOR,
-                   SEGMENT_EQUIDISTANT_INTERIOR, BACKGROUND_CONSTANT, BACKGROUND_SKY, TRAIN_TABLES, TRAIN_MLP,
+                   SEGMENT_EQUIDISTANT_INTERIOR, SEGMENT_UNIFORM, BACKGROUND_CONSTANT, BACKGROUND_SKY, TRAIN_TABLES, TRAIN_MLP,
                    TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP, DEPTH_LOCAL_T, DEPTH_WORLD_T)
 from ._ffi import check as _check, lib as _lib, ptr as _ptr
 
@@ -341,21 +341,24 @@ class NeuralBVHModule:
 
     @staticmethod
     def params(kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0, max_traversal_steps=0,
-               visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False, array_order=False):
+               visibility_threshold=0.0, image_width=0, sort_rays=False, wavefront=False, array_order=False,
+               segment_seed=0):
         """nbvh_query_params. sort_rays: trace in the coherence order; array_order: in array order
-        even where the scene's launch shape (ray_order, set by autotune) would sort."""
+        even where the scene's launch shape (ray_order, set by autotune) would sort; segment_seed:
+        the rays' LCG streams under the uniform segment distribution."""
         flags = ((_ffi.QUERY_SORT_RAYS if sort_rays else 0) | (_ffi.QUERY_WAVEFRONT if wavefront else 0) |
                  (_ffi.QUERY_ARRAY_ORDER if array_order else 0))
-        return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, flags)
+        return QueryParams(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, flags,
+                           segment_seed)
 
     def neural_bvh_traversal(self, rays, kind=QUERY_NEURAL_CLOSEST, precision=PREC_FP32, lod=0,
                              max_traversal_steps=0, visibility_threshold=0.0, image_width=0, sort_rays=False,
-                             wavefront=False, array_order=False):
+                             wavefront=False, array_order=False, segment_seed=0):
         """Host rays (RAY_DTYPE) -> host hits (HIT_DTYPE); synchronous."""
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)
         p = self.params(kind, precision, lod, max_traversal_steps, visibility_threshold, image_width, sort_rays,
-                        wavefront, array_order)
+                        wavefront, array_order, segment_seed)
         self.ctx.check(_lib().nbvh_query_host(self.ctx.handle, self.scene.handle, ctypes.byref(p), _ptr(rays),
                                               len(rays), _ptr(hits)))
         return hits

@@ -35,7 +35,7 @@ TRAIN_TABLES, TRAIN_MLP, TRAIN_GRAD_TABLES, TRAIN_GRAD_MLP = 0, 1, 2, 3
 QUERY_NEURAL_CLOSEST, QUERY_NEURAL_ANY, QUERY_TRIANGLE_CLOSEST, QUERY_TRIANGLE_ANY = 0, 1, 2, 3
 HIT_NONE, HIT_NEURAL, HIT_TRIANGLE = 0, 1, 2
 FLAG_STACK_OVERFLOW, FLAG_STEP_CAP = 1, 2
-SEGMENT_EQUIDISTANT_INTERIOR, SEGMENT_EQUIDISTANT_EXTERIOR = 0, 1
+SEGMENT_EQUIDISTANT_INTERIOR, SEGMENT_EQUIDISTANT_EXTERIOR, SEGMENT_UNIFORM = 0, 1, 2
 DEPTH_LOCAL_T, DEPTH_WORLD_T = 0, 1
 NO_NEURAL_CHILD = 0xFFFFFFFF
 
@@ -65,7 +65,7 @@ class NetworkConfig(ctypes.Structure):
 class QueryParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("precision", ctypes.c_uint32), ("lod", ctypes.c_uint32),
                 ("max_traversal_steps", ctypes.c_uint32), ("visibility_threshold", ctypes.c_float),
-                ("image_width", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("image_width", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("segment_seed", ctypes.c_uint32)]
 
 
 class Camera(ctypes.Structure):

@@ -218,9 +218,40 @@ __device__ __forceinline__ uint32_t morton30(float x, float y, float z)
     return m;
 }
 
+// LCG of get_lcg_rng / lcg_randomf (include/cuda/utils/lcg_rng.cuh:12-72).
+__device__ __forceinline__ uint32_t murmur_mix(uint32_t hash, uint32_t k)
+{
+    k *= 0xcc9e2d51u;
+    k = (k << 15) | (k >> 17);
+    k *= 0x1b873593u;
+    hash ^= k;
+    return ((hash << 13) | (hash >> 19)) * 5u + 0xe6546b64u;
+}
+__device__ __forceinline__ uint32_t murmur_fmix(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+// LCG state of get_lcg_rng(index = sample, frame, linear).
+__device__ __forceinline__ uint32_t lcg_seed(uint32_t sample, uint32_t frame, uint32_t linear)
+{
+    return murmur_fmix(murmur_mix(murmur_mix(0u, linear), 512u * frame + sample));
+}
+__device__ __forceinline__ float lcg_float(uint32_t& state)
+{
+    state = state * 1664525u + 1013904223u;
+    return ldexpf((float)state, -32);
+}
+
 // Segment -> P encoded points in the neural scene box
-// (src/cuda/neural/neural_bvh_module_kernels.cu:10-97,266-303; EquidistantExterior/Interior).
-__device__ __forceinline__ void encode_segment(const NetArgs& a, const Ray& r, float mint, float maxt, float* pts)
+// (src/cuda/neural/neural_bvh_module_kernels.cu:10-97,266-303; EquidistantExterior/Interior, and
+// Uniform with t_k = lcg_randomf of the ray's stream `rng`, advanced P times).
+__device__ __forceinline__ void encode_segment(const NetArgs& a, const Ray& r, float mint, float maxt, float* pts,
+                                               uint32_t* rng = nullptr)
 {
     if (mint < 0.f) {
         const float t = mint;
@@ -238,7 +269,7 @@ __device__ __forceinline__ void encode_segment(const NetArgs& a, const Ray& r, f
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (k < (int)a.n_points) {
-            const float t = a.seg_t[k];
+            const float t = (rng != nullptr && a.seg_uniform) ? lcg_float(*rng) : a.seg_t[k];
             const float w = 1.f - t;
             pts[3 * k + 0] = w * eox + t * eix;
             pts[3 * k + 1] = w * eoy + t * eiy;

@@ -8,34 +8,7 @@
 namespace nbvh {
 namespace dev {
 
-// get_lcg_rng + lcg_randomf (include/cuda/utils/lcg_rng.cuh:12-72).
-__device__ __forceinline__ uint32_t murmur_mix(uint32_t hash, uint32_t k)
-{
-    k *= 0xcc9e2d51u;
-    k = (k << 15) | (k >> 17);
-    k *= 0x1b873593u;
-    hash ^= k;
-    return ((hash << 13) | (hash >> 19)) * 5u + 0xe6546b64u;
-}
-__device__ __forceinline__ uint32_t murmur_fmix(uint32_t h)
-{
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
-}
-// LCG state of get_lcg_rng(index = sample, frame, linear).
-__device__ __forceinline__ uint32_t lcg_seed(uint32_t sample, uint32_t frame, uint32_t linear)
-{
-    return murmur_fmix(murmur_mix(murmur_mix(0u, linear), 512u * frame + sample));
-}
-__device__ __forceinline__ float lcg_float(uint32_t& state)
-{
-    state = state * 1664525u + 1013904223u;
-    return ldexpf((float)state, -32);
-}
+// murmur_mix, murmur_fmix, lcg_seed and lcg_float (get_lcg_rng, lcg_randomf) are in common.h.
 
 // ortho_basis (include/cuda/utils/common.cuh:21-35).
 __device__ __forceinline__ void ortho_basis(const float n[3], float t[3], float b[3])

@@ -162,6 +162,7 @@ struct Slot {
     Ray ray;
     float seg_mint, seg_maxt;
     uint32_t seg_node;
+    uint32_t rng;  // the ray's LCG stream (uniform segment distribution)
     HitState h;
 };
 
@@ -276,6 +277,7 @@ __device__ __forceinline__ bool fill_slots(const NetArgs& a, const Nodes& nodes,
             s.state    = SLOT_RAY;
             s.ray      = load_ray(batch_rays<PARTS>(b, s.part), s.ray_id);
             s.any_hit  = batch_any_hit<PARTS>(b, s.part, a.any_hit);
+            s.rng      = a.seg_uniform ? lcg_seed(a.seg_seed, 0u, s.ray_id) : 0u;
             hit_state_init(s.h, s.ray.tmax);
             st.n = 0;
             st.push(0u);

@@ -140,6 +140,8 @@ void refresh_args(nbvh_scene* s)
         offset = 0.5f;
     }
     for (uint32_t k = 0; k < 4; k++) a.seg_t[k] = k < P ? delta * ((float)k + offset) : 0.f;
+    a.seg_uniform = cfg.segment_distribution == NBVH_SEGMENT_UNIFORM ? 1u : 0u;
+    a.seg_seed    = 0u;
     const auto lv = grid_levels(cfg.grid);
     for (size_t l = 0; l < lv.size(); l++) a.level[l] = lv[l];
     a.n_dense = 0;
@@ -252,6 +254,9 @@ NetArgs neural_args(const nbvh_scene* s, const nbvh_query_params* p)
     const float thr = p->visibility_threshold == 0.f ? 0.5f : p->visibility_threshold;
     require(thr > 0.f && thr < 1.f, "visibility_threshold must be in (0, 1)");
     a.thr_logit = thr == 0.5f ? 0.f : (float)std::log((double)thr / (1.0 - (double)thr));
+    a.seg_seed  = p->segment_seed;
+    if (a.seg_uniform && (p->flags & NBVH_QUERY_WAVEFRONT) != 0u)
+        fail(NBVH_ERR_UNSUPPORTED, "the uniform segment distribution is not available in wavefront rounds");
     return a;
 }
 
@@ -1323,6 +1328,10 @@ NBVH_API int32_t nbvh_path_trace(nbvh_ctx* ctx, const nbvh_scene* neural, const 
         check_path_params(ctx, neural, environment, p, width, height);
         require(cam != nullptr, "NULL argument");
         const bool tri_object = p->query.kind == NBVH_QUERY_TRIANGLE_CLOSEST;
+        // A path's rays move between array positions from depth to depth, and a uniform segment
+        // distribution seeds each ray's stream by its position: not reproducible in a frame.
+        if (!tri_object && neural->has_net && neural->cfg.segment_distribution == NBVH_SEGMENT_UNIFORM)
+            fail(NBVH_ERR_UNSUPPORTED, "path tracing a network with the uniform segment distribution");
         if (stats) *stats = nbvh_path_stats{};
         const uint32_t n = width * height;
         if (n == 0) return;

@@ -397,6 +397,8 @@ NBVH_API int32_t nbvh_learner_create(nbvh_ctx* ctx, nbvh_scene* scene, const nbv
         check_config(*cfg);
         if (scene->bvh.empty()) fail(NBVH_ERR_STATE, "tree-cut learning needs the scene's triangles");
         if (!scene->has_net) fail(NBVH_ERR_STATE, "tree-cut learning needs the scene's network (its encoding)");
+        if (scene->cfg.segment_distribution == NBVH_SEGMENT_UNIFORM)
+            fail(NBVH_ERR_UNSUPPORTED, "tree-cut learning with the uniform segment distribution");
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         auto l = std::make_unique<nbvh_learner>();
         l->ctx   = ctx;

@@ -27,7 +27,8 @@ void network_validate(const nbvh_network_config& c)
     if (g.base_resolution < 1 || g.base_resolution > 4096) fail(NBVH_ERR_UNSUPPORTED, "grid: base_resolution must be 1..4096");
     if (!(g.per_level_scale >= 1.f && g.per_level_scale <= 4.f)) fail(NBVH_ERR_UNSUPPORTED, "grid: per_level_scale must be in [1, 4]");
     if (c.n_points < 1 || c.n_points > 4) fail(NBVH_ERR_UNSUPPORTED, "network: n_points must be 1..4");
-    if (c.segment_distribution > 1) fail(NBVH_ERR_UNSUPPORTED, "network: segment_distribution must be 0 or 1");
+    if (c.segment_distribution > NBVH_SEGMENT_UNIFORM)
+        fail(NBVH_ERR_UNSUPPORTED, "network: segment_distribution must be 0, 1 or 2");
     if (c.n_neurons != 64) fail(NBVH_ERR_UNSUPPORTED, "network: n_neurons must be 64");
     if (c.n_hidden_layers < 1 || c.n_hidden_layers > 8) fail(NBVH_ERR_UNSUPPORTED, "network: n_hidden_layers must be 1..8");
     if (c.n_outputs != 8) fail(NBVH_ERR_UNSUPPORTED, "network: n_outputs must be 8 (vis, depth, normal3, albedo3)");

@@ -137,7 +137,8 @@ bool same_key(const Request& a, const Request& b)
     return a.scene == b.scene && is_triangle(a.params.kind) == is_triangle(b.params.kind) &&
            a.params.precision == b.params.precision && a.params.lod == b.params.lod &&
            a.params.max_traversal_steps == b.params.max_traversal_steps &&
-           a.params.visibility_threshold == b.params.visibility_threshold;
+           a.params.visibility_threshold == b.params.visibility_threshold &&
+           a.params.segment_seed == b.params.segment_seed;
 }
 
 }  // namespace

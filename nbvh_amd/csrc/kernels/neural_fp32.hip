@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64, NBVH_FP32_WAVES_PER_EU) void neural_query_fp32_
         float x[K0];
         if (s.state == SLOT_SEG) {
             float pts[3 * P];
-            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, pts);
+            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, pts, &s.rng);
             grid_features<P, L, F, float>(a, a.tables_f32, pts, x);
         } else {
 #pragma unroll

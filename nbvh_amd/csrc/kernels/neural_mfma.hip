@@ -541,7 +541,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
-            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
+            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own, &s.rng);
         } else {
 #pragma unroll
             for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, NBVH_MFMA_WAVES_PER_EU) void n
     while (fill_slots<PARTS>(a, nodes, s, st, b, q)) {
         float own[3 * P];
         if (s.state == SLOT_SEG) {
-            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own);
+            encode_segment(a, s.ray, s.seg_mint, s.seg_maxt, own, &s.rng);
         } else {
 #pragma unroll
             for (int c = 0; c < 3 * P; c++) own[c] = 0.5f;

@@ -33,6 +33,7 @@ struct NetArgs {
     uint32_t n_levels, n_features, n_points, n_inputs;  // n_inputs = P * L * F
     uint32_t n_neurons, n_hidden_layers, n_outputs, lod;
     float    seg_t[4];                 // segment point parameters t_k
+    uint32_t seg_uniform, seg_seed;    // uniform distribution: t_k from the ray's LCG stream, its seed
     float    enc_center[3];            // neural scene bbox centre, (max + min) * 0.5
     float    enc_extent[3];            // bbox extent * (1 + node_extent_dilation + fixed_node_dilation)
     float    thr_logit;                // visibility threshold on the logit (0 for 0.5)

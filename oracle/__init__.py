@@ -32,7 +32,8 @@ class OracleNet(ctypes.Structure):
                 ("tables", ctypes.c_void_p), ("mlp", ctypes.c_void_p),
                 ("bbox_min", ctypes.c_float * 3), ("bbox_max", ctypes.c_float * 3),
                 ("depth_mode", ctypes.c_uint32), ("depth_linear", ctypes.c_uint32),
-                ("interpolate_depth", ctypes.c_uint32), ("interp_eps", ctypes.c_float)]
+                ("interpolate_depth", ctypes.c_uint32), ("interp_eps", ctypes.c_float),
+                ("segment_seed", ctypes.c_uint32)]
 
 
 def build():
@@ -111,7 +112,10 @@ class Net:
         total = lib().oracle_grid_levels(ctypes.byref(self.s), _p(scale), _p(res), _p(size), _p(off), _p(dense))
         return dict(scale=scale, res=res, size=size, offset=off, dense=dense, n_params=total)
 
-    def encode(self, rays, mint, maxt, mode=FP32):
+    def encode(self, rays, mint, maxt, mode=FP32, segment_seed=0):
+        """Encoded points of segments (rays[i], mint[i], maxt[i]); under the uniform distribution
+        segment i takes the first n_points draws of stream lcg_seed(segment_seed, 0, i)."""
+        self.s.segment_seed = segment_seed
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         mint = np.ascontiguousarray(mint, np.float32)
         maxt = np.ascontiguousarray(maxt, np.float32)
@@ -135,7 +139,8 @@ class Net:
         return outs, logits
 
     def neural_query(self, nodes, rays, any_hit=False, mode=FP32, lod=0, max_steps=1000, threshold=0.5,
-                     log_segments=False):
+                     log_segments=False, segment_seed=0):
+        self.s.segment_seed = segment_seed
         nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         hits = np.zeros(len(rays), HIT_DTYPE)

@@ -298,6 +298,9 @@ typedef struct oracle_net {
     // (0 LocalT, 1 WorldT), linear depth activation, depth interpolation and its epsilon.
     uint32_t     depth_mode, depth_linear, interpolate_depth;
     float        interp_eps;
+    // Uniform segment distribution (distribution 2): ray i's LCG stream is lcg_seed(segment_seed,
+    // 0, i) (neural_bvh_module_kernels.cu:53-67, the query's stream seeding).
+    uint32_t     segment_seed;
 } oracle_net;
 }
 
@@ -357,7 +360,7 @@ template <> inline double fma_r<double>(double a, double b, double c) { return a
 
 // Segment encoding (neural_bvh_module_kernels.cu:10-97,266-303). mode: float or double.
 template <typename R>
-void encode_segment(const oracle_net &net, const Ray &ray, float mint, float maxt, R *pts)
+void encode_segment(const oracle_net &net, const Ray &ray, float mint, float maxt, R *pts, uint32_t *rng)
 {
     if (mint < 0.f) std::swap(mint, maxt);
     R eo[3], ei[3];
@@ -380,7 +383,8 @@ void encode_segment(const oracle_net &net, const Ray &ray, float mint, float max
         offset = 0.5f;
     }
     for (uint32_t k = 0; k < P; k++) {
-        const float t = delta * ((float)k + offset);
+        // Uniform: t = lcg_randomf(rng) per point (neural_bvh_module_kernels.cu:53-67)
+        const float t = net.distribution == 2 ? lcg_float(*rng) : delta * ((float)k + offset);
         for (int c = 0; c < 3; c++) pts[3 * k + c] = ((R)1 - (R)t) * eo[c] + (R)t * ei[c];
     }
 }
@@ -561,13 +565,14 @@ void neural_ray(const oracle_net &net, const std::vector<Level> &lv, const Node 
     uint32_t node;
     int      st = nbvh_intersect(nodes, lod, ray, stack, ss, ray.tmax, mint, maxt, node);
     const uint32_t P = net.n_points;
+    uint32_t rng = lcg_seed(net.segment_seed, 0u, ray_idx);
     while (st == 1) {
         if (segments == max_steps + 1) {
             flags |= 2u;
             break;
         }
         R pts[12], z[16];
-        encode_segment<R>(net, ray, mint, maxt, pts);
+        encode_segment<R>(net, ray, mint, maxt, pts, &rng);
         network_eval<R>(net, lv, pts, z);
         (void)P;
         segments++;
@@ -1064,17 +1069,19 @@ ORACLE_API uint64_t oracle_grid_levels(const oracle_net *net, float *scale, uint
     return total * net->n_features;
 }
 
-// Encoded segment points for n (ray, mint, maxt): out float[n * 3P] (mode 0) / double (mode 1).
+// Encoded segment points for n (ray, mint, maxt): out float[n * 3P] (mode 0) / double (mode 1);
+// under the uniform distribution segment i takes the first P draws of stream i.
 ORACLE_API void oracle_encode(const oracle_net *net, const void *rays, const float *mint, const float *maxt, uint32_t n,
                               uint32_t mode, void *out)
 {
     const Ray *R = (const Ray *)rays;
     const uint32_t D = 3 * net->n_points;
     for (uint32_t i = 0; i < n; i++) {
+        uint32_t rng = lcg_seed(net->segment_seed, 0u, i);  // uniform: segment i draws from stream i
         if (mode == 0)
-            encode_segment<float>(*net, R[i], mint[i], maxt[i], (float *)out + (size_t)i * D);
+            encode_segment<float>(*net, R[i], mint[i], maxt[i], (float *)out + (size_t)i * D, &rng);
         else
-            encode_segment<double>(*net, R[i], mint[i], maxt[i], (double *)out + (size_t)i * D);
+            encode_segment<double>(*net, R[i], mint[i], maxt[i], (double *)out + (size_t)i * D, &rng);
     }
 }
 

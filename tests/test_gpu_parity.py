@@ -161,6 +161,56 @@ def test_decode_options_bit_exact(nb, orc, ctx, bench, world_t, linear, interp):
     scene.close()
 
 
+def test_uniform_segment_distribution(nb, orc, ctx, bench):
+    """The uniform segment distribution (random segment points at inference,
+    neural_bvh_module_kernels.cu:53-67): FP32 closest and any hit bit-exact with the oracle for two
+    seeds (each ray's stream seeded by its index, n_points draws per evaluated segment), other
+    seeds other hits, FP16 hit/miss within 0.5 % of the FP64 oracle; a multi-part launch seeds by
+    the index within each part (a part's hits equal its own query); wavefront rounds, path tracing
+    and the learner refuse it."""
+    from nbvh_amd import hip
+    cfg = nb.network_config(segment_distribution=nb.SEGMENT_UNIFORM)
+    scene = nb.Scene(ctx, bench["v"], bench["t"])
+    scene.build_nbvh(256)
+    scene.init_network(cfg, seed=5, table_range=1e-1)
+    cfg2, tables, mlp = scene.network()
+    assert cfg2.segment_distribution == nb.SEGMENT_UNIFORM
+    bmin, bmax = scene.bbox()
+    net = orc.Net(cfg2, tables, mlp, bmin, bmax)
+    mod = nb.NeuralBVHModule(ctx, scene)
+    _, rays = camera_rays(nb, orc, 96, 96)
+    rays = np.concatenate([rays, random_rays(5000, 21, inside_frac=0.4)])
+    by_seed = {}
+    for seed in (0, 4242):
+        for any_hit in (False, True):
+            kind = nb.QUERY_NEURAL_ANY if any_hit else nb.QUERY_NEURAL_CLOSEST
+            got = mod.neural_bvh_traversal(rays, kind=kind, precision=nb.PREC_FP32, segment_seed=seed)
+            exp = net.neural_query(scene.nbvh(), rays, any_hit=any_hit, mode=orc.FP32, segment_seed=seed)
+            assert 0.1 < (nb.hit_kind(exp) == nb.HIT_NEURAL).mean() < 0.95
+            assert hits_equal(got, exp), (seed, any_hit)
+            by_seed[(seed, any_hit)] = got
+        g16 = mod.neural_bvh_traversal(rays, precision=nb.PREC_FP16, segment_seed=seed)
+        e64 = net.neural_query(scene.nbvh(), rays, mode=orc.FP64, segment_seed=seed)
+        assert (nb.hit_kind(g16) != nb.hit_kind(e64)).mean() <= 0.005
+    assert not np.array_equal(by_seed[(0, False)]["t"], by_seed[(4242, False)]["t"])
+    # two parts in one launch: each part's streams are seeded by the index within the part
+    halves = [rays[:4000], rays[4000:]]
+    bufs = [(hip.Buffer.from_array(r), hip.Buffer(len(r) * 48)) for r in halves]
+    mod.query_parts_device(mod.params(precision=nb.PREC_FP32, segment_seed=9),
+                           [(b.ptr, len(r), h.ptr, False) for (b, h), r in zip(bufs, halves)])
+    hip.synchronize()
+    for (b, h), r in zip(bufs, halves):
+        assert hits_equal(h.download(nb.HIT_DTYPE)[:len(r)],
+                          mod.neural_bvh_traversal(r, precision=nb.PREC_FP32, segment_seed=9))
+    for call in (lambda: mod.neural_bvh_traversal(rays[:100], wavefront=True),
+                 lambda: mod.path_trace(nb.camera_look_at((0, 0, 3), (0, 0, 0), (0, 1, 0), 45.0, 16, 16), 16, 16),
+                 lambda: nb.Learner(ctx, scene)):
+        with pytest.raises(nb.NbvhError) as e:
+            call()
+        assert e.value.status == nb._ffi.ERR_UNSUPPORTED
+    scene.close()
+
+
 def test_neural_query_step_cap_bit_exact(nb, orc, bench):
     rays = random_rays(4000, 8, inside_frac=0.5)
     got = bench["module"].neural_bvh_traversal(rays, max_traversal_steps=1)
