@@ -164,7 +164,9 @@ def settings(config):
     dist = {"EquidistantInterior": 0, "EquidistantExterior": 1}.get(bm.get("segment_distribution",
                                                                             "EquidistantExterior"))
     if dist is None:
-        raise ValueError(f"segment_distribution {bm.get('segment_distribution')} is not supported")
+        # Uniform is a query option here (nbvh_query with segment_seed); the harness's learner and
+        # path tracer refuse it.
+        raise ValueError(f"segment_distribution {bm.get('segment_distribution')} is not supported by the harness")
     depth_mode = {"LocalT": 0, "WorldT": 1}.get(pt.get("depth_learning_mode", "LocalT"))
     if depth_mode is None:
         raise ValueError(f"depth_learning_mode {pt.get('depth_learning_mode')} is not supported")
