@@ -182,10 +182,12 @@ struct QueryOptions {
     float visibility_threshold   = 0.f;   // 0: 0.5
     uint32_t image_width         = 0;     // scheduling hint for image-ordered rays
     uint32_t flags               = 0;     // nbvh_query_flags (NBVH_QUERY_SORT_RAYS)
+    uint32_t segment_seed        = 0;     // rays' LCG streams under NBVH_SEGMENT_UNIFORM
     nbvh_query_params params() const
     {
         nbvh_query_params p{};
         p.flags                = flags;
+        p.segment_seed         = segment_seed;
         p.kind                 = kind;
         p.precision            = precision;
         p.lod                  = lod;
