@@ -167,3 +167,27 @@ def test_timeout_unknown_ticket_and_failed_request(nb, ctx, scenes):
         hits, info = srv.wait(srv.submit(a, p, random_rays(100, 72)))
         assert info["status"] == 0
         empty.close()
+
+
+def test_uniform_segments_alone_and_in_the_mix(nb, orc, ctx):
+    """A network with the uniform segment distribution: each ray's stream is seeded by its index in
+    its own request, so a request's hits are the same alone, batched with other requests of the
+    same seed, and traced by a direct query; requests of other seeds are not batched with it."""
+    v, t, *_ = bench_scene(nb)
+    s = nb.Scene(ctx, v, t)
+    s.build_nbvh(256)
+    s.init_network(nb.network_config(segment_distribution=nb.SEGMENT_UNIFORM), seed=7, table_range=1e-1)
+    m = nb.NeuralBVHModule(ctx, s)
+    reqs = [random_rays(n, 40 + i, inside_frac=0.3) for i, n in enumerate((3000, 1200, 2500, 800))]
+    seeds = [11, 11, 11, 12]
+    exp = [m.neural_bvh_traversal(r, precision=nb.PREC_FP32, segment_seed=sd) for r, sd in zip(reqs, seeds)]
+    with nb.Server(ctx) as srv:
+        alone = [srv.wait(srv.submit(s, nb.NeuralBVHModule.params(precision=nb.PREC_FP32, segment_seed=sd), r))
+                 for r, sd in zip(reqs, seeds)]
+        tickets = [srv.submit(s, nb.NeuralBVHModule.params(precision=nb.PREC_FP32, segment_seed=sd), r)
+                   for r, sd in zip(reqs, seeds)]
+        mixed = [srv.wait(tk) for tk in tickets]
+    for (ha, ia), (hm, im), e in zip(alone, mixed, exp):
+        assert ia["status"] == 0 and im["status"] == 0
+        assert hits_equal(ha, e) and hits_equal(hm, e)
+    s.close()
