@@ -901,8 +901,8 @@ def main():
                     help="also run a device-to-device copy of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the neural query is not sharded across GPUs in this build "
-              "(rays are independent; see DESIGN.md for how the batch would shard). Run with --gpus 1.",
+        print("bench.py measures one GPU: the neural query is not sharded across GPUs here (rays are "
+              "independent, so a batch would be split evenly, one share per GPU). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
