@@ -299,6 +299,38 @@ def train_bench(nb, hip, ctx, oracle, batch=1 << 16, steps=100, pool=1 << 20):
     return out
 
 
+def mlp_mfma_bench(nb, hip, ctx, n=1 << 21, steps=10):
+    """The network kernel (nbvh_network_inference: hash-grid gathers + the 96-64x4-8 MLP) on n
+    encoded segments with tables of 2^14 entries per level (cache-resident, 16 levels x 2
+    features), so that the MLP rather than the gathers sets the time: MLP TFLOP/s (37,888 FLOP per
+    segment) against MI355X's dense peaks (FP16/BF16 2.5 PFLOP/s on MFMA, FP32 157.3 TFLOP/s)."""
+    v, t = nb.icosphere(3, seed=7, amplitude=0.1)
+    sc = nb.Scene(ctx, v, t)
+    sc.build_nbvh(32)
+    sc.init_network(nb.network_config(log2_hashmap_size=14), seed=1, table_range=1e-2)
+    mod = nb.NeuralBVHModule(ctx, sc)
+    x = np.random.default_rng(3).uniform(0.05, 0.95, (n, 9)).astype(np.float32)
+    d_in, d_out = hip.Buffer.from_array(x), hip.Buffer(n * 8 * 4)
+    s = hip.Stream()
+    out = {"workload": f"{n} segments, hash grid 16 x 2 at 2^14 entries per level (cache-resident), MLP 96-64x4-8"}
+    for name, prec, peak in (("fp16", nb.PREC_FP16, 2500.0), ("bf16", nb.PREC_BF16, 2500.0), ("fp32", nb.PREC_FP32, 157.3)):
+        mod.network_inference_device(prec, d_in.ptr, n, d_out.ptr, stream=s.handle)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(s)
+        for _ in range(steps):
+            mod.network_inference_device(prec, d_in.ptr, n, d_out.ptr, stream=s.handle)
+        e1.record(s)
+        s.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        tf = n * FLOP_PER_SEGMENT / (ms * 1e-3) / 1e12
+        out[name] = {"ms": round(ms, 4), "mlp_tflops": round(tf, 1), "frac_of_dense_peak": round(tf / peak, 4)}
+    s.destroy()
+    sc.close()
+    log("mlp on MFMA (cache-resident tables): " + ", ".join(
+        f"{k} {out[k]['mlp_tflops']} TFLOP/s ({out[k]['frac_of_dense_peak']:.1%} of peak)" for k in ("fp16", "bf16", "fp32")))
+    return out
+
+
 def atomic_ceiling(entries=6529022, waves_per_cu=8, per_lane=192):
     """Rate of random FP32 atomic adds (pairs on the two floats of one entry, as the table-gradient
     scatter issues them) into a table of the headline grid's size (nbvh_amd/libnbvh_bench.so,
@@ -1019,6 +1051,7 @@ def main():
         if prec != nb.PREC_FP32:
             result["chess_nbvh"] = chess_bench(nb, hip, ctx, oracle)
         if not args.no_gather_ceiling:
+            result["mlp_mfma"] = mlp_mfma_bench(nb, hip, ctx)
             result["gather_ceiling"] = gather_ceiling()
             # The query's memory-side lines (its profiled FETCH_SIZE + WRITE_SIZE traffic / 128 B;
             # the hit writes are ~1 % of it) against the random-line rate measured for a table of
