@@ -292,6 +292,32 @@ def test_neural_query_vs_fp64(nb, orc, bench, precision):
     assert np.percentile(rel, 99) < {"fp16": 1e-2, "bf16": 3e-2, "fp8": 0.1}[precision], np.percentile(rel, 99)
 
 
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp8"])
+def test_bounce_rays_vs_fp64(nb, orc, bench, precision):
+    """Diffuse bounce rays (the path tracer's secondary rays: origins on the surface, inside a
+    leaf box, cosine-distributed directions) against the FP64 oracle, with the same bars as the
+    primary rays: hit/miss disagreement <= 0.5 % (FP16, BF16) or <= 5 % (FP8), p99 relative depth
+    error on agreeing hits."""
+    from nbvh_amd import hip
+    prec = {"fp16": nb.PREC_FP16, "bf16": nb.PREC_BF16, "fp8": nb.PREC_FP8}[precision]
+    _, rays = camera_rays(nb, orc, 160, 120)
+    hits = bench["module"].neural_bvh_traversal(rays, precision=nb.PREC_FP32)
+    dr, dh, dn = hip.Buffer.from_array(rays), hip.Buffer.from_array(hits), hip.Buffer(len(rays) * 32)
+    bench["module"].scatter_diffuse_device(dr.ptr, dh.ptr, len(rays), dn.ptr, sample=3, bounce=1)
+    hip.synchronize()
+    bounce = dn.download(nb.RAY_DTYPE)
+    bounce = bounce[bounce["tmax"] > 0]
+    assert len(bounce) > 5000
+    got = bench["module"].neural_bvh_traversal(bounce, precision=prec)
+    exp = bench["net"].neural_query(bench["nbvh"], bounce, mode=orc.FP64)
+    kg, ke = nb.hit_kind(got), nb.hit_kind(exp)
+    assert (kg != ke).mean() <= {"fp16": 0.005, "bf16": 0.005, "fp8": 0.05}[precision], (kg != ke).mean()
+    same = (kg == 1) & (ke == 1) & (got["node_id"] == exp["node_id"])
+    assert same.sum() > 500
+    rel = np.abs(got["t"][same] - exp["t"][same]) / np.maximum(exp["t"][same], 1e-3)
+    assert np.percentile(rel, 99) < {"fp16": 1e-2, "bf16": 3e-2, "fp8": 0.1}[precision], np.percentile(rel, 99)
+
+
 def test_errors_through_abi(nb, ctx):
     scene = nb.Scene(ctx)
     mod = nb.NeuralBVHModule(ctx, scene)
